@@ -1,0 +1,154 @@
+/*
+ * kore_amd -- MI355X (gfx950) solver backend for Kore's solve path.
+ *
+ * C ABI: plain pointers and sizes.  Complex numbers are interleaved
+ * (re, im) doubles, so numpy complex128 arrays pass without copies.  Host
+ * matrices and vectors are in Kore's own (section-major) unknown ordering;
+ * the library permutes them to degree-major order internally.
+ *
+ * Status codes: 0 OK; > 0 numerical warning; < 0 error (message from
+ * kore_last_error(), which is per host thread).
+ *
+ * Each entry point names the reference operation it replaces.  The
+ * reference (Kore, bin/solve.py) drives PETSc/SLEPc/MUMPS through
+ * petsc4py/slepc4py; the equivalent calls are cited as file:line.
+ */
+#ifndef KORE_AMD_H
+#define KORE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KORE_OK 0
+#define KORE_WARN_IR_NOT_CONVERGED 1
+#define KORE_ERR_ARG (-1)
+#define KORE_ERR_HIP (-2)
+#define KORE_ERR_OOM (-3)
+#define KORE_ERR_STATE (-4)
+#define KORE_ERR_SINGULAR (-5)
+#define KORE_ERR_UNSUPPORTED (-6)
+
+#define KORE_PREC_FP64 0   /* complex128 factor */
+#define KORE_PREC_C64 1    /* complex64 factor (FP32 MFMA); FP64 refinement in kore_solve */
+
+#define KORE_TIMER_INV 0
+#define KORE_TIMER_GEMM 1
+#define KORE_TIMER_SKINNY 2
+#define KORE_TIMER_SPMV 3
+#define KORE_TIMER_OTHER 4
+#define KORE_TIMER_COUNT 5
+
+typedef struct kore_ctx kore_ctx;
+typedef struct kore_sys kore_sys;
+typedef struct kore_ws kore_ws;
+
+/* Field content and truncation of a Kore problem: the globals of
+ * bin/parameters.py (m, lmax, symm, hydro, magnetic, thermal,
+ * compositional) and bin/utils.py:26-52 (N1, symmB0). */
+typedef struct {
+  int32_t m, lmax, symm, N1;
+  int32_t hydro, magnetic, thermal, compositional;
+  int32_t symmB0;
+} kore_layout;
+
+typedef struct {
+  int32_t iterations;      /* refinement sweeps performed (1 = plain solve) */
+  int32_t converged;       /* 1 if the refinement met ir_tol (or none was asked) */
+  double backward_error;   /* max over columns of ||Rr|| / (||RM||_F ||x|| + ||Rb||), R = row scaling; -1 if not computed */
+} kore_solve_info;
+
+typedef struct {
+  int32_t levels;          /* reduction levels */
+  int32_t nblk;            /* degree blocks */
+  double flops;            /* algorithmic real flops of the factorisation actually run */
+  int64_t bytes;           /* device bytes held by the factor */
+  double ms;               /* device time, HIP events on the library stream */
+} kore_factor_info;
+
+/* ---- library / context ---------------------------------------------------- */
+int32_t kore_version(void);
+const char* kore_last_error(void);
+
+/* Creates the device context (one HIP stream at default priority).
+ * Replaces: PETSc/SLEPc initialisation, bin/solve.py:14-18, 31-34. */
+int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out);
+void kore_ctx_destroy(kore_ctx* ctx);
+int32_t kore_ctx_sync(kore_ctx* ctx);
+int64_t kore_ctx_bytes_in_use(const kore_ctx* ctx);
+/* Per-kernel-class device timing (HIP events around each launch). */
+int32_t kore_ctx_set_timing(kore_ctx* ctx, int32_t enable);
+int32_t kore_ctx_timing(kore_ctx* ctx, int32_t timer_class, int64_t* count, double* total_ms);
+int32_t kore_ctx_timing_reset(kore_ctx* ctx);
+
+/* ---- layout (host only) ---------------------------------------------------- */
+/* Degree-major permutation and block sizes: new index i holds Kore unknown
+ * perm[i].  Follows bin/utils.py:26-44 and :174-183 (ell) and the section
+ * offsets of bin/assemble.py:613-614, 730-731, 826-827, 925-926, 1022-1023.
+ * blk_sizes/perm may be NULL (query); *nblk and *sizmat are always set. */
+int32_t kore_layout_blocks(const kore_layout* lay, int64_t* sizmat, int32_t* nblk, int32_t* blk_sizes,
+                           int32_t blk_cap, int64_t* perm);
+/* Predicted device bytes of a factorisation (admission control). */
+int64_t kore_factor_bytes(const kore_layout* lay, int32_t precision);
+
+/* ---- system ------------------------------------------------------------------ */
+/* Uploads A (complex CSR) and B (real or complex CSR), both sizmat x sizmat
+ * in Kore's ordering, as read from A.npz / B.npz by ut.load_csr.
+ * Replaces: the MPIAIJ matrix set-up of bin/solve.py:40-60 (A) and 65-86 (B).
+ * B may be NULL (forced problems: b_indptr = NULL). */
+int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat,
+                        const int32_t* a_indptr, const int32_t* a_indices, const double* a_data,
+                        const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
+                        int32_t b_is_complex, kore_sys** out);
+void kore_sys_destroy(kore_sys* sys);
+/* ||A - sigma B||_F of the stored (unscaled) matrices. */
+int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
+
+/* Factors A - sigma*B by block cyclic reduction over the degree blocks.
+ * Replaces: the numeric LU of ST sinvert inside E.solve() (bin/solve.py:123)
+ * and of PC lu inside K.solve() (bin/solve.py:220-227); MUMPS. */
+int32_t kore_factor(kore_sys* sys, double sre, double sim, int32_t precision, kore_factor_info* info);
+/* Drops the factor (device memory is returned). */
+int32_t kore_factor_release(kore_sys* sys);
+
+/* Solves (A - sigma B) x = b for nrhs columns (host arrays, Kore order,
+ * column-major with leading dimensions ldb/ldx).  With a complex64 factor the
+ * solution is refined in FP64 (residuals by FP64 SpMV) until the backward
+ * error <= ir_tol or ir_maxit sweeps; with an FP64 factor ir_maxit extra
+ * sweeps are optional (0 = plain solve).
+ * Replaces: KSPSolve of bin/solve.py:227 (forced problems). */
+int32_t kore_solve(kore_sys* sys, int32_t nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
+                   double ir_tol, int32_t ir_maxit, kore_solve_info* info);
+
+/* ---- Krylov workspace (device, degree-major order) ------------------------- */
+/* A block of ncol device vectors of length sizmat (complex128).
+ * Replaces: SLEPc's BV basis object inside EPS Krylov-Schur. */
+int32_t kore_ws_create(kore_sys* sys, int32_t ncol, kore_ws** out);
+void kore_ws_destroy(kore_ws* ws);
+/* Host <-> device columns, host side in Kore order.
+ * Replaces: VecSetValues / Scatter.toZero gathers (bin/solve.py:143-158). */
+int32_t kore_ws_set(kore_ws* ws, int32_t c0, int32_t nc, const double* host, int64_t ld);
+int32_t kore_ws_get(kore_ws* ws, int32_t c0, int32_t nc, double* host, int64_t ld);
+/* V[:, dst+j] = (A - sigma B)^{-1} B V[:, src+j], j < nc  (the shift-and-invert
+ * operator).  Replaces: STApply of ST sinvert (MatMult(B) + MUMPS solve). */
+int32_t kore_ws_apply_op(kore_ws* ws, int32_t src, int32_t dst, int32_t nc, double ir_tol, int32_t ir_maxit,
+                         kore_solve_info* info);
+/* V[:, dst+j] = (ca A + cb B) V[:, src+j].  Replaces: MatMult (true residuals). */
+int32_t kore_ws_spmv(kore_ws* ws, int32_t src, int32_t dst, int32_t nc, double ca_re, double ca_im,
+                     double cb_re, double cb_im);
+/* H (na x nb, column-major, host) = V[:, a0:a0+na]^H V[:, b0:b0+nb].
+ * Replaces: BVDot / BVDotVec of the Gram-Schmidt orthogonalisation. */
+int32_t kore_ws_dot(kore_ws* ws, int32_t a0, int32_t na, int32_t b0, int32_t nb, double* H);
+/* V[:, d0:d0+nd] = beta V[:, d0:d0+nd] + alpha V[:, s0:s0+ns] Q  (Q host ns x nd,
+ * column-major); destination and source ranges must not overlap.
+ * Replaces: BVMult / BVMultInPlace (orthogonalisation update, restart). */
+int32_t kore_ws_gemm(kore_ws* ws, int32_t d0, int32_t nd, int32_t s0, int32_t ns, const double* Q,
+                     double alpha, double beta);
+/* V[:, dst+j] = V[:, src+j]. */
+int32_t kore_ws_copy(kore_ws* ws, int32_t src, int32_t dst, int32_t nc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,500 @@
+"""Assembly of Kore's sparse A and B (the input side of the solve path).
+
+Restates, for the Boussinesq hydrodynamic case with an inner core, what the
+reference computes in three stages:
+
+1. radial operators ``r^p d^d/dr^d`` in the Gegenbauer basis of each equation
+   (bin/submatrices.py:138-188 and 533-590);
+2. per-degree blocks, linear combinations of those operators with
+   l-dependent coefficients (bin/operators.py:22-194);
+3. global placement in section-major order, boundary-condition rows and the
+   Frobenius normalisation by ||B|| (bin/assemble.py:437-594, 606-813,
+   1160-1171, 1188-1296).
+
+The spectral building blocks are written from the ultraspherical-method
+identities (Olver & Townsend, SIAM Review 55, 2013):
+
+* conversion ``C^(lam) -> C^(lam+1)``:  C^(lam)_k = lam/(lam+k) (C^(lam+1)_k - C^(lam+1)_{k-2})
+  and T_k = (C^(1)_k - C^(1)_{k-2})/2;
+* differentiation: d^lam/dx^lam T_k = 2^(lam-1) (lam-1)! k C^(lam)_{k-lam};
+* multiplication by a(x) = sum_i a_i C^(lam)_i(x): evaluated here as the matrix
+  polynomial sum_i a_i C^(lam)_i(X) of the (truncated, then cropped) Jacobi
+  operator X of multiplication by x, using the three-term recurrence
+  x C_k = [(k+1) C_{k+1} + (k+2lam-1) C_{k-1}] / (2(k+lam)).
+
+Everything is vectorised over degrees: each section's operators are put on a
+common sparsity pattern and the per-degree blocks are one small dense product
+of a coefficient table with that pattern's values.
+"""
+from __future__ import annotations
+
+import math
+from functools import lru_cache
+
+import numpy as np
+import scipy.sparse as ss
+from numpy.polynomial import chebyshev as npcheb
+
+from .params import Params, ell
+
+
+# ---------------------------------------------------------------------------
+# spectral building blocks
+# ---------------------------------------------------------------------------
+
+def cheb_rpow(p: int, N: int, ricb: float) -> np.ndarray:
+    """Chebyshev coefficients (length N) of r^p on the radial domain.
+
+    With an inner core the domain [ricb, 1] is the image of x in [-1, 1]
+    under r = (1+ricb)/2 + (1-ricb)/2 x; without one, r = x.
+    (Kore obtains the same numbers by a DCT of r^p at Gauss points,
+    bin/utils.py:251-269; a polynomial of degree p < N is reproduced exactly.)
+    """
+    if ricb > 0:
+        a, b = (1.0 + ricb) / 2.0, (1.0 - ricb) / 2.0
+    else:
+        a, b = 0.0, 1.0
+    mono = np.array([math.comb(p, j) * a ** (p - j) * b ** j for j in range(p + 1)])
+    c = npcheb.poly2cheb(mono)
+    out = np.zeros(N)
+    out[: len(c)] = c
+    return out
+
+
+def conversion(lam: int, N: int) -> ss.csr_matrix:
+    """S_lam: coefficients in C^(lam) (lam = 0: Chebyshev T) -> C^(lam+1)."""
+    k = np.arange(N, dtype=float)
+    if lam == 0:
+        d0 = np.full(N, 0.5)
+        d0[0] = 1.0
+        d2 = np.full(max(N - 2, 0), -0.5)
+    else:
+        d0 = lam / (lam + k)
+        d2 = -lam / (lam + k[2:])
+    return ss.diags([d0, d2], [0, 2], shape=(N, N), format="csr")
+
+
+def derivative(lam: int, N: int, ricb: float) -> ss.csr_matrix:
+    """D_lam: lam-th radial derivative, Chebyshev T -> C^(lam) coefficients."""
+    scale = (2.0 / (1.0 - ricb)) ** lam if ricb > 0 else 1.0
+    c = scale * math.factorial(lam - 1) * 2.0 ** (lam - 1)
+    k = np.arange(lam, N, dtype=float)
+    return ss.diags([c * k], [lam], shape=(N, N), format="csr")
+
+
+def jacobi_x(lam: int, K: int) -> ss.csr_matrix:
+    """Multiplication by x in the C^(lam) basis, truncated to K x K."""
+    k = np.arange(K, dtype=float)
+    if lam == 0:
+        sub = np.full(K - 1, 0.5)
+        sub[0] = 1.0                      # x T_0 = T_1
+        sup = np.full(K - 1, 0.5)         # x T_k = (T_{k+1} + T_{k-1})/2
+    else:
+        sub = (k[:-1] + 1) / (2 * (k[:-1] + lam))          # X[k+1, k]
+        sup = (k[1:] + 2 * lam - 1) / (2 * (k[1:] + lam))  # X[k-1, k]
+    return ss.diags([sub, sup], [-1, 1], shape=(K, K), format="csr")
+
+
+def multiplication(a: np.ndarray, lam: int) -> ss.csr_matrix:
+    """M_lam[a]: multiplication by sum_i a_i C^(lam)_i(x), N x N, N = len(a)."""
+    N = len(a)
+    nz = np.nonzero(a)[0]
+    if len(nz) == 0:
+        return ss.csr_matrix((N, N))
+    deg = int(nz[-1])
+    K = N + deg + 2                      # crop after building: exact for rows/cols < N
+    X = jacobi_x(lam, K)
+    I = ss.identity(K, format="csr")
+    c_prev = I
+    acc = a[0] * I
+    if deg >= 1:
+        c_cur = X if lam == 0 else (2.0 * lam) * X
+        acc = acc + a[1] * c_cur
+        for i in range(1, deg):
+            if lam == 0:
+                c_next = 2.0 * (X @ c_cur) - c_prev
+            else:
+                c_next = (2.0 * (i + lam) * (X @ c_cur) - (i + 2 * lam - 1) * c_prev) / (i + 1)
+            c_prev, c_cur = c_cur, c_next
+            if a[i + 1] != 0:
+                acc = acc + a[i + 1] * c_cur
+    return ss.csr_matrix(acc[:N, :N])
+
+
+class RadialOps:
+    """Radial operators ``r^p D^d`` expressed in the C^(g) basis of a section.
+
+    ``op(p, d, g)`` equals Kore's ``rP_Dd_<section>`` (bin/submatrices.py:545,
+    582-588): conversion C^(d) -> C^(g), times multiplication by r^p in
+    C^(d), times the d-th derivative, with the top ``g`` rows emptied to make
+    room for the boundary conditions (inner-core case).
+    """
+
+    def __init__(self, N: int, ricb: float):
+        if ricb <= 0:
+            raise NotImplementedError("assembly without an inner core (ricb = 0) is not supported")
+        self.N, self.ricb = N, ricb
+        self._S = [conversion(l, N) for l in range(4)]
+
+    def convert(self, frm: int, to: int):
+        out = ss.identity(self.N, format="csr")
+        for lam in range(frm, to):
+            out = self._S[lam] @ out
+        return out
+
+    @lru_cache(maxsize=None)
+    def op(self, p: int, d: int, g: int) -> ss.csr_matrix:
+        N = self.N
+        a = self.convert(0, d) @ cheb_rpow(p, N, self.ricb)
+        a[np.abs(a) < 1e-15 * max(1.0, np.abs(a).max())] = 0.0
+        M = multiplication(a, d)
+        mat = self.convert(d, g) @ M
+        if d > 0:
+            mat = mat @ derivative(d, N, self.ricb)
+        mat = ss.csr_matrix(mat)
+        chop = g
+        if chop > 0:
+            mat = ss.vstack([ss.csr_matrix((chop, N)), mat[:-chop, :]], format="csr")
+        mat.eliminate_zeros()
+        return mat
+
+
+def boundary_values(x: float, N: int, nder: int, ricb: float) -> np.ndarray:
+    """T_k^{(i)} at x = +-1 w.r.t. r, k = 0..N-1, i = 0..nder  (shape N x (nder+1)).
+
+    Closed form: T_k^{(i)}(+-1) = (+-1)^(k+i) prod_{t<i} (k^2 - t^2)/(2t+1),
+    times (dx/dr)^i = (2/(1-ricb))^i (bin/bc_variables.py:9-32 tabulates the
+    same values).
+    """
+    k = np.arange(N, dtype=float)
+    out = np.zeros((N, nder + 1))
+    dxdr = 2.0 / (1.0 - ricb) if ricb > 0 else 1.0
+    prod = np.ones(N)
+    for i in range(nder + 1):
+        if i > 0:
+            prod = prod * (k ** 2 - (i - 1) ** 2) / (2 * (i - 1) + 1)
+        sign = np.where(((k + i) % 2 == 1) & (x < 0), -1.0, 1.0)
+        out[:, i] = sign * prod * dxdr ** i
+    return out
+
+
+# ---------------------------------------------------------------------------
+# per-degree coefficient tables  (bin/operators.py:22-194)
+# ---------------------------------------------------------------------------
+# Each block is  sum_j coef_j(l) * op_j ; ops are named (p, d) in the section
+# basis (g = 4 for the poloidal "u" section, g = 2 for the toroidal "v").
+
+U_OPS = [(0, 0), (2, 0), (3, 1), (4, 2), (2, 2), (3, 3), (4, 4)]   # diagonal, section u
+U_OFF = [(3, 0), (4, 1)]                                          # u <- utor(l+-1)
+V_OPS = [(0, 0), (2, 0), (1, 1), (2, 2)]                          # diagonal, section v
+V_OFF = [(1, 0), (2, 1)]                                          # v <- upol(l+-1)
+
+
+def _c_minus(l, m):
+    return (l ** 2 - 1) * np.sqrt(l ** 2 - m ** 2) / (2 * l - 1.0)
+
+
+def _c_plus(l, m):
+    return l * (l + 2.0) * np.sqrt((l + m + 1.0) * (l - m + 1)) / (2.0 * l + 3.0)
+
+
+def _L(l):
+    l = np.asarray(l, dtype=float)
+    return l, l * (l + 1)
+
+
+# Individual terms over U_OPS = [r0D0, r2D0, r3D1, r4D2, r2D2, r3D3, r4D4]
+def u_term_u(l):
+    """u(l, 'u', 'upol'):  L (L r2D0 - 2 r3D1 - r4D2)   (bin/operators.py:28-35)."""
+    l, L = _L(l)
+    z = 0 * L
+    return np.stack([z, L * L, -2 * L, -L, z, z, z], axis=-1).astype(complex)
+
+
+def u_term_coriolis(l, par: Params):
+    """coriolis(l, 'u', 'upol', 0):  2 i m (-L r2D0 + 2 r3D1 + r4D2) Gaspard   (bin/operators.py:58-63)."""
+    l, L = _L(l)
+    z = 0 * L
+    c = 2j * par.m * par.Gaspard
+    return np.stack([z, -c * L, 2 * c + z, c + z, z, z, z], axis=-1).astype(complex)
+
+
+def u_term_viscous(l, par: Params):
+    """viscous_diffusion(l, 'u', 'upol', 0):
+    Ek L (-L (l+2)(l-1) r0D0 + 2 L r2D2 - 4 r3D3 - r4D4)   (bin/operators.py:169-174)."""
+    l, L = _L(l)
+    z = 0 * L
+    V = par.ViscosD
+    return np.stack([-V * L * L * (l + 2) * (l - 1), z, z, z, 2 * V * L * L, -4 * V * L, -V * L],
+                    axis=-1).astype(complex)
+
+
+def u_diag_coefs(l, par: Params, wf: float):
+    """A block of the 2curl (poloidal) equation: i w u + Coriolis - viscous
+    (bin/assemble.py:623-626)."""
+    return 1j * wf * u_term_u(l) + u_term_coriolis(l, par) - u_term_viscous(l, par)
+
+
+def u_mass_coefs(l):
+    """-u(l) of the 2curl equation (the B block, bin/assemble.py:445)."""
+    return -u_term_u(l)
+
+
+def u_off_coefs(l, par: Params, side: int):
+    """coriolis(l, 'u', 'utor', +-1): poloidal degree l <- toroidal l+side
+    (bin/operators.py:65-87), over U_OFF = [r3D0, r4D1]."""
+    l = np.asarray(l, dtype=float)
+    G, m = par.Gaspard, par.m
+    if side < 0:
+        C = _c_minus(l, m)
+        return np.stack([2 * G * C * (l - 1), -2 * G * C], axis=-1).astype(complex)
+    C = _c_plus(l, m)
+    return np.stack([-2 * G * C * (l + 2), -2 * G * C], axis=-1).astype(complex)
+
+
+# Individual terms over V_OPS = [r0D0, r2D0, r1D1, r2D2]
+def v_term_u(l):
+    """u(l, 'v', 'utor'):  L r2D0   (bin/operators.py:37-42)."""
+    l, L = _L(l)
+    z = 0 * L
+    return np.stack([z, L, z, z], axis=-1).astype(complex)
+
+
+def v_term_coriolis(l, par: Params):
+    """coriolis(l, 'v', 'utor', 0):  -2 i m r2D0 Gaspard   (bin/operators.py:115-122)."""
+    l, L = _L(l)
+    z = 0 * L
+    return np.stack([z, -2j * par.m * par.Gaspard + z, z, z], axis=-1).astype(complex)
+
+
+def v_term_viscous(l, par: Params):
+    """viscous_diffusion(l, 'v', 'utor', 0):  Ek L (-L r0D0 + 2 r1D1 + r2D2)
+    (bin/operators.py:188-192)."""
+    l, L = _L(l)
+    V = par.ViscosD
+    return np.stack([-V * L * L, 0 * L, 2 * V * L, V * L], axis=-1).astype(complex)
+
+
+def v_diag_coefs(l, par: Params, wf: float):
+    """A block of the 1curl (toroidal) equation (bin/assemble.py:756-759)."""
+    return 1j * wf * v_term_u(l) + v_term_coriolis(l, par) - v_term_viscous(l, par)
+
+
+def v_mass_coefs(l):
+    return -v_term_u(l)
+
+
+def v_off_coefs(l, par: Params, side: int):
+    """coriolis(l, 'v', 'upol', +-1): toroidal degree l <- poloidal l+side
+    (bin/operators.py:91-113), over V_OFF = [r1D0, r2D1]."""
+    l = np.asarray(l, dtype=float)
+    G, m = par.Gaspard, par.m
+    if side < 0:
+        C = _c_minus(l, m)
+        return np.stack([2 * G * C * (l - 1), -2 * G * C], axis=-1).astype(complex)
+    C = _c_plus(l, m)
+    return np.stack([-2 * G * C * (l + 2), -2 * G * C], axis=-1).astype(complex)
+
+
+def degree_block(par: Params, R: "RadialOps", term: str, l: int):
+    """One per-degree operator block, by term name:
+    'u:u', 'cori:u', 'visc:u', 'cori:u:-1', 'cori:u:+1', 'u:v', 'cori:v', 'visc:v',
+    'cori:v:-1', 'cori:v:+1'.  Mirrors the functions of bin/operators.py:22-194."""
+    tabs = {
+        "u:u": (U_OPS, 4, lambda: u_term_u([l])),
+        "cori:u": (U_OPS, 4, lambda: u_term_coriolis([l], par)),
+        "visc:u": (U_OPS, 4, lambda: u_term_viscous([l], par)),
+        "cori:u:-1": (U_OFF, 4, lambda: u_off_coefs([l], par, -1)),
+        "cori:u:+1": (U_OFF, 4, lambda: u_off_coefs([l], par, +1)),
+        "u:v": (V_OPS, 2, lambda: v_term_u([l])),
+        "cori:v": (V_OPS, 2, lambda: v_term_coriolis([l], par)),
+        "visc:v": (V_OPS, 2, lambda: v_term_viscous([l], par)),
+        "cori:v:-1": (V_OFF, 2, lambda: v_off_coefs([l], par, -1)),
+        "cori:v:+1": (V_OFF, 2, lambda: v_off_coefs([l], par, +1)),
+    }
+    ops, g, coef = tabs[term]
+    c = coef()[0]
+    out = None
+    for cj, (p, d) in zip(c, ops):
+        if cj == 0:
+            continue
+        t = cj * R.op(p, d, g)
+        out = t if out is None else out + t
+    return ss.csr_matrix(out) if out is not None else ss.csr_matrix((par.N1, par.N1))
+
+
+# ---------------------------------------------------------------------------
+# vectorised placement
+# ---------------------------------------------------------------------------
+
+class _Pattern:
+    """A set of operators of one section on their union sparsity pattern."""
+
+    def __init__(self, mats):
+        coo = [m.tocoo() for m in mats]
+        rows = np.concatenate([c.row for c in coo])
+        cols = np.concatenate([c.col for c in coo])
+        N = mats[0].shape[1]
+        key = np.unique(rows.astype(np.int64) * N + cols)
+        self.row = (key // N).astype(np.int64)
+        self.col = (key % N).astype(np.int64)
+        self.vals = np.zeros((len(mats), len(key)))
+        for j, c in enumerate(coo):
+            pos = np.searchsorted(key, c.row.astype(np.int64) * N + c.col)
+            np.add.at(self.vals[j], pos, c.data)
+
+    def blocks(self, coefs):
+        """coefs: (nblk, nops) -> values (nblk, nnz)."""
+        return coefs @ self.vals
+
+
+def _triplets(values, pat, row_off, col_off):
+    nblk = values.shape[0]
+    r = (row_off[:, None] + pat.row[None, :]).ravel()
+    c = (col_off[:, None] + pat.col[None, :]).ravel()
+    return values.ravel(), r, c
+
+
+def _bc_rows(par: Params, Tb, Ta, section: str, l):
+    """Dense boundary-condition rows for one section (bin/assemble.py:1188-1296)."""
+    rows = []
+    if section == "u":
+        if par.bco == 1:
+            rows += [Tb[:, 0], Tb[:, 1]]                  # P = 0, P' = 0 at the CMB
+        else:
+            rows += [Tb[:, 0], Tb[:, 2]]                  # P = 0, r P'' = 0 (rcmb = 1)
+        if par.bci == 1:
+            rows += [Ta[:, 0], Ta[:, 1]]
+        else:
+            rows += [Ta[:, 0], par.ricb * Ta[:, 2]]
+    else:
+        rows += [Tb[:, 0] if par.bco == 1 else (-Tb[:, 1] + Tb[:, 0])]
+        rows += [Ta[:, 0] if par.bci == 1 else (-par.ricb * Ta[:, 1] + Ta[:, 0])]
+    return np.array(rows)
+
+
+def assemble(par: Params, which: str = "eigen"):
+    """Build Kore's matrices in the reference's section-major ordering.
+
+    which = "eigen":  returns (A, B, Bnorm) with A, B divided by ||B||_F,
+                      B real (bin/assemble.py:578-591, 1160-1171).
+    which = "forced": returns A(omega) (not normalised) with omega =
+                      forcing_frequency (bin/assemble.py:623, 756).
+    which = "unscaled": returns (A0, B0), the un-normalised operator at
+                      omega = 0 and the un-normalised mass matrix, so that
+                      A(omega) = A0 - i omega B0 (the identity of
+                      bin/assemble.py:445/623 and 461/756).
+    """
+    if not par.hydro or par.magnetic or par.thermal or par.compositional:
+        raise NotImplementedError("only the hydrodynamic assembly is implemented")
+    if par.anelastic:
+        raise NotImplementedError("anelastic assembly is not implemented")
+    if par.Ek == 0:
+        raise NotImplementedError("inviscid assembly is not implemented")
+    N, N1, nb, n = par.N, par.N1, par.nb, par.n
+    R = RadialOps(N, par.ricb)
+    top, bot, _ = ell(par.m, par.lmax, par.symm)
+    pos_top = {int(l): k for k, l in enumerate(top)}
+    pos_bot = {int(l): k for k, l in enumerate(bot)}
+    wf = par.wf if which == "forced" else 0.0
+
+    pu = _Pattern([R.op(p, d, 4) for (p, d) in U_OPS])
+    puo = _Pattern([R.op(p, d, 4) for (p, d) in U_OFF])
+    pv = _Pattern([R.op(p, d, 2) for (p, d) in V_OPS])
+    pvo = _Pattern([R.op(p, d, 2) for (p, d) in V_OFF])
+
+    kt = np.arange(nb)
+    rows_u = kt * N1
+    rows_v = n + kt * N1
+
+    dat, ri, ci = [], [], []
+
+    def add(t):
+        dat.append(t[0]); ri.append(t[1]); ci.append(t[2])
+
+    # section u: diagonal blocks
+    add(_triplets(pu.blocks(u_diag_coefs(top, par, wf)), pu, rows_u, rows_u))
+    # section u <- utor(l-1), utor(l+1)
+    for side in (-1, 1):
+        sel = np.array([int(l) + side in pos_bot for l in top])
+        if sel.any():
+            ls = top[sel]
+            cols = n + np.array([pos_bot[int(l) + side] for l in ls]) * N1
+            add(_triplets(puo.blocks(u_off_coefs(ls, par, side)), puo, rows_u[sel], cols))
+    # section v: diagonal blocks
+    add(_triplets(pv.blocks(v_diag_coefs(bot, par, wf)), pv, rows_v, rows_v))
+    for side in (-1, 1):
+        sel = np.array([int(l) + side in pos_top for l in bot])
+        if sel.any():
+            ls = bot[sel]
+            cols = np.array([pos_top[int(l) + side] for l in ls]) * N1
+            add(_triplets(pvo.blocks(v_off_coefs(ls, par, side)), pvo, rows_v[sel], cols))
+    # boundary-condition rows (top rows of every diagonal block)
+    Tb = boundary_values(1.0, N, 5, par.ricb)
+    Ta = boundary_values(-1.0, N, 4, par.ricb)
+    for sec, r0, ls in (("u", rows_u, top), ("v", rows_v, bot)):
+        bc = _bc_rows(par, Tb, Ta, sec, None)            # (nrows, N1), same for every l
+        nr = bc.shape[0]
+        rr = (r0[:, None, None] + np.arange(nr)[None, :, None] + 0 * np.arange(N1)[None, None, :])
+        cc = (r0[:, None, None] + 0 * np.arange(nr)[None, :, None] + np.arange(N1)[None, None, :])
+        vv = np.broadcast_to(bc[None].astype(complex), rr.shape)
+        add((vv.ravel(), rr.ravel(), cc.ravel()))
+
+    S = par.sizmat
+    A = ss.csr_matrix((np.concatenate(dat), (np.concatenate(ri), np.concatenate(ci))),
+                      shape=(S, S), dtype=complex)
+    A.eliminate_zeros()
+    A.sort_indices()
+
+    if which == "forced":
+        return A
+
+    bdat, bri, bci = [], [], []
+    t = _triplets(pu.blocks(u_mass_coefs(top)).real, pu, rows_u, rows_u)
+    bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
+    t = _triplets(pv.blocks(v_mass_coefs(bot)).real, pv, rows_v, rows_v)
+    bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
+    B = ss.csr_matrix((np.concatenate(bdat), (np.concatenate(bri), np.concatenate(bci))),
+                      shape=(S, S), dtype=float)
+    B.eliminate_zeros()
+    B.sort_indices()
+    if which == "unscaled":
+        return A, B
+    Bnorm = float(np.sqrt(np.sum(B.data ** 2)))
+    return (A / Bnorm).tocsr(), (B / Bnorm).tocsr(), Bnorm
+
+
+def forcing_vector(par: Params) -> ss.csr_matrix:
+    """B_forced column for the boundary-flow forcings (bin/assemble.py:280-330).
+
+    Implemented: forcing = 7 (longitudinal libration, no-slip, symm = 1).
+    """
+    if par.forcing != 7:
+        raise NotImplementedError("forcing %d is not implemented" % par.forcing)
+    if not (par.symm == 1 and par.bci == 1 and par.bco == 1 and par.m in (0, 2)):
+        raise ValueError("longitudinal libration requires m = 0 or 2, symm = 1 and no-slip boundaries")
+    top, bot, _ = ell(par.m, par.lmax, par.symm)
+    w = par.forcing_frequency
+    if par.m == 0:
+        pos = par.n + par.N1 * int(np.where(bot == 1)[0][0])
+        data = np.array([1j * w * par.forcing_amplitude_cmb / 2,
+                         1j * w * par.forcing_amplitude_icb * par.ricb / 2])
+    else:
+        L = 6.0
+        pos = par.N1 * int(np.where(top == 2)[0][0])
+        data = np.array([-w / L * par.forcing_amplitude_cmb,
+                         -w / L * par.forcing_amplitude_icb * par.ricb ** 2], dtype=complex)
+    rows = np.array([pos, pos + 1])
+    return ss.csr_matrix((data, (rows, np.zeros(2, dtype=int))), shape=(par.sizmat, 1))
+
+
+def save_npz_csr(path: str, M) -> None:
+    """Write a CSR matrix in Kore's uncompressed npz layout (bin/assemble.py:591)."""
+    M = ss.csr_matrix(M)
+    np.savez(path, data=M.data, indices=M.indices, indptr=M.indptr, shape=M.shape)
+
+
+def load_csr(path: str) -> ss.csr_matrix:
+    """Kore's loader (bin/utils.py:1253-1256)."""
+    z = np.load(path)
+    return ss.csr_matrix((z["data"], z["indices"], z["indptr"]), shape=tuple(z["shape"]))

@@ -1,0 +1,336 @@
+"""ctypes binding of the HIP library (include/kore_amd.h).
+
+There is no fallback: if ``libkore_amd.so`` is missing or cannot be loaded,
+``lib()`` raises ``KoreLibraryError``.  Build it with
+``python -m kore_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+import weakref
+
+import numpy as np
+import scipy.sparse as ss
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.environ.get("KORE_AMD_LIB", os.path.join(HERE, "libkore_amd.so"))
+
+KORE_OK = 0
+KORE_WARN_IR_NOT_CONVERGED = 1
+PREC_FP64 = 0
+PREC_C64 = 1
+TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_OTHER = range(5)
+TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "other"]
+
+
+class KoreLibraryError(RuntimeError):
+    pass
+
+
+class KoreError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("kore_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in
+                ("m", "lmax", "symm", "N1", "hydro", "magnetic", "thermal", "compositional", "symmB0")]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("backward_error", C.c_double)]
+
+
+class FactorInfo(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("nblk", C.c_int32), ("flops", C.c_double),
+                ("bytes", C.c_int64), ("ms", C.c_double)]
+
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_D = C.c_double
+_PI32 = C.POINTER(C.c_int32)
+_PI64 = C.POINTER(C.c_int64)
+_PD = C.POINTER(C.c_double)
+
+# (name, restype, argtypes) for every symbol of include/kore_amd.h
+SIGNATURES = [
+    ("kore_version", _I32, []),
+    ("kore_last_error", C.c_char_p, []),
+    ("kore_ctx_create", _I32, [_I32, _I64, C.POINTER(_P)]),
+    ("kore_ctx_destroy", None, [_P]),
+    ("kore_ctx_sync", _I32, [_P]),
+    ("kore_ctx_bytes_in_use", _I64, [_P]),
+    ("kore_ctx_set_timing", _I32, [_P, _I32]),
+    ("kore_ctx_timing", _I32, [_P, _I32, _PI64, _PD]),
+    ("kore_ctx_timing_reset", _I32, [_P]),
+    ("kore_layout_blocks", _I32, [C.POINTER(Layout), _PI64, _PI32, _PI32, _I32, _PI64]),
+    ("kore_factor_bytes", _I64, [C.POINTER(Layout), _I32]),
+    ("kore_sys_create", _I32, [_P, C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PI32, _PI32, _PD, _I32,
+                               C.POINTER(_P)]),
+    ("kore_sys_destroy", None, [_P]),
+    ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
+    ("kore_factor", _I32, [_P, _D, _D, _I32, C.POINTER(FactorInfo)]),
+    ("kore_factor_release", _I32, [_P]),
+    ("kore_solve", _I32, [_P, _I32, _PD, _I64, _PD, _I64, _D, _I32, C.POINTER(SolveInfo)]),
+    ("kore_ws_create", _I32, [_P, _I32, C.POINTER(_P)]),
+    ("kore_ws_destroy", None, [_P]),
+    ("kore_ws_set", _I32, [_P, _I32, _I32, _PD, _I64]),
+    ("kore_ws_get", _I32, [_P, _I32, _I32, _PD, _I64]),
+    ("kore_ws_apply_op", _I32, [_P, _I32, _I32, _I32, _D, _I32, C.POINTER(SolveInfo)]),
+    ("kore_ws_spmv", _I32, [_P, _I32, _I32, _I32, _D, _D, _D, _D]),
+    ("kore_ws_dot", _I32, [_P, _I32, _I32, _I32, _I32, _PD]),
+    ("kore_ws_gemm", _I32, [_P, _I32, _I32, _I32, _I32, _PD, _D, _D]),
+    ("kore_ws_copy", _I32, [_P, _I32, _I32, _I32]),
+]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load the HIP library (once).  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIBPATH):
+            raise KoreLibraryError("HIP library %s not found; build it with `python -m kore_amd.build`" % LIBPATH)
+        try:
+            L = C.CDLL(LIBPATH)
+        except OSError as e:
+            raise KoreLibraryError("cannot load %s: %s" % (LIBPATH, e))
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return L
+
+
+def _check(rc, allow_warn=True):
+    if rc < 0 or (rc > 0 and not allow_warn):
+        msg = lib().kore_last_error()
+        raise KoreError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_PD)
+
+
+def make_layout(m, lmax, symm, N1, hydro=1, magnetic=0, thermal=0, compositional=0, symmB0=-1) -> Layout:
+    return Layout(int(m), int(lmax), int(symm), int(N1), int(hydro), int(magnetic), int(thermal),
+                  int(compositional), int(symmB0))
+
+
+def layout_from_params(par) -> Layout:
+    return make_layout(par.m, par.lmax, par.symm, par.N1, par.hydro, par.magnetic, par.thermal,
+                       par.compositional, par.symmB0 if par.magnetic else -1)
+
+
+def layout_blocks(lay: Layout):
+    """(sizmat, block sizes, permutation) computed by the library (host only)."""
+    L = lib()
+    n = C.c_int64(0)
+    nb = C.c_int32(0)
+    _check(L.kore_layout_blocks(C.byref(lay), C.byref(n), C.byref(nb), None, 0, None))
+    sizes = np.zeros(nb.value, dtype=np.int32)
+    perm = np.zeros(n.value, dtype=np.int64)
+    _check(L.kore_layout_blocks(C.byref(lay), C.byref(n), C.byref(nb), sizes.ctypes.data_as(_PI32), nb.value,
+                                perm.ctypes.data_as(_PI64)))
+    return n.value, sizes, perm
+
+
+def factor_bytes(lay: Layout, precision: int) -> int:
+    return int(lib().kore_factor_bytes(C.byref(lay), int(precision)))
+
+
+class Context:
+    """A device context: one HIP stream, a memory budget, kernel timers."""
+
+    def __init__(self, device: int = 0, mem_budget: int = 0):
+        L = lib()
+        p = _P()
+        _check(L.kore_ctx_create(int(device), int(mem_budget), C.byref(p)))
+        self._p = p
+        self._children = weakref.WeakSet()
+
+    @property
+    def handle(self):
+        return self._p
+
+    def sync(self):
+        _check(lib().kore_ctx_sync(self._p))
+
+    def bytes_in_use(self) -> int:
+        return int(lib().kore_ctx_bytes_in_use(self._p))
+
+    def set_timing(self, on: bool):
+        _check(lib().kore_ctx_set_timing(self._p, 1 if on else 0))
+
+    def timing(self):
+        out = {}
+        for k, name in enumerate(TIMER_NAMES):
+            cnt = C.c_int64(0)
+            ms = C.c_double(0)
+            _check(lib().kore_ctx_timing(self._p, k, C.byref(cnt), C.byref(ms)))
+            out[name] = (cnt.value, ms.value)
+        return out
+
+    def timing_reset(self):
+        _check(lib().kore_ctx_timing_reset(self._p))
+
+    def close(self):
+        if getattr(self, "_p", None):
+            for ch in list(self._children):
+                ch.close()
+            lib().kore_ctx_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class System:
+    """A + sigma B system resident on the device (Kore ordering on the host)."""
+
+    def __init__(self, ctx: Context, lay: Layout, A, B=None):
+        L = lib()
+        A = ss.csr_matrix(A)
+        n = A.shape[0]
+        self.n = n
+        self.ctx = ctx
+        self.lay = lay
+        self._a = (A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                   np.ascontiguousarray(A.data.astype(np.complex128)))
+        args_b = [None, None, None, 0]
+        if B is not None:
+            B = ss.csr_matrix(B)
+            bc = np.iscomplexobj(B.data)
+            bdata = np.ascontiguousarray(B.data.astype(np.complex128 if bc else np.float64))
+            self._b = (B.indptr.astype(np.int32), B.indices.astype(np.int32), bdata)
+            args_b = [self._b[0].ctypes.data_as(_PI32), self._b[1].ctypes.data_as(_PI32),
+                      self._b[2].ctypes.data_as(_PD), 1 if bc else 0]
+        p = _P()
+        _check(L.kore_sys_create(ctx.handle, C.byref(lay), n, self._a[0].ctypes.data_as(_PI32),
+                                 self._a[1].ctypes.data_as(_PI32), _dptr(self._a[2]),
+                                 args_b[0], args_b[1], args_b[2], args_b[3], C.byref(p)))
+        self._p = p
+        self._children = weakref.WeakSet()
+        ctx._children.add(self)
+        self.sigma = None
+        self.precision = None
+        self.last_factor = None
+
+    @property
+    def handle(self):
+        return self._p
+
+    def norm(self, sigma=0j) -> float:
+        out = C.c_double(0)
+        _check(lib().kore_sys_norm(self._p, complex(sigma).real, complex(sigma).imag, C.byref(out)))
+        return out.value
+
+    def factor(self, sigma, precision=PREC_FP64) -> FactorInfo:
+        info = FactorInfo()
+        sigma = complex(sigma)
+        _check(lib().kore_factor(self._p, sigma.real, sigma.imag, int(precision), C.byref(info)))
+        self.sigma = sigma
+        self.precision = precision
+        self.last_factor = info
+        return info
+
+    def release_factor(self):
+        _check(lib().kore_factor_release(self._p))
+
+    def solve(self, b, ir_tol=1e-14, ir_maxit=0):
+        """x = (A - sigma B)^{-1} b for b of shape (n,) or (n, nrhs) (Kore order)."""
+        b = np.asarray(b, dtype=np.complex128)
+        one = b.ndim == 1
+        bb = np.asfortranarray(b.reshape(self.n, -1))
+        x = np.zeros_like(bb, order="F")
+        info = SolveInfo()
+        rc = lib().kore_solve(self._p, bb.shape[1], _dptr(bb), self.n,
+                              _dptr(x), self.n, float(ir_tol), int(ir_maxit), C.byref(info))
+        _check(rc)
+        return (x[:, 0] if one else x), info
+
+    def close(self):
+        if getattr(self, "_p", None):
+            for ch in list(self._children):
+                ch.close()
+            lib().kore_sys_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Workspace:
+    """ncol device vectors (degree-major order) for the Krylov iteration."""
+
+    def __init__(self, sysm: System, ncol: int):
+        self.sys = sysm
+        self.n = sysm.n
+        self.ncol = ncol
+        p = _P()
+        _check(lib().kore_ws_create(sysm.handle, int(ncol), C.byref(p)))
+        self._p = p
+        sysm._children.add(self)
+
+    def set(self, c0, X):
+        X = np.asfortranarray(np.asarray(X, dtype=np.complex128).reshape(self.n, -1))
+        _check(lib().kore_ws_set(self._p, int(c0), X.shape[1], _dptr(X), self.n))
+
+    def get(self, c0, nc):
+        X = np.zeros((self.n, nc), dtype=np.complex128, order="F")
+        _check(lib().kore_ws_get(self._p, int(c0), int(nc), _dptr(X), self.n))
+        return X
+
+    def apply_op(self, src, dst, nc, ir_tol=1e-14, ir_maxit=0):
+        info = SolveInfo()
+        rc = lib().kore_ws_apply_op(self._p, int(src), int(dst), int(nc), float(ir_tol), int(ir_maxit),
+                                    C.byref(info))
+        _check(rc)
+        return info
+
+    def spmv(self, src, dst, nc, ca, cb):
+        ca, cb = complex(ca), complex(cb)
+        _check(lib().kore_ws_spmv(self._p, int(src), int(dst), int(nc), ca.real, ca.imag, cb.real, cb.imag))
+
+    def dot(self, a0, na, b0, nb):
+        H = np.zeros((na, nb), dtype=np.complex128, order="F")
+        _check(lib().kore_ws_dot(self._p, int(a0), int(na), int(b0), int(nb), _dptr(H)))
+        return H
+
+    def gemm(self, d0, nd, s0, ns, Q, alpha=1.0, beta=0.0):
+        Q = np.asfortranarray(np.asarray(Q, dtype=np.complex128).reshape(ns, nd))
+        _check(lib().kore_ws_gemm(self._p, int(d0), int(nd), int(s0), int(ns), _dptr(Q),
+                                  float(alpha), float(beta)))
+
+    def copy(self, src, dst, nc):
+        _check(lib().kore_ws_copy(self._p, int(src), int(dst), int(nc)))
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().kore_ws_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

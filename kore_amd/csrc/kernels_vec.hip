@@ -1,0 +1,412 @@
+// Bandwidth-bound kernels of the Kore solve path (gfx950).
+//
+//   spmv          (ca A + cb B) X on the union CSR pattern, 16 lanes per row
+//   row_scale     power-of-two row equilibration of A - sigma B
+//   build_blocks  scatter R(A - sigma B) into the dense level-0 blocks
+//   skinny        batched Y = beta Y + alpha (A1 X1 + A2 X2), nrhs <= 16,
+//                 the block sweeps of the cyclic-reduction solve
+//   gather/scatter/axpy/norms/zero/copy   vector plumbing
+#include "kore_device.hpp"
+#include "kore_kernels.hpp"
+
+namespace kore {
+
+// ---------------------------------------------------------------------------
+// CSR SpMV, 16 lanes per row
+// ---------------------------------------------------------------------------
+template <int NR>
+__global__ __launch_bounds__(256) void spmv_kernel(CsrDev m, cx<double> ca, cx<double> cb,
+                                                   const cx<double>* __restrict__ X, int64_t ldx,
+                                                   cx<double>* __restrict__ Y, int64_t ldy, int nrhs,
+                                                   double beta) {
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + g;
+  if (row >= m.n) return;
+  const int s = m.indptr[row], e = m.indptr[row + 1];
+  cx<double> acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = czero<double>();
+  for (int k = s + gl; k < e; k += 16) {
+    const int j = m.indices[k];
+    const cx<double> v = cadd(cmul(ca, m.va[k]), cmul(cb, m.vb[k]));
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) acc[c] = cfma(acc[c], v, X[(int64_t)c * ldx + j]);
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) {
+      acc[c].re += __shfl_xor(acc[c].re, off, 64);
+      acc[c].im += __shfl_xor(acc[c].im, off, 64);
+    }
+  }
+  if (gl == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (c < nrhs) {
+        cx<double>* y = Y + (int64_t)c * ldy + row;
+        cx<double> v = acc[c];
+        if (beta != 0.0) {
+          const cx<double> o = *y;
+          v.re += beta * o.re;
+          v.im += beta * o.im;
+        }
+        *y = v;
+      }
+    }
+  }
+}
+
+hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re, double cb_im,
+                       const cx<double>* X, int64_t ldx, cx<double>* Y, int64_t ldy, int nrhs,
+                       double beta, hipStream_t s) {
+  if (m.n == 0) return hipSuccess;
+  const cx<double> ca{ca_re, ca_im}, cb{cb_re, cb_im};
+  const dim3 grid((unsigned)((m.n + 15) / 16));
+  for (int c0 = 0; c0 < nrhs; c0 += 16) {
+    const int nr = nrhs - c0 < 16 ? nrhs - c0 : 16;
+    const cx<double>* x = X + (int64_t)c0 * ldx;
+    cx<double>* y = Y + (int64_t)c0 * ldy;
+    if (nr == 1)
+      hipLaunchKernelGGL(spmv_kernel<1>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+    else if (nr <= 4)
+      hipLaunchKernelGGL(spmv_kernel<4>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+    else
+      hipLaunchKernelGGL(spmv_kernel<16>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// row equilibration
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void row_scale_kernel(CsrDev m, cx<double> cs, double* __restrict__ scale) {
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + g;
+  if (row >= m.n) return;
+  const int s = m.indptr[row], e = m.indptr[row + 1];
+  double mx = 0.0;
+  for (int k = s + gl; k < e; k += 16) {
+    const cx<double> v = csub(m.va[k], cmul(cs, m.vb[k]));
+    mx = fmax(mx, sqrt(v.re * v.re + v.im * v.im));
+  }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (gl == 0) {
+    double sc = 1.0;
+    if (mx > 0.0 && isfinite(mx)) sc = ldexp(1.0, -(int)rint(log2(mx)));
+    scale[row] = sc;
+  }
+}
+
+hipError_t launch_row_scale(const CsrDev& m, double sre, double sim, double* scale, hipStream_t s) {
+  if (m.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m.n + 15) / 16)), dim3(256), 0, s, m,
+                     cx<double>{sre, sim}, scale);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// level-0 block scatter
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void build_blocks_kernel(CsrDev m, cx<double> cs, const double* __restrict__ scale,
+                                                           const int32_t* __restrict__ blk_of_row,
+                                                           const int64_t* __restrict__ blk_off,
+                                                           cx<T>* const* __restrict__ Dp,
+                                                           cx<T>* const* __restrict__ Lp,
+                                                           cx<T>* const* __restrict__ Up) {
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + g;
+  if (row >= m.n) return;
+  const int kb = blk_of_row[row];
+  const int64_t ro = blk_off[kb];
+  const int64_t nk = blk_off[kb + 1] - ro;
+  const int64_t li = row - ro;
+  const double sc = scale[row];
+  const int s = m.indptr[row], e = m.indptr[row + 1];
+  for (int k = s + gl; k < e; k += 16) {
+    const int j = m.indices[k];
+    const int kc = blk_of_row[j];
+    const int64_t lj = j - blk_off[kc];
+    const cx<double> v = csub(m.va[k], cmul(cs, m.vb[k]));
+    const cx<T> w{(T)(sc * v.re), (T)(sc * v.im)};
+    cx<T>* dst = nullptr;
+    if (kc == kb) dst = Dp[kb];
+    else if (kc == kb - 1) dst = Lp[kb];
+    else if (kc == kb + 1) dst = Up[kb];
+    if (dst) dst[lj * nk + li] = w;
+  }
+}
+
+template <class T>
+hipError_t launch_build_blocks(const CsrDev& m, double sre, double sim, const double* scale,
+                               const int32_t* blk_of_row, const int64_t* blk_off, cx<T>* const* Dp,
+                               cx<T>* const* Lp, cx<T>* const* Up, hipStream_t s) {
+  if (m.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(build_blocks_kernel<T>, dim3((unsigned)((m.n + 15) / 16)), dim3(256), 0, s, m,
+                     cx<double>{sre, sim}, scale, blk_of_row, blk_off, Dp, Lp, Up);
+  return hipGetLastError();
+}
+template hipError_t launch_build_blocks<double>(const CsrDev&, double, double, const double*, const int32_t*,
+                                                const int64_t*, cx<double>* const*, cx<double>* const*,
+                                                cx<double>* const*, hipStream_t);
+template hipError_t launch_build_blocks<float>(const CsrDev&, double, double, const double*, const int32_t*,
+                                               const int64_t*, cx<float>* const*, cx<float>* const*,
+                                               cx<float>* const*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// batched skinny products for the solve sweeps
+// ---------------------------------------------------------------------------
+// Workgroup: 64 rows x 4 k-slices.  Lanes of a wave share the k index, so the
+// X entries they read from LDS are broadcasts and the A column segment they
+// read from global memory is contiguous.
+constexpr int SK_KC = 128;
+
+template <class T, int NR>
+__global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
+  const SkinnyDesc<T> d = descs[blockIdx.y];
+  const int r0 = blockIdx.x * 64;
+  if (r0 >= d.m) return;
+  const int rl = threadIdx.x & 63, ks = threadIdx.x >> 6;
+  const int row = r0 + rl;
+  __shared__ cx<T> sX[SK_KC][NR];
+  __shared__ cx<T> sP[3][64][NR];
+  cx<T> acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = czero<T>();
+  for (int p = 0; p < 2; ++p) {
+    const cx<T>* __restrict__ A = d.A[p];
+    const cx<T>* __restrict__ X = d.X[p];
+    const int K = d.k[p];
+    if (A == nullptr || K <= 0) continue;
+    const int64_t lda = d.lda[p], ldx = d.ldx[p];
+    for (int kc0 = 0; kc0 < K; kc0 += SK_KC) {
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < SK_KC * NR; idx += 256) {
+        const int kk = idx / NR, c = idx - kk * NR;
+        cx<T> v = czero<T>();
+        if (kc0 + kk < K && c < nrhs) v = X[(int64_t)c * ldx + kc0 + kk];
+        sX[kk][c] = v;
+      }
+      __syncthreads();
+      const int kend = (K - kc0) < SK_KC ? (K - kc0) : SK_KC;
+      if (row < d.m) {
+        for (int kk = ks; kk < kend; kk += 4) {
+          const cx<T> a = A[(int64_t)(kc0 + kk) * lda + row];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a, sX[kk][c]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (ks > 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) sP[ks - 1][rl][c] = acc[c];
+  }
+  __syncthreads();
+  if (ks == 0 && row < d.m) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (c < nrhs) {
+        cx<T> v = acc[c];
+        v = cadd(v, sP[0][rl][c]);
+        v = cadd(v, sP[1][rl][c]);
+        v = cadd(v, sP[2][rl][c]);
+        cx<T>* y = d.Y + (int64_t)c * d.ldy + row;
+        cx<T> o{d.alpha * v.re, d.alpha * v.im};
+        if (d.beta != T(0)) {
+          const cx<T> old = *y;
+          o.re += d.beta * old.re;
+          o.im += d.beta * old.im;
+        }
+        *y = o;
+      }
+    }
+  }
+}
+
+template <class T>
+hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s) {
+  if (ndesc <= 0 || max_m <= 0) return hipSuccess;
+  if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((max_m + 63) / 64), (unsigned)ndesc);
+  if (nrhs == 1)
+    hipLaunchKernelGGL((skinny_kernel<T, 1>), grid, dim3(256), 0, s, d_descs, nrhs);
+  else if (nrhs <= 2)
+    hipLaunchKernelGGL((skinny_kernel<T, 2>), grid, dim3(256), 0, s, d_descs, nrhs);
+  else if (nrhs <= 4)
+    hipLaunchKernelGGL((skinny_kernel<T, 4>), grid, dim3(256), 0, s, d_descs, nrhs);
+  else if (nrhs <= 8)
+    hipLaunchKernelGGL((skinny_kernel<T, 8>), grid, dim3(256), 0, s, d_descs, nrhs);
+  else
+    hipLaunchKernelGGL((skinny_kernel<T, 16>), grid, dim3(256), 0, s, d_descs, nrhs);
+  return hipGetLastError();
+}
+template hipError_t launch_skinny<double>(const SkinnyDesc<double>*, int, int, int, hipStream_t);
+template hipError_t launch_skinny<float>(const SkinnyDesc<float>*, int, int, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// vector plumbing
+// ---------------------------------------------------------------------------
+template <class Tin, class Tout>
+__global__ void gather_scale_kernel(const cx<Tin>* __restrict__ x, int64_t ldx, cx<Tout>* __restrict__ y,
+                                    int64_t ldy, int64_t n, const int64_t* __restrict__ perm,
+                                    const double* __restrict__ scale, double mult) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  const int64_t src = perm ? perm[i] : i;
+  const cx<Tin> v = x[(int64_t)c * ldx + src];
+  const double f = mult * (scale ? scale[i] : 1.0);
+  y[(int64_t)c * ldy + i] = cx<Tout>{(Tout)(f * (double)v.re), (Tout)(f * (double)v.im)};
+}
+
+template <class Tin, class Tout>
+hipError_t launch_gather_scale(const cx<Tin>* x, int64_t ldx, cx<Tout>* y, int64_t ldy, int64_t n, int nrhs,
+                               const int64_t* perm, const double* scale, double mult, hipStream_t s) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL((gather_scale_kernel<Tin, Tout>), dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s,
+                     x, ldx, y, ldy, n, perm, scale, mult);
+  return hipGetLastError();
+}
+template hipError_t launch_gather_scale<double, double>(const cx<double>*, int64_t, cx<double>*, int64_t, int64_t,
+                                                        int, const int64_t*, const double*, double, hipStream_t);
+template hipError_t launch_gather_scale<double, float>(const cx<double>*, int64_t, cx<float>*, int64_t, int64_t,
+                                                       int, const int64_t*, const double*, double, hipStream_t);
+template hipError_t launch_gather_scale<float, double>(const cx<float>*, int64_t, cx<double>*, int64_t, int64_t,
+                                                       int, const int64_t*, const double*, double, hipStream_t);
+
+__global__ void scatter_kernel(const cx<double>* __restrict__ x, int64_t ldx, cx<double>* __restrict__ y,
+                               int64_t ldy, int64_t n, const int64_t* __restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  y[(int64_t)c * ldy + perm[i]] = x[(int64_t)c * ldx + i];
+}
+
+hipError_t launch_scatter(const cx<double>* x, int64_t ldx, cx<double>* y, int64_t ldy, int64_t n, int nrhs,
+                          const int64_t* perm, hipStream_t s) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, x, ldx, y, ldy, n,
+                     perm);
+  return hipGetLastError();
+}
+
+template <class Td>
+__global__ void axpy_kernel(cx<double>* __restrict__ x, int64_t ldx, const cx<Td>* __restrict__ d, int64_t ldd,
+                            int64_t n, double mult) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  const cx<Td> v = d[(int64_t)c * ldd + i];
+  cx<double>& o = x[(int64_t)c * ldx + i];
+  o.re += mult * (double)v.re;
+  o.im += mult * (double)v.im;
+}
+
+template <class Td>
+hipError_t launch_axpy(cx<double>* x, int64_t ldx, const cx<Td>* d, int64_t ldd, int64_t n, int nrhs, double mult,
+                       hipStream_t s) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL((axpy_kernel<Td>), dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, x, ldx, d, ldd,
+                     n, mult);
+  return hipGetLastError();
+}
+template hipError_t launch_axpy<double>(cx<double>*, int64_t, const cx<double>*, int64_t, int64_t, int, double,
+                                        hipStream_t);
+template hipError_t launch_axpy<float>(cx<double>*, int64_t, const cx<float>*, int64_t, int64_t, int, double,
+                                       hipStream_t);
+
+template <bool MAXABS>
+__global__ __launch_bounds__(256) void colred_kernel(const cx<double>* __restrict__ x, int64_t ldx, int64_t n,
+                                                     double* __restrict__ partial, int nrhs) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const cx<double> v = x[(int64_t)c * ldx + i];
+    if (MAXABS)
+      acc = fmax(acc, sqrt(v.re * v.re + v.im * v.im));
+    else
+      acc += v.re * v.re + v.im * v.im;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(acc, off, 64);
+    acc = MAXABS ? fmax(acc, o) : acc + o;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int q = 1; q < 4; ++q) r = MAXABS ? fmax(r, red[q]) : r + red[q];
+    partial[(int64_t)blockIdx.x * nrhs + c] = r;
+  }
+}
+
+hipError_t launch_colnorm2(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial, int nblocks,
+                           hipStream_t s) {
+  if (nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL(colred_kernel<false>, dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial, nrhs);
+  return hipGetLastError();
+}
+hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial, int nblocks,
+                            hipStream_t s) {
+  if (nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL(colred_kernel<true>, dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial, nrhs);
+  return hipGetLastError();
+}
+
+__global__ void sum_parts_kernel(const double* __restrict__ in, int nparts, int len, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += in[(int64_t)p * len + i];
+  out[i] = s;
+}
+
+hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s) {
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((len + 255) / 256), dim3(256), 0, s, in, nparts, len, out);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void zero_kernel(cx<T>* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = czero<T>();
+}
+
+template <class T>
+hipError_t launch_zero(cx<T>* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(zero_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+template hipError_t launch_zero<double>(cx<double>*, int64_t, hipStream_t);
+template hipError_t launch_zero<float>(cx<float>*, int64_t, hipStream_t);
+
+__global__ void copy_cols_kernel(const cx<double>* __restrict__ x, int64_t ldx, cx<double>* __restrict__ y,
+                                 int64_t ldy, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  y[(int64_t)c * ldy + i] = x[(int64_t)c * ldx + i];
+}
+
+hipError_t launch_copy_cols(const cx<double>* x, int64_t ldx, cx<double>* y, int64_t ldy, int64_t n, int nrhs,
+                            hipStream_t s) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_cols_kernel, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, x, ldx, y, ldy, n);
+  return hipGetLastError();
+}
+
+}  // namespace kore

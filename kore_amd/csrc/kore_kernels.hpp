@@ -1,0 +1,100 @@
+// Kernel descriptors and launchers (host-visible declarations).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kore {
+
+template <class T> struct cx;
+
+// C = beta*C + alpha*(op(A1)*B1 + op(A2)*B2); column-major; op = identity
+// (opA = 0) or conjugate transpose of a K x m stored A (opA = 1).
+template <class T> struct GemmDesc {
+  const cx<T>* A[2];
+  const cx<T>* B[2];
+  cx<T>* C;
+  int64_t lda[2], ldb[2], ldc;
+  int m, n, k[2];
+  int opA;
+  int tiles_m, tiles_n;
+  T alpha, beta;
+};
+
+// in-place inverse of an n x n column-major block
+template <class T> struct InvDesc {
+  cx<T>* M;
+  int64_t ld;
+  int n;
+  int* ipiv;   // n ints of device scratch
+  int* info;   // 0 or (1-based column of the first zero pivot)
+};
+
+// Y = beta*Y + alpha*(A1*X1 + A2*X2) for skinny X (nrhs <= 16 columns);
+// A column-major m x k.
+template <class T> struct SkinnyDesc {
+  const cx<T>* A[2];
+  const cx<T>* X[2];
+  cx<T>* Y;
+  int64_t lda[2], ldx[2], ldy;
+  int m, k[2];
+  T alpha, beta;
+};
+
+template <class T>
+hipError_t launch_gemm_batched(const GemmDesc<T>* d_descs, int ndesc, int max_tiles, hipStream_t s);
+template <class T>
+hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hipStream_t s);
+template <class T>
+hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s);
+
+// ---- sparse / vector kernels (kernels_vec.hip) -----------------------------
+// CSR on the union pattern of A and B, rows and columns already in l-major
+// order; va, vb are the complex values of A and B on that pattern.
+struct CsrDev {
+  int64_t n;
+  const int32_t* indptr;
+  const int32_t* indices;
+  const cx<double>* va;
+  const cx<double>* vb;
+};
+
+// Y[:, c] = beta*Y[:, c] + (ca*A + cb*B) X[:, c],  c < nrhs  (complex128)
+hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re, double cb_im,
+                       const cx<double>* X, int64_t ldx, cx<double>* Y, int64_t ldy, int nrhs,
+                       double beta, hipStream_t s);
+// s[i] = 2^-round(log2 max_j |a_ij - sigma b_ij|)
+hipError_t launch_row_scale(const CsrDev& m, double sre, double sim, double* scale, hipStream_t s);
+// Scatter R (A - sigma B) into the dense level-0 blocks.
+template <class T>
+hipError_t launch_build_blocks(const CsrDev& m, double sre, double sim, const double* scale,
+                               const int32_t* blk_of_row, const int64_t* blk_off,
+                               cx<T>* const* Dp, cx<T>* const* Lp, cx<T>* const* Up,
+                               hipStream_t s);
+// y[i] = scale[i] * x[perm[i]]   (perm may be null: identity), with type conversion
+template <class Tin, class Tout>
+hipError_t launch_gather_scale(const cx<Tin>* x, int64_t ldx, cx<Tout>* y, int64_t ldy, int64_t n,
+                               int nrhs, const int64_t* perm, const double* scale, double mult,
+                               hipStream_t s);
+// y[perm[i]] = x[i]
+hipError_t launch_scatter(const cx<double>* x, int64_t ldx, cx<double>* y, int64_t ldy, int64_t n,
+                          int nrhs, const int64_t* perm, hipStream_t s);
+// x[:, c] += mult * d[:, c]   (d complex64 or complex128)
+template <class Td>
+hipError_t launch_axpy(cx<double>* x, int64_t ldx, const cx<Td>* d, int64_t ldd, int64_t n, int nrhs,
+                       double mult, hipStream_t s);
+// partial[b*nrhs + c] = sum over rows in chunk b of |x[:, c]|^2 ; nblocks written to *nb
+hipError_t launch_colnorm2(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial,
+                           int nblocks, hipStream_t s);
+// out[c] = max over chunks of per-chunk max |x| ... (max abs per column)
+hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial,
+                            int nblocks, hipStream_t s);
+// sum nparts consecutive arrays of len doubles: out[i] = sum_p in[p*len + i]
+hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s);
+// fill complex buffer with zeros (graph-capture-safe clear)
+template <class T>
+hipError_t launch_zero(cx<T>* p, int64_t n, hipStream_t s);
+// copy columns (complex128)
+hipError_t launch_copy_cols(const cx<double>* x, int64_t ldx, cx<double>* y, int64_t ldy, int64_t n,
+                            int nrhs, hipStream_t s);
+
+}  // namespace kore

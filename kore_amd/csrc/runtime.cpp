@@ -1,0 +1,1296 @@
+// Host runtime of the Kore MI355X solver backend: C ABI, degree-major
+// layout, block-cyclic-reduction plan, factor/solve, mixed-precision
+// refinement and the device Krylov workspace.
+//
+// Division of labour: everything that streams sizmat-long data or touches
+// the factor runs on the device (kernels_dense.hip, kernels_vec.hip);
+// the host builds the permutation, the union CSR pattern of A and B in
+// degree-major order, the reduction plan (descriptor lists uploaded once
+// per factorisation) and the row equilibration.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kore_amd.h"
+#include "kore_device.hpp"
+#include "kore_kernels.hpp"
+
+using namespace kore;
+using zd = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(...)                                                                             \
+  do {                                                                                           \
+    hipError_t _e = (__VA_ARGS__);                                                               \
+    if (_e != hipSuccess)                                                                        \
+      return fail(KORE_ERR_HIP, "%s failed: %s (%s:%d)", #__VA_ARGS__, hipGetErrorString(_e), __FILE__, \
+                  __LINE__);                                                                     \
+  } while (0)
+
+#define TRY(...)                  \
+  do {                            \
+    int32_t _r = (__VA_ARGS__);   \
+    if (_r < 0) return _r;        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct TimerRec {
+  int cls;
+  hipEvent_t a, b;
+};
+
+struct kore_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t budget = 0;
+  std::atomic<int64_t> in_use{0};
+  bool timing = false;
+  std::vector<TimerRec> pending;
+  std::vector<hipEvent_t> pool;
+  double tot_ms[KORE_TIMER_COUNT] = {0};
+  int64_t cnt[KORE_TIMER_COUNT] = {0};
+};
+
+static hipEvent_t get_event(kore_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Timed {
+  kore_ctx* c;
+  int cls;
+  hipEvent_t a = nullptr;
+  Timed(kore_ctx* ctx, int k) : c(ctx), cls(k) {
+    if (c->timing) {
+      a = get_event(c);
+      if (a) hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (c->timing && a) {
+      hipEvent_t b = get_event(c);
+      if (b) {
+        hipEventRecord(b, c->stream);
+        c->pending.push_back({cls, a, b});
+      }
+    }
+  }
+};
+
+static void resolve_timers(kore_ctx* c) {
+  if (c->pending.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& t : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      c->tot_ms[t.cls] += ms;
+      c->cnt[t.cls] += 1;
+    }
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->pending.clear();
+}
+
+struct DevBuf {
+  kore_ctx* c = nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  int32_t alloc(kore_ctx* ctx, size_t nbytes) {
+    release();
+    c = ctx;
+    if (nbytes == 0) return KORE_OK;
+    if (c->budget > 0 && c->in_use.load() + (int64_t)nbytes > c->budget)
+      return fail(KORE_ERR_OOM, "device memory budget exceeded: need %zu bytes, %lld of %lld in use", nbytes,
+                  (long long)c->in_use.load(), (long long)c->budget);
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(KORE_ERR_OOM, "hipMalloc(%zu) failed: %s", nbytes, hipGetErrorString(e));
+    }
+    bytes = nbytes;
+    c->in_use += (int64_t)nbytes;
+    return KORE_OK;
+  }
+  void release() {
+    if (p) {
+      hipFree(p);
+      c->in_use -= (int64_t)bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class U> U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+// Pinned staging for small host->device uploads (descriptors, small
+// matrices) so that they stay asynchronous on the library stream.
+struct Uploader {
+  kore_ctx* c = nullptr;
+  char* host = nullptr;
+  size_t cap = 0, off = 0;
+  ~Uploader() {
+    if (host) hipHostFree(host);
+  }
+  int32_t init(kore_ctx* ctx, size_t bytes) {
+    c = ctx;
+    cap = bytes;
+    HIP_TRY(hipHostMalloc((void**)&host, cap, hipHostMallocDefault));
+    return KORE_OK;
+  }
+  int32_t put(void* dst, const void* src, size_t nbytes) {
+    if (nbytes > cap) {
+      HIP_TRY(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      return KORE_OK;
+    }
+    if (off + nbytes > cap) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      off = 0;
+    }
+    memcpy(host + off, src, nbytes);
+    HIP_TRY(hipMemcpyAsync(dst, host + off, nbytes, hipMemcpyHostToDevice, c->stream));
+    off += (nbytes + 255) & ~size_t(255);
+    return KORE_OK;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// layout
+// ---------------------------------------------------------------------------
+static void ell_lists(int m, int lmax, int vsymm, std::vector<int>& top, std::vector<int>& bot,
+                      std::vector<int>& all) {
+  const int lm1 = lmax - m + 1;
+  const int s = vsymm > 0 ? 1 : 0;
+  const int sg = m > 0 ? 1 : 0;
+  all.clear();
+  top.clear();
+  bot.clear();
+  for (int l = m + 1 - sg; l < lmax + 2 - sg; ++l) all.push_back(l);
+  const int first = (sg + s) % 2;
+  for (int i = first; i < lm1; i += 2) top.push_back(all[i]);
+  for (int i = 1 - first; i < lm1; i += 2) bot.push_back(all[i]);
+}
+
+static int32_t layout_build(const kore_layout* L, int64_t* sizmat, std::vector<int>* sizes,
+                            std::vector<int64_t>* perm) {
+  if (!L) return fail(KORE_ERR_ARG, "layout is NULL");
+  if (L->N1 <= 0 || L->m < 0 || L->lmax < L->m) return fail(KORE_ERR_ARG, "bad layout sizes");
+  if ((L->lmax - L->m + 1) % 2 != 0)
+    return fail(KORE_ERR_ARG, "lmax - m + 1 must be even (lmax=%d, m=%d)", L->lmax, L->m);
+  if ((L->symm != 1 && L->symm != -1) || (L->symmB0 != 1 && L->symmB0 != -1))
+    return fail(KORE_ERR_ARG, "symm and symmB0 must be +1 or -1");
+  if (!(L->hydro || L->magnetic || L->thermal || L->compositional)) return fail(KORE_ERR_ARG, "no fields");
+  std::vector<int> top, bot, all, mtop, mbot, mall;
+  ell_lists(L->m, L->lmax, L->symm, top, bot, all);
+  ell_lists(L->m, L->lmax, L->symm * L->symmB0, mtop, mbot, mall);
+  const int64_t nb = (L->lmax - L->m + 1) / 2;
+  const int64_t n = (int64_t)L->N1 * nb;
+  struct Sec {
+    int64_t off;
+    const std::vector<int>* lst;
+  };
+  std::vector<Sec> secs;
+  int64_t off = 0;
+  if (L->hydro) {
+    secs.push_back({off, &top});
+    secs.push_back({off + n, &bot});
+    off += 2 * n;
+  }
+  if (L->magnetic) {
+    const std::vector<int>* f_l = L->symmB0 == 1 ? &mtop : &mbot;
+    const std::vector<int>* g_l = L->symmB0 == 1 ? &mbot : &mtop;
+    secs.push_back({off, f_l});
+    secs.push_back({off + n, g_l});
+    off += 2 * n;
+  }
+  if (L->thermal) {
+    secs.push_back({off, &top});
+    off += n;
+  }
+  if (L->compositional) {
+    secs.push_back({off, &top});
+    off += n;
+  }
+  *sizmat = off;
+  if (sizes) sizes->clear();
+  if (perm) {
+    perm->clear();
+    perm->reserve(off);
+  }
+  for (int l : all) {
+    int sz = 0;
+    for (auto& s : secs) {
+      auto it = std::find(s.lst->begin(), s.lst->end(), l);
+      if (it == s.lst->end()) continue;
+      const int64_t idx = it - s.lst->begin();
+      if (perm)
+        for (int r = 0; r < L->N1; ++r) perm->push_back(s.off + idx * L->N1 + r);
+      sz += L->N1;
+    }
+    if (sz && sizes) sizes->push_back(sz);
+  }
+  if (perm && (int64_t)perm->size() != off) return fail(KORE_ERR_ARG, "internal: permutation size mismatch");
+  return KORE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// factor (block cyclic reduction)
+// ---------------------------------------------------------------------------
+struct FactorBase {
+  virtual ~FactorBase() = default;
+  int prec = 0;
+  double flops = 0;
+  int64_t bytes = 0;
+  int nlev = 0;
+};
+
+template <class T> struct Level {
+  InvDesc<T>* inv = nullptr;
+  int ninv = 0, maxn_inv = 0;
+  GemmDesc<T>* g1 = nullptr;
+  int ng1 = 0, t1 = 0;
+  GemmDesc<T>* g2 = nullptr;
+  int ng2 = 0, t2 = 0;
+  SkinnyDesc<T>* fw = nullptr;
+  int nfw = 0, mfw = 0;
+  SkinnyDesc<T>* bw1 = nullptr;
+  int nbw1 = 0, mbw1 = 0;
+  SkinnyDesc<T>* bw2 = nullptr;
+  int nbw2 = 0, mbw2 = 0;
+};
+
+template <class T> struct Factor : FactorBase {
+  DevBuf mem, aux, desc, vecs;
+  std::vector<cx<T>*> D, L, U;
+  std::vector<Level<T>> lev;
+  InvDesc<T>* root_inv = nullptr;
+  SkinnyDesc<T>* root_solve = nullptr;
+  int root_n = 0;
+  cx<T>** dD = nullptr;
+  cx<T>** dL = nullptr;
+  cx<T>** dU = nullptr;
+  int* info = nullptr;
+  cx<T>* Bv = nullptr;  // solve vectors, sizmat x 16, degree-major
+  cx<T>* Xv = nullptr;
+  int nblk = 0;
+  int64_t ndlu = 0;    // elements of the D/L/U region (cleared before the scatter)
+};
+
+struct kore_sys {
+  kore_ctx* ctx = nullptr;
+  kore_layout lay{};
+  int64_t n = 0;
+  int nblk = 0, nmax = 0;
+  std::vector<int> bsz;
+  std::vector<int64_t> boff;
+  std::vector<int64_t> perm;
+  std::vector<int32_t> h_ip, h_ix, h_blk;
+  std::vector<zd> h_va, h_vb;
+  bool has_b = false;
+  DevBuf d_ip, d_ix, d_va, d_vb, d_perm, d_blk, d_boff, d_scale, d_scr, d_part;
+  CsrDev csr{};
+  // scratch vectors (complex128, sizmat x 16 each): bb (rhs), xr (solution),
+  // rr (residual), st (staging / scaled residual)
+  cx<double>* bb = nullptr;
+  cx<double>* xr = nullptr;
+  cx<double>* rr = nullptr;
+  cx<double>* st = nullptr;
+  double* part = nullptr;
+  int npart = 64;
+  std::unique_ptr<FactorBase> f;
+  double sre = 0, sim = 0;
+  double normRM = 0;   // ||R (A - sigma B)||_F
+  std::vector<double> h_scale;
+  Uploader up;
+};
+
+struct kore_ws {
+  kore_sys* sys = nullptr;
+  int ncol = 0;
+  DevBuf V, part, qbuf, descb;
+  Uploader up;
+};
+
+static constexpr int MAXR = 16;
+
+template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
+  kore_ctx* c = s->ctx;
+  const int nb = s->nblk, nmax = s->nmax;
+  F->nblk = nb;
+  // level id lists
+  std::vector<std::vector<int>> ids;
+  {
+    std::vector<int> cur(nb);
+    for (int k = 0; k < nb; ++k) cur[k] = k;
+    while (cur.size() > 1) {
+      ids.push_back(cur);
+      std::vector<int> nx;
+      for (size_t t = 0; t < cur.size(); t += 2) nx.push_back(cur[t]);
+      cur.swap(nx);
+    }
+    ids.push_back(cur);  // root
+  }
+  const int nlev = (int)ids.size() - 1;
+  F->nlev = nlev;
+  // memory plan: D_k (n_k^2), L_k, U_k (n_k x nmax), E/F per level
+  auto nk = [&](int k) { return (int64_t)s->bsz[k]; };
+  int64_t cnt = 0;
+  std::vector<int64_t> offD(nb), offL(nb, -1), offU(nb, -1);
+  for (int k = 0; k < nb; ++k) {
+    offD[k] = cnt;
+    cnt += nk(k) * nk(k);
+  }
+  for (int k = 0; k < nb; ++k) {
+    if (k > 0) {
+      offL[k] = cnt;
+      cnt += nk(k) * nmax;
+    }
+    if (k < nb - 1) {
+      offU[k] = cnt;
+      cnt += nk(k) * nmax;
+    }
+  }
+  const int64_t cntDLU = cnt;
+  std::vector<std::vector<int64_t>> offE(nlev), offF(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    const auto& id = ids[l];
+    const int T_ = (int)id.size();
+    offE[l].assign(T_, -1);
+    offF[l].assign(T_, -1);
+    for (int t = 0; t < T_; t += 2) {
+      if (t - 1 >= 0) {
+        offE[l][t] = cnt;
+        cnt += nk(id[t]) * nk(id[t - 1]);
+      }
+      if (t + 1 <= T_ - 1) {
+        offF[l][t] = cnt;
+        cnt += nk(id[t]) * nk(id[t + 1]);
+      }
+    }
+  }
+  TRY(F->mem.alloc(c, (size_t)cnt * sizeof(cx<T>)));
+  cx<T>* base = F->mem.template as<cx<T>>();
+  F->D.resize(nb);
+  F->L.assign(nb, nullptr);
+  F->U.assign(nb, nullptr);
+  for (int k = 0; k < nb; ++k) {
+    F->D[k] = base + offD[k];
+    if (offL[k] >= 0) F->L[k] = base + offL[k];
+    if (offU[k] >= 0) F->U[k] = base + offU[k];
+  }
+  // aux: ipiv (nb * nmax ints), info (nb ints), pointer arrays, solve vectors
+  const size_t auxbytes = (size_t)nb * nmax * sizeof(int) + (size_t)nb * sizeof(int) + 3 * (size_t)nb * sizeof(void*) + 256;
+  TRY(F->aux.alloc(c, auxbytes));
+  char* ab = F->aux.template as<char>();
+  int* ipiv = reinterpret_cast<int*>(ab);
+  F->info = ipiv + (size_t)nb * nmax;
+  char* pp = reinterpret_cast<char*>(F->info + nb);
+  pp = reinterpret_cast<char*>(((uintptr_t)pp + 63) & ~uintptr_t(63));
+  F->dD = reinterpret_cast<cx<T>**>(pp);
+  F->dL = F->dD + nb;
+  F->dU = F->dL + nb;
+  TRY(F->vecs.alloc(c, 2 * (size_t)s->n * MAXR * sizeof(cx<T>)));
+  F->Bv = F->vecs.template as<cx<T>>();
+  F->Xv = F->Bv + (size_t)s->n * MAXR;
+  F->bytes = (int64_t)(F->mem.bytes + F->aux.bytes + F->vecs.bytes);
+
+  // descriptors
+  std::vector<InvDesc<T>> vinv;
+  std::vector<GemmDesc<T>> vg;
+  std::vector<SkinnyDesc<T>> vs;
+  struct Rng {
+    size_t off, n;
+  };
+  std::vector<Rng> rinv(nlev + 1), rg1(nlev), rg2(nlev), rfw(nlev), rb1(nlev), rb2(nlev);
+  F->lev.resize(nlev);
+  double flops = 0;
+  auto tiles = [](int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); };
+  auto mkgemm = [&](cx<T>* C, int m, int n, const cx<T>* A1, const cx<T>* B1, int k1, int lda1, int ldb1,
+                    const cx<T>* A2, const cx<T>* B2, int k2, int lda2, int ldb2, T alpha, T beta) {
+    GemmDesc<T> g{};
+    g.A[0] = A1; g.B[0] = B1; g.k[0] = A1 ? k1 : 0; g.lda[0] = lda1; g.ldb[0] = ldb1;
+    g.A[1] = A2; g.B[1] = B2; g.k[1] = A2 ? k2 : 0; g.lda[1] = lda2; g.ldb[1] = ldb2;
+    g.C = C; g.ldc = m; g.m = m; g.n = n; g.opA = 0;
+    g.tiles_m = (m + 63) / 64; g.tiles_n = (n + 63) / 64;
+    g.alpha = alpha; g.beta = beta;
+    flops += 8.0 * m * n * ((A1 ? k1 : 0) + (A2 ? k2 : 0));
+    return g;
+  };
+  auto vecp = [&](cx<T>* v, int blk) { return v + s->boff[blk]; };
+  const int ldv = (int)s->n;
+  auto mkskin = [&](cx<T>* Y, int m, const cx<T>* A1, int lda1, const cx<T>* X1, int k1, const cx<T>* A2, int lda2,
+                    const cx<T>* X2, int k2, T alpha, T beta) {
+    SkinnyDesc<T> d{};
+    d.A[0] = A1; d.X[0] = X1; d.k[0] = A1 ? k1 : 0; d.lda[0] = lda1; d.ldx[0] = ldv;
+    d.A[1] = A2; d.X[1] = X2; d.k[1] = A2 ? k2 : 0; d.lda[1] = lda2; d.ldx[1] = ldv;
+    d.Y = Y; d.ldy = ldv; d.m = m; d.alpha = alpha; d.beta = beta;
+    return d;
+  };
+  for (int l = 0; l < nlev; ++l) {
+    const auto& id = ids[l];
+    const int T_ = (int)id.size();
+    Level<T>& L = F->lev[l];
+    // inversions of odd positions
+    rinv[l].off = vinv.size();
+    for (int t = 1; t < T_; t += 2) {
+      const int k = id[t];
+      vinv.push_back(InvDesc<T>{F->D[k], nk(k), (int)nk(k), ipiv + (size_t)k * nmax, F->info + k});
+      flops += 8.0 * nk(k) * nk(k) * nk(k);
+      L.maxn_inv = std::max(L.maxn_inv, (int)nk(k));
+    }
+    rinv[l].n = vinv.size() - rinv[l].off;
+    // g1: E_t = L_t Dinv_{t-1}, F_t = U_t Dinv_{t+1}
+    rg1[l].off = vg.size();
+    for (int t = 0; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      if (t - 1 >= 0) {
+        const int kp = id[t - 1];
+        vg.push_back(mkgemm(base + offE[l][t], m, (int)nk(kp), F->L[k], F->D[kp], (int)nk(kp), m, (int)nk(kp),
+                            nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+        L.t1 = std::max(L.t1, tiles(m, (int)nk(kp)));
+      }
+      if (t + 1 <= T_ - 1) {
+        const int kn = id[t + 1];
+        vg.push_back(mkgemm(base + offF[l][t], m, (int)nk(kn), F->U[k], F->D[kn], (int)nk(kn), m, (int)nk(kn),
+                            nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+        L.t1 = std::max(L.t1, tiles(m, (int)nk(kn)));
+      }
+    }
+    rg1[l].n = vg.size() - rg1[l].off;
+    // g2: D_t -= E U_{t-1} + F L_{t+1};  L_t = -E L_{t-1};  U_t = -F U_{t+1}
+    rg2[l].off = vg.size();
+    for (int t = 0; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
+      if (hasE || hasF) {
+        const int kp = hasE ? id[t - 1] : -1, kn = hasF ? id[t + 1] : -1;
+        vg.push_back(mkgemm(F->D[k], m, m, hasE ? base + offE[l][t] : nullptr, hasE ? F->U[kp] : nullptr,
+                            hasE ? (int)nk(kp) : 0, m, hasE ? (int)nk(kp) : 0, hasF ? base + offF[l][t] : nullptr,
+                            hasF ? F->L[kn] : nullptr, hasF ? (int)nk(kn) : 0, m, hasF ? (int)nk(kn) : 0, T(-1),
+                            T(1)));
+        L.t2 = std::max(L.t2, tiles(m, m));
+      }
+      if (t - 2 >= 0) {
+        const int kp = id[t - 1], kpp = id[t - 2];
+        vg.push_back(mkgemm(F->L[k], m, (int)nk(kpp), base + offE[l][t], F->L[kp], (int)nk(kp), m, (int)nk(kp),
+                            nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
+        L.t2 = std::max(L.t2, tiles(m, (int)nk(kpp)));
+      }
+      if (t + 2 <= T_ - 1) {
+        const int kn = id[t + 1], knn = id[t + 2];
+        vg.push_back(mkgemm(F->U[k], m, (int)nk(knn), base + offF[l][t], F->U[kn], (int)nk(kn), m, (int)nk(kn),
+                            nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
+        L.t2 = std::max(L.t2, tiles(m, (int)nk(knn)));
+      }
+    }
+    rg2[l].n = vg.size() - rg2[l].off;
+    // solve: forward  B_t -= E_t B_{t-1} + F_t B_{t+1}   (t even)
+    rfw[l].off = vs.size();
+    for (int t = 0; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
+      if (!hasE && !hasF) continue;
+      vs.push_back(mkskin(vecp(F->Bv, k), m, hasE ? base + offE[l][t] : nullptr, m,
+                          hasE ? vecp(F->Bv, id[t - 1]) : nullptr, hasE ? (int)nk(id[t - 1]) : 0,
+                          hasF ? base + offF[l][t] : nullptr, m, hasF ? vecp(F->Bv, id[t + 1]) : nullptr,
+                          hasF ? (int)nk(id[t + 1]) : 0, T(-1), T(1)));
+      L.mfw = std::max(L.mfw, m);
+    }
+    rfw[l].n = vs.size() - rfw[l].off;
+    // backward 1: B_t -= L_t X_{t-1} + U_t X_{t+1}   (t odd)
+    rb1[l].off = vs.size();
+    for (int t = 1; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      const bool hasR = t + 1 <= T_ - 1;
+      vs.push_back(mkskin(vecp(F->Bv, k), m, F->L[k], m, vecp(F->Xv, id[t - 1]), (int)nk(id[t - 1]),
+                          hasR ? F->U[k] : nullptr, m, hasR ? vecp(F->Xv, id[t + 1]) : nullptr,
+                          hasR ? (int)nk(id[t + 1]) : 0, T(-1), T(1)));
+      L.mbw1 = std::max(L.mbw1, m);
+    }
+    rb1[l].n = vs.size() - rb1[l].off;
+    // backward 2: X_t = Dinv_t B_t
+    rb2[l].off = vs.size();
+    for (int t = 1; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      vs.push_back(mkskin(vecp(F->Xv, k), m, F->D[k], m, vecp(F->Bv, k), m, nullptr, 0, nullptr, 0, T(1), T(0)));
+      L.mbw2 = std::max(L.mbw2, m);
+    }
+    rb2[l].n = vs.size() - rb2[l].off;
+  }
+  // root
+  const int kr = ids[nlev][0];
+  rinv[nlev].off = vinv.size();
+  vinv.push_back(InvDesc<T>{F->D[kr], nk(kr), (int)nk(kr), ipiv + (size_t)kr * nmax, F->info + kr});
+  rinv[nlev].n = 1;
+  flops += 8.0 * nk(kr) * nk(kr) * nk(kr);
+  const size_t rroot = vs.size();
+  vs.push_back(mkskin(vecp(F->Xv, kr), (int)nk(kr), F->D[kr], (int)nk(kr), vecp(F->Bv, kr), (int)nk(kr), nullptr, 0,
+                      nullptr, 0, T(1), T(0)));
+  F->root_n = (int)nk(kr);
+  F->flops = flops;
+
+  // upload descriptors
+  const size_t bi = vinv.size() * sizeof(InvDesc<T>);
+  const size_t bg = vg.size() * sizeof(GemmDesc<T>);
+  const size_t bsk = vs.size() * sizeof(SkinnyDesc<T>);
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  TRY(F->desc.alloc(c, al(bi) + al(bg) + al(bsk) + 256));
+  char* db = F->desc.template as<char>();
+  InvDesc<T>* dinv = reinterpret_cast<InvDesc<T>*>(db);
+  GemmDesc<T>* dg = reinterpret_cast<GemmDesc<T>*>(db + al(bi));
+  SkinnyDesc<T>* dsk = reinterpret_cast<SkinnyDesc<T>*>(db + al(bi) + al(bg));
+  HIP_TRY(hipMemcpyAsync(dinv, vinv.data(), bi, hipMemcpyHostToDevice, c->stream));
+  if (bg) HIP_TRY(hipMemcpyAsync(dg, vg.data(), bg, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dsk, vs.data(), bsk, hipMemcpyHostToDevice, c->stream));
+  std::vector<cx<T>*> hp(3 * (size_t)nb, nullptr);
+  for (int k = 0; k < nb; ++k) {
+    hp[k] = F->D[k];
+    hp[nb + k] = F->L[k];
+    hp[2 * nb + k] = F->U[k];
+  }
+  HIP_TRY(hipMemcpyAsync(F->dD, hp.data(), hp.size() * sizeof(void*), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int l = 0; l < nlev; ++l) {
+    Level<T>& L = F->lev[l];
+    L.inv = dinv + rinv[l].off; L.ninv = (int)rinv[l].n;
+    L.g1 = dg + rg1[l].off; L.ng1 = (int)rg1[l].n;
+    L.g2 = dg + rg2[l].off; L.ng2 = (int)rg2[l].n;
+    L.fw = dsk + rfw[l].off; L.nfw = (int)rfw[l].n;
+    L.bw1 = dsk + rb1[l].off; L.nbw1 = (int)rb1[l].n;
+    L.bw2 = dsk + rb2[l].off; L.nbw2 = (int)rb2[l].n;
+  }
+  F->root_inv = dinv + rinv[nlev].off;
+  F->root_solve = dsk + rroot;
+  F->ndlu = cntDLU;
+  return KORE_OK;
+}
+
+template <class T> static int64_t factor_count_bytes(const std::vector<int>& bsz, int64_t n) {
+  const int nb = (int)bsz.size();
+  int nmax = 0;
+  for (int b : bsz) nmax = std::max(nmax, b);
+  int64_t cnt = 0;
+  for (int k = 0; k < nb; ++k) {
+    cnt += (int64_t)bsz[k] * bsz[k];
+    if (k > 0) cnt += (int64_t)bsz[k] * nmax;
+    if (k < nb - 1) cnt += (int64_t)bsz[k] * nmax;
+  }
+  std::vector<int> cur(nb);
+  for (int k = 0; k < nb; ++k) cur[k] = k;
+  while (cur.size() > 1) {
+    const int T_ = (int)cur.size();
+    for (int t = 0; t < T_; t += 2) {
+      if (t - 1 >= 0) cnt += (int64_t)bsz[cur[t]] * bsz[cur[t - 1]];
+      if (t + 1 <= T_ - 1) cnt += (int64_t)bsz[cur[t]] * bsz[cur[t + 1]];
+    }
+    std::vector<int> nx;
+    for (int t = 0; t < T_; t += 2) nx.push_back(cur[t]);
+    cur.swap(nx);
+  }
+  return cnt * (int64_t)sizeof(cx<T>) + 2 * n * MAXR * (int64_t)sizeof(cx<T>) +
+         (int64_t)nb * (nmax + 1) * 4 + 3 * nb * 8 + 4096;
+}
+
+template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
+  kore_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  {
+    Timed tm(c, KORE_TIMER_OTHER);
+    HIP_TRY(launch_zero<T>(F->mem.template as<cx<T>>(), F->ndlu, st));
+    HIP_TRY(launch_build_blocks<T>(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_blk.as<int32_t>(),
+                                   s->d_boff.as<int64_t>(), F->dD, F->dL, F->dU, st));
+  }
+  for (int l = 0; l < F->nlev; ++l) {
+    Level<T>& L = F->lev[l];
+    {
+      Timed tm(c, KORE_TIMER_INV);
+      HIP_TRY(launch_gj_inverse<T>(L.inv, L.ninv, L.maxn_inv, st));
+    }
+    {
+      Timed tm(c, KORE_TIMER_GEMM);
+      HIP_TRY(launch_gemm_batched<T>(L.g1, L.ng1, L.t1, st));
+    }
+    {
+      Timed tm(c, KORE_TIMER_GEMM);
+      HIP_TRY(launch_gemm_batched<T>(L.g2, L.ng2, L.t2, st));
+    }
+  }
+  {
+    Timed tm(c, KORE_TIMER_INV);
+    HIP_TRY(launch_gj_inverse<T>(F->root_inv, 1, F->root_n, st));
+  }
+  return KORE_OK;
+}
+
+template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs) {
+  kore_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  Timed tm(c, KORE_TIMER_SKINNY);
+  for (int l = 0; l < F->nlev; ++l) {
+    Level<T>& L = F->lev[l];
+    HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
+  }
+  HIP_TRY(launch_skinny<T>(F->root_solve, 1, F->root_n, nrhs, st));
+  for (int l = F->nlev - 1; l >= 0; --l) {
+    Level<T>& L = F->lev[l];
+    HIP_TRY(launch_skinny<T>(L.bw1, L.nbw1, L.mbw1, nrhs, st));
+    HIP_TRY(launch_skinny<T>(L.bw2, L.nbw2, L.mbw2, nrhs, st));
+  }
+  return KORE_OK;
+}
+
+// column 2-norms of a complex128 block (degree-major, ld = n) -> host
+static int32_t colnorms(kore_sys* s, const cx<double>* x, int64_t ld, int nrhs, double* out) {
+  kore_ctx* c = s->ctx;
+  HIP_TRY(launch_colnorm2(x, ld, s->n, nrhs, s->part, s->npart, c->stream));
+  std::vector<double> h((size_t)s->npart * nrhs);
+  HIP_TRY(hipMemcpyAsync(h.data(), s->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int j = 0; j < nrhs; ++j) {
+    double a = 0;
+    for (int b = 0; b < s->npart; ++b) a += h[(size_t)b * nrhs + j];
+    out[j] = std::sqrt(a);
+  }
+  return KORE_OK;
+}
+
+// Solve (A - sigma B) x = b for nrhs <= 16 columns; b, x complex128 on the
+// device in degree-major order (leading dimensions ldb, ldx).
+static int32_t solve_dev(kore_sys* s, const cx<double>* b, int64_t ldb, cx<double>* x, int64_t ldx, int nrhs,
+                         double ir_tol, int ir_maxit, kore_solve_info* info) {
+  kore_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  const int64_t n = s->n;
+  const double* scale = s->d_scale.as<double>();
+  if (!s->f) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
+  if (nrhs < 1 || nrhs > MAXR) return fail(KORE_ERR_ARG, "nrhs must be 1..16 per device solve");
+  if (info) {
+    info->iterations = 0;
+    info->converged = 1;
+    info->backward_error = -1.0;
+  }
+  const bool c64 = s->f->prec == KORE_PREC_C64;
+  const bool refine = c64 || ir_maxit > 0;
+  std::vector<double> nb_(nrhs), nx_(nrhs), nr_(nrhs);
+  if (refine) {
+    HIP_TRY(launch_gather_scale<double, double>(b, ldb, s->st, n, n, nrhs, nullptr, scale, 1.0, st));
+    TRY(colnorms(s, s->st, n, nrhs, nb_.data()));
+  }
+  const int maxsweep = c64 ? std::max(1, ir_maxit) : 1 + std::max(0, ir_maxit);
+  double eta_prev = 1e300;
+  int it = 0;
+  for (it = 1; it <= maxsweep; ++it) {
+    const cx<double>* rhs = (it == 1) ? b : s->rr;
+    const int64_t ldr = (it == 1) ? ldb : n;
+    if (c64) {
+      Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
+      {
+        Timed tm(c, KORE_TIMER_OTHER);
+        HIP_TRY(launch_gather_scale<double, float>(rhs, ldr, F->Bv, n, n, nrhs, nullptr, scale, 1.0, st));
+      }
+      TRY(bcr_solve<float>(s, F, nrhs));
+      Timed tm(c, KORE_TIMER_OTHER);
+      if (it == 1) HIP_TRY(launch_gather_scale<float, double>(F->Xv, n, x, ldx, n, nrhs, nullptr, nullptr, 1.0, st));
+      else HIP_TRY(launch_axpy<float>(x, ldx, F->Xv, n, n, nrhs, 1.0, st));
+    } else {
+      Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
+      {
+        Timed tm(c, KORE_TIMER_OTHER);
+        HIP_TRY(launch_gather_scale<double, double>(rhs, ldr, F->Bv, n, n, nrhs, nullptr, scale, 1.0, st));
+      }
+      TRY(bcr_solve<double>(s, F, nrhs));
+      Timed tm(c, KORE_TIMER_OTHER);
+      if (it == 1) HIP_TRY(launch_gather_scale<double, double>(F->Xv, n, x, ldx, n, nrhs, nullptr, nullptr, 1.0, st));
+      else HIP_TRY(launch_axpy<double>(x, ldx, F->Xv, n, n, nrhs, 1.0, st));
+    }
+    if (!refine) break;
+    // r = b - (A - sigma B) x ; eta = ||R r|| / (||RM|| ||x|| + ||R b||)
+    {
+      Timed tm(c, KORE_TIMER_SPMV);
+      HIP_TRY(launch_copy_cols(b, ldb, s->rr, n, n, nrhs, st));
+      HIP_TRY(launch_spmv(s->csr, -1.0, 0.0, s->sre, s->sim, x, ldx, s->rr, n, nrhs, 1.0, st));
+      HIP_TRY(launch_gather_scale<double, double>(s->rr, n, s->st, n, n, nrhs, nullptr, scale, 1.0, st));
+    }
+    TRY(colnorms(s, s->st, n, nrhs, nr_.data()));
+    TRY(colnorms(s, x, ldx, nrhs, nx_.data()));
+    double eta = 0;
+    for (int j = 0; j < nrhs; ++j) {
+      const double den = s->normRM * nx_[j] + nb_[j];
+      eta = std::max(eta, den > 0 ? nr_[j] / den : 0.0);
+    }
+    if (info) info->backward_error = eta;
+    if (eta <= ir_tol) break;
+    if (it >= 2 && eta > 0.5 * eta_prev) {  // stalled
+      if (info) info->converged = 0;
+      break;
+    }
+    eta_prev = eta;
+    if (it == maxsweep && info) info->converged = 0;
+  }
+  if (info) info->iterations = std::min(it, maxsweep);
+  return KORE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t kore_version(void) { return 100; }
+const char* kore_last_error(void) { return g_err.c_str(); }
+
+int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out) {
+  if (!out) return fail(KORE_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(KORE_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  auto c = std::make_unique<kore_ctx>();
+  c->device = device;
+  c->budget = mem_budget_bytes;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return KORE_OK;
+}
+
+void kore_ctx_destroy(kore_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  resolve_timers(ctx);
+  for (auto e : ctx->pool) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int32_t kore_ctx_sync(kore_ctx* ctx) {
+  if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KORE_OK;
+}
+
+int64_t kore_ctx_bytes_in_use(const kore_ctx* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int32_t kore_ctx_set_timing(kore_ctx* ctx, int32_t enable) {
+  if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
+  resolve_timers(ctx);
+  ctx->timing = enable != 0;
+  return KORE_OK;
+}
+
+int32_t kore_ctx_timing(kore_ctx* ctx, int32_t cls, int64_t* count, double* total_ms) {
+  if (!ctx || cls < 0 || cls >= KORE_TIMER_COUNT) return fail(KORE_ERR_ARG, "bad timing query");
+  resolve_timers(ctx);
+  if (count) *count = ctx->cnt[cls];
+  if (total_ms) *total_ms = ctx->tot_ms[cls];
+  return KORE_OK;
+}
+
+int32_t kore_ctx_timing_reset(kore_ctx* ctx) {
+  if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
+  resolve_timers(ctx);
+  for (int i = 0; i < KORE_TIMER_COUNT; ++i) {
+    ctx->cnt[i] = 0;
+    ctx->tot_ms[i] = 0;
+  }
+  return KORE_OK;
+}
+
+int32_t kore_layout_blocks(const kore_layout* lay, int64_t* sizmat, int32_t* nblk, int32_t* blk_sizes,
+                           int32_t blk_cap, int64_t* perm) {
+  std::vector<int> sizes;
+  std::vector<int64_t> p;
+  int64_t n = 0;
+  TRY(layout_build(lay, &n, &sizes, perm ? &p : nullptr));
+  if (sizmat) *sizmat = n;
+  if (nblk) *nblk = (int32_t)sizes.size();
+  if (blk_sizes) {
+    if (blk_cap < (int32_t)sizes.size()) return fail(KORE_ERR_ARG, "blk_sizes capacity too small");
+    for (size_t i = 0; i < sizes.size(); ++i) blk_sizes[i] = sizes[i];
+  }
+  if (perm) memcpy(perm, p.data(), p.size() * sizeof(int64_t));
+  return KORE_OK;
+}
+
+int64_t kore_factor_bytes(const kore_layout* lay, int32_t precision) {
+  std::vector<int> sizes;
+  int64_t n = 0;
+  if (layout_build(lay, &n, &sizes, nullptr) < 0) return -1;
+  return precision == KORE_PREC_C64 ? factor_count_bytes<float>(sizes, n) : factor_count_bytes<double>(sizes, n);
+}
+
+int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr,
+                        const int32_t* a_indices, const double* a_data, const int32_t* b_indptr,
+                        const int32_t* b_indices, const double* b_data, int32_t b_is_complex, kore_sys** out) {
+  if (!ctx || !lay || !out || !a_indptr || !a_indices || !a_data) return fail(KORE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto s = std::make_unique<kore_sys>();
+  s->ctx = ctx;
+  s->lay = *lay;
+  int64_t n = 0;
+  TRY(layout_build(lay, &n, &s->bsz, &s->perm));
+  if (n != sizmat) return fail(KORE_ERR_ARG, "matrix size %lld does not match the layout (%lld)", (long long)sizmat, (long long)n);
+  s->n = n;
+  s->nblk = (int)s->bsz.size();
+  s->boff.assign(s->nblk + 1, 0);
+  for (int k = 0; k < s->nblk; ++k) s->boff[k + 1] = s->boff[k] + s->bsz[k];
+  s->nmax = *std::max_element(s->bsz.begin(), s->bsz.end());
+  if (s->nmax > 512) return fail(KORE_ERR_UNSUPPORTED, "degree block of %d unknowns exceeds the 512 supported", s->nmax);
+  s->h_blk.resize(n);
+  for (int k = 0; k < s->nblk; ++k)
+    for (int64_t i = s->boff[k]; i < s->boff[k + 1]; ++i) s->h_blk[i] = k;
+  std::vector<int64_t> inv(n);
+  for (int64_t i = 0; i < n; ++i) inv[s->perm[i]] = i;
+  // validate CSR
+  auto check_csr = [&](const int32_t* ip, const int32_t* ix, const char* nm) -> int32_t {
+    if (ip[0] != 0) return fail(KORE_ERR_ARG, "%s: indptr[0] != 0", nm);
+    for (int64_t r = 0; r < n; ++r) {
+      if (ip[r + 1] < ip[r]) return fail(KORE_ERR_ARG, "%s: indptr not monotone at row %lld", nm, (long long)r);
+    }
+    const int64_t nnz = ip[n];
+    for (int64_t k = 0; k < nnz; ++k)
+      if (ix[k] < 0 || ix[k] >= n) return fail(KORE_ERR_ARG, "%s: column index out of range", nm);
+    return KORE_OK;
+  };
+  TRY(check_csr(a_indptr, a_indices, "A"));
+  s->has_b = b_indptr != nullptr;
+  if (s->has_b) {
+    if (!b_indices || !b_data) return fail(KORE_ERR_ARG, "B arrays incomplete");
+    TRY(check_csr(b_indptr, b_indices, "B"));
+  }
+  // union pattern in degree-major order
+  s->h_ip.assign(n + 1, 0);
+  std::vector<std::pair<int32_t, std::pair<zd, zd>>> row;
+  s->h_ix.reserve((size_t)a_indptr[n] + (s->has_b ? b_indptr[n] : 0));
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t r = s->perm[i];
+    row.clear();
+    for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
+      row.push_back({(int32_t)inv[a_indices[k]], {zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)}});
+    if (s->has_b)
+      for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) {
+        const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
+        row.push_back({(int32_t)inv[b_indices[k]], {zd(0, 0), bv}});
+      }
+    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    const int kb = s->h_blk[i];
+    for (size_t q = 0; q < row.size();) {
+      int32_t col = row[q].first;
+      zd a = 0, b = 0;
+      while (q < row.size() && row[q].first == col) {
+        a += row[q].second.first;
+        b += row[q].second.second;
+        ++q;
+      }
+      if (a == zd(0, 0) && b == zd(0, 0)) continue;
+      const int kc = s->h_blk[col];
+      if (kc < kb - 1 || kc > kb + 1)
+        return fail(KORE_ERR_UNSUPPORTED,
+                    "matrix is not block-tridiagonal in degree order (row %lld couples block %d to %d)",
+                    (long long)i, kb, kc);
+      s->h_ix.push_back(col);
+      s->h_va.push_back(a);
+      s->h_vb.push_back(b);
+    }
+    s->h_ip[i + 1] = (int32_t)s->h_ix.size();
+  }
+  const size_t nnz = s->h_ix.size();
+  TRY(s->d_ip.alloc(ctx, (n + 1) * sizeof(int32_t)));
+  TRY(s->d_ix.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(int32_t)));
+  TRY(s->d_va.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(zd)));
+  TRY(s->d_vb.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(zd)));
+  TRY(s->d_perm.alloc(ctx, n * sizeof(int64_t)));
+  TRY(s->d_blk.alloc(ctx, n * sizeof(int32_t)));
+  TRY(s->d_boff.alloc(ctx, (s->nblk + 1) * sizeof(int64_t)));
+  TRY(s->d_scale.alloc(ctx, n * sizeof(double)));
+  TRY(s->d_scr.alloc(ctx, 4 * (size_t)n * MAXR * sizeof(cx<double>)));
+  TRY(s->d_part.alloc(ctx, (size_t)s->npart * MAXR * sizeof(double)));
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(s->d_ip.p, s->h_ip.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (nnz) {
+    HIP_TRY(hipMemcpyAsync(s->d_ix.p, s->h_ix.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_va.p, s->h_va.data(), nnz * sizeof(zd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_vb.p, s->h_vb.data(), nnz * sizeof(zd), hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(s->d_perm.p, s->perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_blk.p, s->h_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_boff.p, s->boff.data(), (s->nblk + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  s->csr = CsrDev{n, s->d_ip.as<int32_t>(), s->d_ix.as<int32_t>(), s->d_va.as<cx<double>>(), s->d_vb.as<cx<double>>()};
+  cx<double>* scr = s->d_scr.as<cx<double>>();
+  s->bb = scr;
+  s->xr = scr + (size_t)n * MAXR;
+  s->rr = scr + 2 * (size_t)n * MAXR;
+  s->st = scr + 3 * (size_t)n * MAXR;
+  s->part = s->d_part.as<double>();
+  TRY(s->up.init(ctx, 1 << 20));
+  *out = s.release();
+  return KORE_OK;
+}
+
+void kore_sys_destroy(kore_sys* sys) {
+  if (!sys) return;
+  hipSetDevice(sys->ctx->device);
+  hipStreamSynchronize(sys->ctx->stream);
+  delete sys;
+}
+
+int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
+  if (!s || !out) return fail(KORE_ERR_ARG, "NULL argument");
+  const zd sg(sre, sim);
+  double acc = 0;
+  for (size_t k = 0; k < s->h_va.size(); ++k) acc += std::norm(s->h_va[k] - sg * s->h_vb[k]);
+  *out = std::sqrt(acc);
+  return KORE_OK;
+}
+
+int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore_factor_info* info) {
+  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
+  if (precision != KORE_PREC_FP64 && precision != KORE_PREC_C64) return fail(KORE_ERR_ARG, "bad precision");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  s->f.reset();
+  s->sre = sre;
+  s->sim = sim;
+  // power-of-two row equilibration of A - sigma B and ||R(A - sigma B)||_F
+  const zd sg(sre, sim);
+  s->h_scale.assign(s->n, 1.0);
+  double nrm = 0;
+  for (int64_t i = 0; i < s->n; ++i) {
+    double mx = 0;
+    for (int32_t k = s->h_ip[i]; k < s->h_ip[i + 1]; ++k) mx = std::max(mx, std::abs(s->h_va[k] - sg * s->h_vb[k]));
+    double sc = 1.0;
+    if (mx > 0 && std::isfinite(mx)) sc = std::ldexp(1.0, -(int)std::nearbyint(std::log2(mx)));
+    s->h_scale[i] = sc;
+    for (int32_t k = s->h_ip[i]; k < s->h_ip[i + 1]; ++k) nrm += sc * sc * std::norm(s->h_va[k] - sg * s->h_vb[k]);
+  }
+  s->normRM = std::sqrt(nrm);
+  HIP_TRY(hipMemcpyAsync(s->d_scale.p, s->h_scale.data(), s->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int32_t rc = KORE_OK;
+  std::vector<int> hinfo(s->nblk, 0);
+  if (precision == KORE_PREC_FP64) {
+    auto F = std::make_unique<Factor<double>>();
+    F->prec = precision;
+    rc = factor_build<double>(s, F.get());
+    if (rc >= 0) {
+      hipEventRecord(e0, c->stream);
+      rc = factor_run<double>(s, F.get());
+      hipEventRecord(e1, c->stream);
+    }
+    if (rc >= 0) {
+      hipMemcpyAsync(hinfo.data(), F->info, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      s->f = std::move(F);
+    }
+  } else {
+    auto F = std::make_unique<Factor<float>>();
+    F->prec = precision;
+    rc = factor_build<float>(s, F.get());
+    if (rc >= 0) {
+      hipEventRecord(e0, c->stream);
+      rc = factor_run<float>(s, F.get());
+      hipEventRecord(e1, c->stream);
+    }
+    if (rc >= 0) {
+      hipMemcpyAsync(hinfo.data(), F->info, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      s->f = std::move(F);
+    }
+  }
+  hipError_t se = hipStreamSynchronize(c->stream);
+  float ms = 0.f;
+  if (rc >= 0 && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc < 0) {
+    s->f.reset();
+    return rc;
+  }
+  if (se != hipSuccess) {
+    s->f.reset();
+    return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
+  }
+  for (int k = 0; k < s->nblk; ++k)
+    if (hinfo[k] != 0) {
+      s->f.reset();
+      return fail(KORE_ERR_SINGULAR, "zero pivot in degree block %d (column %d) during cyclic reduction", k, hinfo[k]);
+    }
+  if (info) {
+    info->levels = s->f->nlev;
+    info->nblk = s->nblk;
+    info->flops = s->f->flops;
+    info->bytes = s->f->bytes;
+    info->ms = ms;
+  }
+  return KORE_OK;
+}
+
+int32_t kore_factor_release(kore_sys* s) {
+  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
+  hipStreamSynchronize(s->ctx->stream);
+  s->f.reset();
+  return KORE_OK;
+}
+
+int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, double* x, int64_t ldx, double ir_tol,
+                   int32_t ir_maxit, kore_solve_info* info) {
+  if (!s || !b || !x || nrhs < 0 || ldb < s->n || ldx < s->n) return fail(KORE_ERR_ARG, "bad solve arguments");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int64_t n = s->n;
+  if (info) {
+    info->iterations = 0;
+    info->converged = 1;
+    info->backward_error = -1;
+  }
+  for (int c0 = 0; c0 < nrhs; c0 += MAXR) {
+    const int nr = std::min(MAXR, nrhs - c0);
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(s->st + (size_t)j * n, b + 2 * (ldb * (int64_t)(c0 + j)), n * sizeof(zd),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_gather_scale<double, double>(s->st, n, s->bb, n, n, nr, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                                c->stream));
+    kore_solve_info li{};
+    TRY(solve_dev(s, s->bb, n, s->xr, n, nr, ir_tol, ir_maxit, &li));
+    HIP_TRY(launch_scatter(s->xr, n, s->st, n, n, nr, s->d_perm.as<int64_t>(), c->stream));
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(x + 2 * (ldx * (int64_t)(c0 + j)), s->st + (size_t)j * n, n * sizeof(zd),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (info) {
+      info->iterations = std::max(info->iterations, li.iterations);
+      info->converged = info->converged && li.converged;
+      info->backward_error = std::max(info->backward_error, li.backward_error);
+    }
+  }
+  return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
+}
+
+// ---- workspace ---------------------------------------------------------------
+int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
+  if (!s || !out || ncol <= 0) return fail(KORE_ERR_ARG, "bad workspace arguments");
+  *out = nullptr;
+  auto w = std::make_unique<kore_ws>();
+  w->sys = s;
+  w->ncol = ncol;
+  kore_ctx* c = s->ctx;
+  TRY(w->V.alloc(c, (size_t)s->n * ncol * sizeof(zd)));
+  TRY(w->part.alloc(c, (size_t)64 * ncol * ncol * sizeof(zd) + (size_t)ncol * ncol * sizeof(zd)));
+  TRY(w->qbuf.alloc(c, (size_t)ncol * ncol * sizeof(zd)));
+  TRY(w->descb.alloc(c, 64 * sizeof(GemmDesc<double>)));
+  TRY(w->up.init(c, 1 << 22));
+  HIP_TRY(launch_zero<double>(w->V.as<cx<double>>(), (int64_t)s->n * ncol, c->stream));
+  *out = w.release();
+  return KORE_OK;
+}
+
+void kore_ws_destroy(kore_ws* w) {
+  if (!w) return;
+  hipStreamSynchronize(w->sys->ctx->stream);
+  delete w;
+}
+
+static inline cx<double>* wcol(kore_ws* w, int j) { return w->V.as<cx<double>>() + (size_t)j * w->sys->n; }
+
+static int32_t ws_check(kore_ws* w, int a, int na) {
+  if (!w) return fail(KORE_ERR_ARG, "workspace is NULL");
+  if (a < 0 || na < 0 || a + na > w->ncol) return fail(KORE_ERR_ARG, "column range [%d, %d) outside workspace of %d", a, a + na, w->ncol);
+  return KORE_OK;
+}
+
+int32_t kore_ws_set(kore_ws* w, int32_t c0, int32_t nc, const double* host, int64_t ld) {
+  TRY(ws_check(w, c0, nc));
+  kore_sys* s = w->sys;
+  const int64_t n = s->n;
+  if (ld < n) return fail(KORE_ERR_ARG, "ld < sizmat");
+  for (int a = 0; a < nc; a += MAXR) {
+    const int nr = std::min(MAXR, nc - a);
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(s->st + (size_t)j * n, host + 2 * ld * (int64_t)(a + j), n * sizeof(zd),
+                             hipMemcpyHostToDevice, s->ctx->stream));
+    HIP_TRY(launch_gather_scale<double, double>(s->st, n, wcol(w, c0 + a), n, n, nr, s->d_perm.as<int64_t>(), nullptr,
+                                                1.0, s->ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return KORE_OK;
+}
+
+int32_t kore_ws_get(kore_ws* w, int32_t c0, int32_t nc, double* host, int64_t ld) {
+  TRY(ws_check(w, c0, nc));
+  kore_sys* s = w->sys;
+  const int64_t n = s->n;
+  if (ld < n) return fail(KORE_ERR_ARG, "ld < sizmat");
+  for (int a = 0; a < nc; a += MAXR) {
+    const int nr = std::min(MAXR, nc - a);
+    HIP_TRY(launch_scatter(wcol(w, c0 + a), n, s->st, n, n, nr, s->d_perm.as<int64_t>(), s->ctx->stream));
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(host + 2 * ld * (int64_t)(a + j), s->st + (size_t)j * n, n * sizeof(zd),
+                             hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  }
+  return KORE_OK;
+}
+
+int32_t kore_ws_apply_op(kore_ws* w, int32_t src, int32_t dst, int32_t nc, double ir_tol, int32_t ir_maxit,
+                         kore_solve_info* info) {
+  TRY(ws_check(w, src, nc));
+  TRY(ws_check(w, dst, nc));
+  kore_sys* s = w->sys;
+  if (!s->has_b) return fail(KORE_ERR_STATE, "system has no B matrix");
+  if (src < dst + nc && dst < src + nc) return fail(KORE_ERR_ARG, "source and destination overlap");
+  const int64_t n = s->n;
+  if (info) {
+    info->iterations = 0;
+    info->converged = 1;
+    info->backward_error = -1;
+  }
+  for (int a = 0; a < nc; a += MAXR) {
+    const int nr = std::min(MAXR, nc - a);
+    {
+      Timed tm(s->ctx, KORE_TIMER_SPMV);
+      HIP_TRY(launch_spmv(s->csr, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
+    }
+    kore_solve_info li{};
+    TRY(solve_dev(s, s->bb, n, wcol(w, dst + a), n, nr, ir_tol, ir_maxit, &li));
+    if (info) {
+      info->iterations = std::max(info->iterations, li.iterations);
+      info->converged = info->converged && li.converged;
+      info->backward_error = std::max(info->backward_error, li.backward_error);
+    }
+  }
+  return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
+}
+
+int32_t kore_ws_spmv(kore_ws* w, int32_t src, int32_t dst, int32_t nc, double ca_re, double ca_im, double cb_re,
+                     double cb_im) {
+  TRY(ws_check(w, src, nc));
+  TRY(ws_check(w, dst, nc));
+  if (src < dst + nc && dst < src + nc) return fail(KORE_ERR_ARG, "source and destination overlap");
+  kore_sys* s = w->sys;
+  Timed tm(s->ctx, KORE_TIMER_SPMV);
+  HIP_TRY(launch_spmv(s->csr, ca_re, ca_im, cb_re, cb_im, wcol(w, src), s->n, wcol(w, dst), s->n, nc, 0.0,
+                      s->ctx->stream));
+  return KORE_OK;
+}
+
+int32_t kore_ws_dot(kore_ws* w, int32_t a0, int32_t na, int32_t b0, int32_t nb, double* H) {
+  TRY(ws_check(w, a0, na));
+  TRY(ws_check(w, b0, nb));
+  if (!H) return fail(KORE_ERR_ARG, "H is NULL");
+  kore_sys* s = w->sys;
+  kore_ctx* c = s->ctx;
+  const int64_t n = s->n;
+  int nchunk = (int)std::min<int64_t>(64, (n + 4095) / 4096);
+  if (nchunk < 1) nchunk = 1;
+  const int64_t chunk = (n + nchunk - 1) / nchunk;
+  std::vector<GemmDesc<double>> ds;
+  cx<double>* part = w->part.as<cx<double>>();
+  cx<double>* res = part + (size_t)64 * w->ncol * w->ncol;
+  for (int q = 0; q < nchunk; ++q) {
+    const int64_t k0 = q * chunk;
+    const int64_t kl = std::min<int64_t>(chunk, n - k0);
+    if (kl <= 0) break;
+    GemmDesc<double> g{};
+    g.A[0] = wcol(w, a0) + k0; g.lda[0] = n;
+    g.B[0] = wcol(w, b0) + k0; g.ldb[0] = n;
+    g.k[0] = (int)kl;
+    g.C = part + (size_t)q * na * nb; g.ldc = na;
+    g.m = na; g.n = nb; g.opA = 1;
+    g.tiles_m = (na + 63) / 64; g.tiles_n = (nb + 63) / 64;
+    g.alpha = 1.0; g.beta = 0.0;
+    ds.push_back(g);
+  }
+  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>();
+  TRY(w->up.put(dd, ds.data(), ds.size() * sizeof(GemmDesc<double>)));
+  {
+    Timed tm(c, KORE_TIMER_GEMM);
+    HIP_TRY(launch_gemm_batched<double>(dd, (int)ds.size(), ds[0].tiles_m * ds[0].tiles_n, c->stream));
+    HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), (int)ds.size(), 2 * na * nb,
+                             reinterpret_cast<double*>(res), c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(H, res, (size_t)na * nb * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return KORE_OK;
+}
+
+int32_t kore_ws_gemm(kore_ws* w, int32_t d0, int32_t nd, int32_t s0, int32_t ns, const double* Q, double alpha,
+                     double beta) {
+  TRY(ws_check(w, d0, nd));
+  TRY(ws_check(w, s0, ns));
+  if (!Q) return fail(KORE_ERR_ARG, "Q is NULL");
+  if (s0 < d0 + nd && d0 < s0 + ns) return fail(KORE_ERR_ARG, "source and destination overlap");
+  if (nd == 0) return KORE_OK;
+  kore_sys* s = w->sys;
+  kore_ctx* c = s->ctx;
+  const int64_t n = s->n;
+  cx<double>* q = w->qbuf.as<cx<double>>();
+  TRY(w->up.put(q, Q, (size_t)ns * nd * sizeof(zd)));
+  GemmDesc<double> g{};
+  g.A[0] = wcol(w, s0); g.lda[0] = n;
+  g.B[0] = q; g.ldb[0] = ns;
+  g.k[0] = ns;
+  g.C = wcol(w, d0); g.ldc = n;
+  g.m = (int)n; g.n = nd; g.opA = 0;
+  g.tiles_m = (int)((n + 63) / 64); g.tiles_n = (nd + 63) / 64;
+  g.alpha = alpha; g.beta = beta;
+  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>() + 63;
+  TRY(w->up.put(dd, &g, sizeof(g)));
+  Timed tm(c, KORE_TIMER_GEMM);
+  HIP_TRY(launch_gemm_batched<double>(dd, 1, std::min(g.tiles_m * g.tiles_n, 65535), c->stream));
+  return KORE_OK;
+}
+
+int32_t kore_ws_copy(kore_ws* w, int32_t src, int32_t dst, int32_t nc) {
+  TRY(ws_check(w, src, nc));
+  TRY(ws_check(w, dst, nc));
+  kore_sys* s = w->sys;
+  HIP_TRY(launch_copy_cols(wcol(w, src), s->n, wcol(w, dst), s->n, s->n, nc, s->ctx->stream));
+  return KORE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as ss
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def load_ops_fixture(name):
+    """Golden operator fixture written by tests/golden/make_fixtures.py."""
+    d = np.load(os.path.join(GOLDEN, "ops_%s.npz" % name))
+    meta = json.loads(str(d["meta"]))
+
+    def get(key):
+        return ss.csr_matrix((d[key + "__data"], d[key + "__indices"], d[key + "__indptr"]),
+                             shape=tuple(d[key + "__shape"]))
+
+    keys = sorted(set(k.split("__")[0] for k in d.files if "__" in k))
+    return d, meta, get, keys
+
+
+# Reference golden (tests/spinover/reference.eig:1 of Kore): lambda = re + i im
+SPINOVER_LAMBDA = complex(-1.019947097016500742e-01, 1.005733690103750355e+00)
+
+
+def params_config1():
+    from kore_amd.params import Params
+    # hydrodynamic inertial modes: m=1, E=1e-4, ricb=0.35, no-slip, symmetric, N=24, lmax=40
+    return Params(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=40, tau=1.0j, nev=5, tol=1e-12)
+
+
+def params_spinover():
+    from kore_amd.params import Params
+    return Params()     # bin/parameters.py defaults = tests/spinover/params.spinover
+
+
+@pytest.fixture(scope="session")
+def config1():
+    from kore_amd import assembly
+    par = params_config1()
+    A, B, _ = assembly.assemble(par)
+    return par, A, B
+
+
+@pytest.fixture(scope="session")
+def spinover():
+    from kore_amd import assembly
+    par = params_spinover()
+    A, B, _ = assembly.assemble(par)
+    return par, A, B
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False

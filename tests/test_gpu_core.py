@@ -1,0 +1,143 @@
+"""Device parity tests of the HIP path against the oracle (MI355X only).
+
+Tolerances: FP64 path and complex64+FP64-refinement path must reproduce the
+oracle's solutions to relative error <= 1e-10 and eigenvalues to <= 1e-10
+(the bar of the north-star metric); the reference's spin-over golden is
+checked at its own rtol 1e-8 (tests/test_spinover.py:21-29 of Kore).
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as ss
+import scipy.sparse.linalg as sla
+
+from conftest import SPINOVER_LAMBDA
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from kore_amd import backend
+    c = backend.Context(0)
+    yield c
+    c.close()
+
+
+def _system(ctx, par, A, B):
+    from kore_amd import backend
+    lay = backend.layout_from_params(par)
+    return backend.System(ctx, lay, A, B)
+
+
+def test_ws_gemm_and_dot(ctx, config1):
+    """GEMM kernel (both operand modes) through the workspace primitives."""
+    from kore_amd import backend
+    par, A, B = config1
+    say("case: ws gemm/dot, sizmat", A.shape[0])
+    S = _system(ctx, par, A, B)
+    ws = backend.Workspace(S, 40)
+    rng = np.random.default_rng(1)
+    n = S.n
+    V = rng.standard_normal((n, 20)) + 1j * rng.standard_normal((n, 20))
+    ws.set(0, V)
+    assert np.allclose(ws.get(0, 20), V, rtol=0, atol=0)
+    H = ws.dot(0, 20, 3, 5)
+    ref = V.conj().T @ V[:, 3:8]
+    assert np.abs(H - ref).max() <= 1e-12 * np.abs(ref).max()
+    Q = rng.standard_normal((20, 7)) + 1j * rng.standard_normal((20, 7))
+    ws.gemm(25, 7, 0, 20, Q, alpha=1.0, beta=0.0)
+    ref = V @ Q
+    got = ws.get(25, 7)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    ws.gemm(25, 7, 0, 20, Q, alpha=-1.0, beta=1.0)
+    assert np.abs(ws.get(25, 7)).max() <= 1e-12 * np.abs(ref).max()
+    ws.close()
+    S.close()
+
+
+def _solve_case(ctx, par, A, B, sigma, precision, nrhs=1, seed=0):
+    from kore_amd import backend
+    S = _system(ctx, par, A, B)
+    S.factor(sigma, precision)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((S.n, nrhs)) + 1j * rng.standard_normal((S.n, nrhs))
+    b /= np.linalg.norm(b, axis=0)
+    x, info = S.solve(b, ir_tol=1e-14, ir_maxit=20)
+    M = (A - sigma * B).tocsc()
+    xr = sla.splu(M).solve(b)
+    err = np.linalg.norm(x - xr, axis=0) / np.linalg.norm(xr, axis=0)
+    S.close()
+    return err, info
+
+
+def test_solve_tiny_fp64(ctx):
+    from kore_amd.params import Params
+    from kore_amd import assembly
+    par = Params(m=1, symm=1, Ek=1e-3, N=12, lmax=8)
+    A, B, _ = assembly.assemble(par)
+    say("case: tiny fp64 N=12 lmax=8 sizmat", A.shape[0])
+    err, info = _solve_case(ctx, par, A, B, 0.3j, 0)
+    say("  rel err", err)
+    assert err.max() <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_config1(ctx, config1, precision):
+    par, A, B = config1
+    say("case: config1 solve precision", precision)
+    err, info = _solve_case(ctx, par, A, B, 1.0j, precision, nrhs=3)
+    say("  rel err", err, "IR iterations", info.iterations, "backward error", info.backward_error)
+    assert err.max() <= 1e-10
+
+
+def test_solve_many_rhs(ctx, config1):
+    par, A, B = config1
+    say("case: config1 solve 21 rhs fp64")
+    err, _ = _solve_case(ctx, par, A, B, 0.5 + 0.7j, 0, nrhs=21, seed=3)
+    assert err.max() <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_eigs_config1(ctx, config1, precision):
+    from kore_amd import eigs
+    from oracle import krylov_schur
+    par, A, B = config1
+    say("case: config1 eigs precision", precision)
+    S = _system(ctx, par, A, B)
+    S.factor(par.tau, precision)
+    r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=4)
+    say("  nconv", r.nconv, "restarts", r.restarts, "ops", r.op_columns)
+    lu = sla.splu((A - par.tau * B).tocsc())
+    lam_o, X_o, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], par.tau, 5, tol=1e-13,
+                                                  maxit=100)
+    assert r.nconv >= 5
+    for lo in lam_o[:5]:
+        d = np.min(np.abs(r.lam - lo)) / abs(lo)
+        say("  lambda", lo, "rel diff", d)
+        assert d <= 1e-10
+    res = eigs.residuals(A, B, r.lam[:5], r.X[:, :5])
+    assert res.max() <= 1e-10
+    S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_spinover_golden(ctx, spinover, precision):
+    from kore_amd import eigs
+    par, A, B = spinover
+    say("case: spinover eigs precision", precision)
+    S = _system(ctx, par, A, B)
+    S.factor(par.tau, precision)
+    r = eigs.eigs(S, par.tau, nev=par.nev, tol=1e-13, maxit=100, block=4)
+    i = np.argmax(r.lam.real)
+    got = r.lam[i]
+    say("  spin-over lambda", got)
+    np.testing.assert_allclose([got.real, got.imag], [SPINOVER_LAMBDA.real, SPINOVER_LAMBDA.imag], rtol=1e-8,
+                               atol=1e-20)
+    S.close()

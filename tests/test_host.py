@@ -1,0 +1,149 @@
+"""CPU tests of the host side: the C-ABI library (load, exports, host-only
+layout entry points) and the block Krylov-Schur driver's logic, run against
+a numpy stand-in of the device workspace (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.linalg as la
+import scipy.sparse.linalg as sla
+
+from conftest import ROOT
+
+from kore_amd import assembly, backend, eigs
+from kore_amd.params import Params
+from oracle import layout
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "kore_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(kore_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = backend.lib()
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    bound = {s[0] for s in backend.SIGNATURES}
+    assert set(names) == bound
+
+
+@pytest.mark.parametrize("m,lmax,symm,N1,kw", [
+    (1, 40, 1, 24, {}), (1, 64, -1, 68, {}), (0, 9, 1, 10, {}), (3, 10, -1, 6, {}),
+    (2, 17, 1, 16, dict(magnetic=1, thermal=1)), (2, 17, 1, 16, dict(magnetic=1, thermal=1, compositional=1)),
+])
+def test_library_layout_matches_oracle(m, lmax, symm, N1, kw):
+    lay = backend.make_layout(m, lmax, symm, N1, **kw)
+    n, sizes, perm = backend.layout_blocks(lay)
+    po, so = layout.lmajor_permutation(m, lmax, symm, N1, **kw)
+    np.testing.assert_array_equal(perm, po)
+    np.testing.assert_array_equal(sizes, so)
+
+
+def test_library_rejects_bad_layout():
+    lay = backend.make_layout(2, 200, 1, 150)        # lmax - m + 1 odd
+    with pytest.raises(backend.KoreError):
+        backend.layout_blocks(lay)
+
+
+def test_factor_bytes_prediction():
+    small = backend.factor_bytes(backend.make_layout(1, 40, 1, 24), 0)
+    big = backend.factor_bytes(backend.make_layout(1, 300, 1, 200), 0)
+    c64 = backend.factor_bytes(backend.make_layout(1, 300, 1, 200), 1)
+    assert 0 < small < big
+    assert 0.45 < c64 / big < 0.55
+
+
+class NumpyWorkspace:
+    """Stand-in for backend.Workspace: same methods, numpy arrays, OP by SuperLU."""
+
+    def __init__(self, sysm, ncol):
+        self.sys = sysm
+        self.n = sysm.n
+        self.V = np.zeros((sysm.n, ncol), dtype=complex)
+
+    def set(self, c0, X):
+        X = np.asarray(X).reshape(self.n, -1)
+        self.V[:, c0:c0 + X.shape[1]] = X
+
+    def get(self, c0, nc):
+        return self.V[:, c0:c0 + nc].copy()
+
+    def apply_op(self, src, dst, nc, ir_tol=0, ir_maxit=0):
+        assert not (src < dst + nc and dst < src + nc)
+        self.V[:, dst:dst + nc] = self.sys.lu.solve(self.sys.B @ self.V[:, src:src + nc])
+
+        class I:
+            iterations = 1
+        return I()
+
+    def dot(self, a0, na, b0, nb):
+        return self.V[:, a0:a0 + na].conj().T @ self.V[:, b0:b0 + nb]
+
+    def gemm(self, d0, nd, s0, ns, Q, alpha=1.0, beta=0.0):
+        assert not (s0 < d0 + nd and d0 < s0 + ns)
+        self.V[:, d0:d0 + nd] = beta * self.V[:, d0:d0 + nd] + alpha * self.V[:, s0:s0 + ns] @ Q
+
+    def copy(self, src, dst, nc):
+        self.V[:, dst:dst + nc] = self.V[:, src:src + nc]
+
+    def close(self):
+        pass
+
+
+class NumpySystem:
+    def __init__(self, A, B, sigma):
+        self.n = A.shape[0]
+        self.B = B
+        self.lu = sla.splu((A - sigma * B).tocsc())
+        self.precision = backend.PREC_FP64
+
+
+@pytest.mark.parametrize("block", [1, 3, 4, 8])
+def test_block_krylov_schur_logic_vs_qz(block):
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.9j
+    w = la.eig(A.toarray(), B.toarray(), right=False)
+    w = w[np.isfinite(w)]
+    w = w[np.argsort(np.abs(w - sigma))]
+    S = NumpySystem(A, B, sigma)
+    r = eigs.eigs(S, sigma, nev=6, tol=1e-13, block=block, workspace=NumpyWorkspace)
+    assert r.nconv >= 6
+    for i in range(6):
+        assert abs(r.lam[i] - w[i]) / abs(w[i]) <= 1e-10
+    res = eigs.residuals(A, B, r.lam[:6], r.X[:, :6])
+    assert res.max() <= 1e-13
+
+
+@pytest.mark.parametrize("which", ["TR", "TI"])
+def test_block_krylov_schur_other_targets(which):
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = -0.05 + 0.5j
+    w = la.eig(A.toarray(), B.toarray(), right=False)
+    w = w[np.isfinite(w)]
+    key = eigs.sort_key(which, w, sigma)
+    S = NumpySystem(A, B, sigma)
+    r = eigs.eigs(S, sigma, nev=3, tol=1e-12, block=4, which=which, workspace=NumpyWorkspace)
+    assert r.nconv >= 3
+    # every returned eigenvalue is a true eigenvalue of the pencil
+    for lam in r.lam[:3]:
+        assert np.min(np.abs(w - lam)) / abs(lam) <= 1e-9
+
+
+def test_restart_path_is_exercised():
+    """Small subspace forces thick restarts; the result must not change."""
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.9j
+    S = NumpySystem(A, B, sigma)
+    r1 = eigs.eigs(S, sigma, nev=4, ncv=12, tol=1e-12, block=2, workspace=NumpyWorkspace)
+    r2 = eigs.eigs(S, sigma, nev=4, ncv=40, tol=1e-12, block=4, workspace=NumpyWorkspace)
+    assert r1.restarts >= 1
+    for i in range(4):
+        assert abs(r1.lam[i] - r2.lam[i]) / abs(r2.lam[i]) <= 1e-10

@@ -1,0 +1,214 @@
+"""CPU tests: the oracle and the assembly pinned against the reference's goldens.
+
+Pins (what a plausible mistake would break):
+* radial operators and every per-degree block vs the reference's own
+  submatrices.py / operators.py outputs (tests/golden/ops_*.npz), rtol 1e-13;
+* boundary rows vs bc_variables.Ta/Tb and the textbook values
+  T_k(+-1) = (+-1)^k, T_k'(+-1) = (+-1)^(k+1) k^2 dx/dr;
+* the assembled pencil end-to-end vs the reference's spin-over golden
+  eigenvalue (tests/spinover/reference.eig:1), rtol 1e-8 as in
+  tests/test_spinover.py:21-29 of Kore;
+* the oracle solves (block Thomas, cyclic reduction) vs SuperLU and the
+  residual bound; Krylov-Schur vs dense QZ (LAPACK zggev);
+* layout invariants (bijection, block-tridiagonality) and A(w) = A0 - i w B0.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as la
+import scipy.sparse as ss
+import scipy.sparse.linalg as sla
+
+from conftest import SPINOVER_LAMBDA, load_ops_fixture, params_config1, params_spinover
+
+from kore_amd import assembly
+from kore_amd.params import Params, ell as kell
+from oracle import bcr, blocktri, krylov_schur, layout
+
+FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_config1), ("spinover", params_spinover)]
+
+
+@pytest.mark.parametrize("name,mkpar", FIXTURES)
+def test_radial_operators_match_reference(name, mkpar):
+    d, meta, get, keys = load_ops_fixture(name)
+    par = mkpar()
+    assert (meta["N"], meta["lmax"], meta["m"], meta["symm"]) == (par.N, par.lmax, par.m, par.symm)
+    R = assembly.RadialOps(par.N, par.ricb)
+    n = 0
+    for k in keys:
+        if not k.startswith("mtx:"):
+            continue
+        lab = k[4:]
+        p, dd, g = int(lab[1]), int(lab[4]), 4 if lab.endswith("_u") else 2
+        ref = get(k).toarray()
+        mine = R.op(p, dd, g).toarray()
+        assert np.abs(ref - mine).max() <= 1e-13 * np.abs(ref).max(), lab
+        n += 1
+    assert n >= 15
+
+
+@pytest.mark.parametrize("name,mkpar", FIXTURES)
+def test_degree_blocks_match_reference(name, mkpar):
+    d, meta, get, keys = load_ops_fixture(name)
+    par = mkpar()
+    R = assembly.RadialOps(par.N, par.ricb)
+    n = 0
+    for k in keys:
+        parts = k.split(":")
+        if parts[0] == "mtx":
+            continue
+        kind, sec, l = parts[0], parts[1], int(parts[-1])
+        if kind in ("u", "visc"):
+            term = kind + ":" + sec
+        elif kind == "cori":
+            off = int(parts[3])
+            term = "cori:" + sec + ("" if off == 0 else (":-1" if off < 0 else ":+1"))
+        else:
+            continue
+        ref = get(k).toarray()
+        mine = assembly.degree_block(par, R, term, l).toarray()
+        assert np.abs(ref - mine).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300), k
+        n += 1
+    assert n >= 100
+
+
+@pytest.mark.parametrize("name,mkpar", FIXTURES)
+def test_boundary_values(name, mkpar):
+    d, meta, get, keys = load_ops_fixture(name)
+    par = mkpar()
+    Tb = assembly.boundary_values(1.0, par.N, 5, par.ricb)
+    Ta = assembly.boundary_values(-1.0, par.N, 4, par.ricb)
+    np.testing.assert_array_equal(Tb, d["Tb"])
+    np.testing.assert_array_equal(Ta, d["Ta"])
+    k = np.arange(par.N)
+    dxdr = 2 / (1 - par.ricb)
+    np.testing.assert_allclose(Tb[:, 0], 1.0)
+    np.testing.assert_allclose(Ta[:, 0], (-1.0) ** k)
+    np.testing.assert_allclose(Tb[:, 1], k ** 2 * dxdr)
+    np.testing.assert_allclose(Ta[:, 1], (-1.0) ** (k + 1) * k ** 2 * dxdr)
+
+
+def test_ell_matches_reference():
+    d, meta, get, keys = load_ops_fixture("spinover")
+    top, bot, ll = kell(meta["m"], meta["lmax"], meta["symm"])
+    np.testing.assert_array_equal(top, d["ell_top"])
+    np.testing.assert_array_equal(bot, d["ell_bot"])
+    np.testing.assert_array_equal(ll, d["ell_all"])
+    p = params_spinover()
+    assert (p.N1, p.n, p.sizmat, p.m_top, p.m_bot) == (meta["N1"], meta["n"], meta["sizmat"], meta["m_top"],
+                                                       meta["m_bot"])
+
+
+def test_spinover_golden_eigenvalue(spinover):
+    """End-to-end pin of assembly + shift-invert: the reference's golden."""
+    par, A, B = spinover
+    lu = sla.splu((A - par.tau * B).tocsc())
+    lam, X, res, it = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], par.tau, par.nev,
+                                                tol=1e-14, maxit=100)
+    got = lam[np.argmax(lam.real)]
+    np.testing.assert_allclose([got.real, got.imag], [SPINOVER_LAMBDA.real, SPINOVER_LAMBDA.imag], rtol=1e-8,
+                               atol=1e-20)
+
+
+def test_layout_invariants(config1):
+    par, A, B = config1
+    perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
+    assert sorted(perm.tolist()) == list(range(par.sizmat))
+    assert np.all(sizes == par.N1) and len(sizes) == par.lmax - par.m + 1
+    M = layout.permute(A - 0.5j * B, perm)
+    layout.block_tridiag(M, sizes)        # raises if not block-tridiagonal
+    # B is zero on the boundary-condition rows (the top 4 / 2 rows of each block)
+    Bl = layout.permute(B, perm).tocsr()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    top, bot, ll = kell(par.m, par.lmax, par.symm)
+    for k, l in enumerate(ll):
+        nbc = 4 if l in top else 2
+        for r in range(off[k], off[k] + nbc):
+            assert Bl.indptr[r + 1] == Bl.indptr[r]
+
+
+def test_layout_brute_force_tiny():
+    """Permutation on a tiny case written out by hand: m=0, lmax=3, symm=1, N1=2.
+    ell: all l = 1..4; pol (symm, m=0) = [2, 4]?  -- computed from the rule
+    first_pol = (sign(m)+s) % 2 = 1 -> pol = l[1::2] = [2, 4], tor = [1, 3]."""
+    top, bot, ll = layout.ell(0, 3, 1)
+    assert list(ll) == [1, 2, 3, 4] and list(top) == [2, 4] and list(bot) == [1, 3]
+    perm, sizes = layout.lmajor_permutation(0, 3, 1, 2)
+    # section-major: upol(l=2) rows 0-1, upol(l=4) rows 2-3, utor(l=1) rows 4-5, utor(l=3) rows 6-7
+    assert perm.tolist() == [4, 5, 0, 1, 6, 7, 2, 3]
+
+
+def test_forced_operator_identity():
+    """A(w) = A0 - i w B0  (bin/assemble.py:445/623, 461/756)."""
+    par = Params(m=1, symm=1, Ek=1e-4, N=24, lmax=40, forcing=7, forcing_frequency=0.7)
+    Af = assembly.assemble(par, which="forced")
+    A0, B0 = assembly.assemble(Params(m=1, symm=1, Ek=1e-4, N=24, lmax=40), which="unscaled")
+    D = (Af - (A0 - 0.7j * B0)).tocoo()
+    assert np.abs(D.data).max(initial=0) <= 1e-12 * np.abs(Af.data).max()
+
+
+def test_normalisation(config1):
+    par, A, B = config1
+    assert abs(ss.linalg.norm(B) - 1.0) < 1e-14
+    assert B.dtype == np.float64 and A.dtype == np.complex128
+
+
+def _medium():
+    par = Params(m=1, symm=1, Ek=1e-5, N=48, lmax=60)
+    A, B, _ = assembly.assemble(par)
+    return par, A, B
+
+
+@pytest.mark.parametrize("case", ["config1", "medium"])
+def test_oracle_solvers_vs_superlu(config1, case):
+    par, A, B = config1 if case == "config1" else _medium()
+    sigma = 0.3 + 0.8j
+    perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
+    M = layout.permute(A - sigma * B, perm)
+    D, L, U, off = layout.block_tridiag(M, sizes)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(M.shape[0]) + 1j * rng.standard_normal(M.shape[0])
+    xs = sla.splu(M.tocsc()).solve(b)
+    for sol in (blocktri.BlockThomas(D, L, U), bcr.BCR(D, L, U)):
+        x = sol.solve(b, off)
+        assert blocktri.relres(M, x, b) <= 1e-16
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-10
+
+
+def test_oracle_bcr_planted_mistake_is_caught(config1):
+    """A dropped Schur term (a plausible BCR bug) must fail the same check."""
+    par, A, B = config1
+    perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
+    M = layout.permute(A - 0.5j * B, perm)
+    D, L, U, off = layout.block_tridiag(M, sizes)
+    Ub = [None if u is None else 0.0 * u for u in U]   # forget the upper couplings
+    bad = bcr.BCR(D, L, Ub)
+    b = np.ones(M.shape[0], dtype=complex)
+    x = bad.solve(b, off)
+    assert blocktri.relres(M, x, b) > 1e-6
+
+
+def test_krylov_schur_vs_qz():
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.9j
+    w = la.eig(A.toarray(), B.toarray(), right=False)
+    w = w[np.isfinite(w)]
+    w = w[np.argsort(np.abs(w - sigma))]
+    lu = sla.splu((A - sigma * B).tocsc())
+    lam, X, res, it = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], sigma, 5, tol=1e-13)
+    assert len(lam) >= 5
+    for i in range(5):
+        assert abs(lam[i] - w[i]) / abs(w[i]) <= 1e-10
+    Ad, Bd = A.toarray(), B.toarray()
+    for i in range(5):
+        r = np.linalg.norm(Ad @ X[:, i] - lam[i] * (Bd @ X[:, i])) / (np.linalg.norm(Ad) * np.linalg.norm(X[:, i]))
+        assert r <= 1e-13
+
+
+def test_sort_keys_agree():
+    from kore_amd.eigs import sort_key as k_product
+    lam = np.array([1 + 2j, -0.5 + 0.1j, 0.3 - 1j, 2.0 + 0j])
+    tau = 0.2 + 0.5j
+    for which in ("TM", "TR", "TI", "LM", "SM", "LR", "SR", "LI", "SI"):
+        np.testing.assert_array_equal(np.argsort(k_product(which, lam, tau)),
+                                      np.argsort(krylov_schur.sort_key(which, lam, tau)))
