@@ -33,12 +33,13 @@ extern "C" {
 #define KORE_PREC_FP64 0   /* complex128 factor */
 #define KORE_PREC_C64 1    /* complex64 factor (FP32 MFMA); FP64 refinement in kore_solve */
 
-#define KORE_TIMER_INV 0
-#define KORE_TIMER_GEMM 1
-#define KORE_TIMER_SKINNY 2
-#define KORE_TIMER_SPMV 3
-#define KORE_TIMER_OTHER 4
-#define KORE_TIMER_COUNT 5
+#define KORE_TIMER_INV 0      /* block inversions of the factorisation        */
+#define KORE_TIMER_GEMM 1     /* Schur-complement GEMMs of the factorisation  */
+#define KORE_TIMER_SKINNY 2   /* solve sweeps (one event pair per kernel)     */
+#define KORE_TIMER_SPMV 3     /* sparse products (B x, residuals)             */
+#define KORE_TIMER_ORTHO 4    /* Krylov workspace dot / gemm                  */
+#define KORE_TIMER_OTHER 5    /* block scatter, gathers, conversions          */
+#define KORE_TIMER_COUNT 6
 
 typedef struct kore_ctx kore_ctx;
 typedef struct kore_sys kore_sys;
@@ -54,15 +55,19 @@ typedef struct {
 } kore_layout;
 
 typedef struct {
-  int32_t iterations;      /* refinement sweeps performed (1 = plain solve) */
+  int32_t iterations;      /* solve sweeps performed (1 = plain solve, no refinement) */
   int32_t converged;       /* 1 if the refinement met ir_tol (or none was asked) */
-  double backward_error;   /* max over columns of ||Rr|| / (||RM||_F ||x|| + ||Rb||), R = row scaling; -1 if not computed */
+  double rel_correction;   /* max over columns of the last ||dx|| / ||x||; -1 without refinement */
 } kore_solve_info;
 
 typedef struct {
   int32_t levels;          /* reduction levels */
   int32_t nblk;            /* degree blocks */
   double flops;            /* algorithmic real flops of the factorisation actually run */
+  double flops_inv;        /*   of which block inversions (8 n^3 each)     */
+  double flops_gemm;       /*   of which Schur-complement GEMMs (8 m n k)  */
+  double solve_bytes;      /* factor bytes one solve sweep reads          */
+  int32_t solve_launches;  /* kernel launches of one solve sweep          */
   int64_t bytes;           /* device bytes held by the factor */
   double ms;               /* device time, HIP events on the library stream */
 } kore_factor_info;
@@ -114,9 +119,10 @@ int32_t kore_factor_release(kore_sys* sys);
 
 /* Solves (A - sigma B) x = b for nrhs columns (host arrays, Kore order,
  * column-major with leading dimensions ldb/ldx).  With a complex64 factor the
- * solution is refined in FP64 (residuals by FP64 SpMV) until the backward
- * error <= ir_tol or ir_maxit sweeps; with an FP64 factor ir_maxit extra
- * sweeps are optional (0 = plain solve).
+ * solution is refined in FP64 (residual b - (A - sigma B) x by FP64 SpMV,
+ * correction by the factor) until the last correction ||dx|| <= ir_tol ||x||,
+ * the corrections stop shrinking, or 1 + ir_maxit sweeps; with an FP64 factor
+ * ir_maxit extra sweeps are optional (0 = plain solve).
  * Replaces: KSPSolve of bin/solve.py:227 (forced problems). */
 int32_t kore_solve(kore_sys* sys, int32_t nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
                    double ir_tol, int32_t ir_maxit, kore_solve_info* info);

@@ -21,8 +21,8 @@ KORE_OK = 0
 KORE_WARN_IR_NOT_CONVERGED = 1
 PREC_FP64 = 0
 PREC_C64 = 1
-TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_OTHER = range(5)
-TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "other"]
+TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER = range(6)
+TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "ortho", "other"]
 
 
 class KoreLibraryError(RuntimeError):
@@ -41,12 +41,13 @@ class Layout(C.Structure):
 
 
 class SolveInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("backward_error", C.c_double)]
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("rel_correction", C.c_double)]
 
 
 class FactorInfo(C.Structure):
     _fields_ = [("levels", C.c_int32), ("nblk", C.c_int32), ("flops", C.c_double),
-                ("bytes", C.c_int64), ("ms", C.c_double)]
+                ("flops_inv", C.c_double), ("flops_gemm", C.c_double), ("solve_bytes", C.c_double),
+                ("solve_launches", C.c_int32), ("bytes", C.c_int64), ("ms", C.c_double)]
 
 
 _P = C.c_void_p

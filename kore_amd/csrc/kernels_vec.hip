@@ -162,39 +162,61 @@ template hipError_t launch_build_blocks<float>(const CsrDev&, double, double, co
 // Workgroup: 64 rows x 4 k-slices.  Lanes of a wave share the k index, so the
 // X entries they read from LDS are broadcasts and the A column segment they
 // read from global memory is contiguous.
-constexpr int SK_KC = 128;
+template <int NR> struct SkChunk { static constexpr int value = NR >= 16 ? 128 : (NR >= 4 ? 256 : 512); };
 
 template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
+  constexpr int KC = SkChunk<NR>::value;
   const SkinnyDesc<T> d = descs[blockIdx.y];
   const int r0 = blockIdx.x * 64;
   if (r0 >= d.m) return;
   const int rl = threadIdx.x & 63, ks = threadIdx.x >> 6;
   const int row = r0 + rl;
-  __shared__ cx<T> sX[SK_KC][NR];
+  __shared__ cx<T> sX[KC][NR];
   __shared__ cx<T> sP[3][64][NR];
   cx<T> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = czero<T>();
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < 3; ++p) {
     const cx<T>* __restrict__ A = d.A[p];
     const cx<T>* __restrict__ X = d.X[p];
     const int K = d.k[p];
     if (A == nullptr || K <= 0) continue;
     const int64_t lda = d.lda[p], ldx = d.ldx[p];
-    for (int kc0 = 0; kc0 < K; kc0 += SK_KC) {
+    const T al = d.alpha[p];
+    for (int kc0 = 0; kc0 < K; kc0 += KC) {
       __syncthreads();
-      for (int idx = threadIdx.x; idx < SK_KC * NR; idx += 256) {
+      for (int idx = threadIdx.x; idx < KC * NR; idx += 256) {
         const int kk = idx / NR, c = idx - kk * NR;
         cx<T> v = czero<T>();
-        if (kc0 + kk < K && c < nrhs) v = X[(int64_t)c * ldx + kc0 + kk];
+        if (kc0 + kk < K && c < nrhs) {
+          v = X[(int64_t)c * ldx + kc0 + kk];
+          v.re *= al;
+          v.im *= al;
+        }
         sX[kk][c] = v;
       }
       __syncthreads();
-      const int kend = (K - kc0) < SK_KC ? (K - kc0) : SK_KC;
+      const int kend = (K - kc0) < KC ? (K - kc0) : KC;
       if (row < d.m) {
-        for (int kk = ks; kk < kend; kk += 4) {
-          const cx<T> a = A[(int64_t)(kc0 + kk) * lda + row];
+        const cx<T>* __restrict__ Ac = A + (int64_t)kc0 * lda + row;
+        int kk = ks;
+        // four independent loads in flight per thread
+        for (; kk + 12 < kend; kk += 16) {
+          const cx<T> a0 = Ac[(int64_t)kk * lda];
+          const cx<T> a1 = Ac[(int64_t)(kk + 4) * lda];
+          const cx<T> a2 = Ac[(int64_t)(kk + 8) * lda];
+          const cx<T> a3 = Ac[(int64_t)(kk + 12) * lda];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            acc[c] = cfma(acc[c], a0, sX[kk][c]);
+            acc[c] = cfma(acc[c], a1, sX[kk + 4][c]);
+            acc[c] = cfma(acc[c], a2, sX[kk + 8][c]);
+            acc[c] = cfma(acc[c], a3, sX[kk + 12][c]);
+          }
+        }
+        for (; kk < kend; kk += 4) {
+          const cx<T> a = Ac[(int64_t)kk * lda];
 #pragma unroll
           for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a, sX[kk][c]);
         }
@@ -216,7 +238,7 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
         v = cadd(v, sP[1][rl][c]);
         v = cadd(v, sP[2][rl][c]);
         cx<T>* y = d.Y + (int64_t)c * d.ldy + row;
-        cx<T> o{d.alpha * v.re, d.alpha * v.im};
+        cx<T> o = v;
         if (d.beta != T(0)) {
           const cx<T> old = *y;
           o.re += d.beta * old.re;
@@ -320,8 +342,8 @@ template hipError_t launch_axpy<double>(cx<double>*, int64_t, const cx<double>*,
 template hipError_t launch_axpy<float>(cx<double>*, int64_t, const cx<float>*, int64_t, int64_t, int, double,
                                        hipStream_t);
 
-template <bool MAXABS>
-__global__ __launch_bounds__(256) void colred_kernel(const cx<double>* __restrict__ x, int64_t ldx, int64_t n,
+template <class T, bool MAXABS>
+__global__ __launch_bounds__(256) void colred_kernel(const cx<T>* __restrict__ x, int64_t ldx, int64_t n,
                                                      double* __restrict__ partial, int nrhs) {
   __shared__ double red[4];
   const int c = blockIdx.y;
@@ -330,11 +352,12 @@ __global__ __launch_bounds__(256) void colred_kernel(const cx<double>* __restric
   const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
   double acc = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
-    const cx<double> v = x[(int64_t)c * ldx + i];
+    const cx<T> v = x[(int64_t)c * ldx + i];
+    const double re = (double)v.re, im = (double)v.im;
     if (MAXABS)
-      acc = fmax(acc, sqrt(v.re * v.re + v.im * v.im));
+      acc = fmax(acc, sqrt(re * re + im * im));
     else
-      acc += v.re * v.re + v.im * v.im;
+      acc += re * re + im * im;
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -350,30 +373,47 @@ __global__ __launch_bounds__(256) void colred_kernel(const cx<double>* __restric
   }
 }
 
-hipError_t launch_colnorm2(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial, int nblocks,
+template <class T>
+hipError_t launch_colnorm2(const cx<T>* x, int64_t ldx, int64_t n, int nrhs, double* partial, int nblocks,
                            hipStream_t s) {
   if (nrhs == 0) return hipSuccess;
-  hipLaunchKernelGGL(colred_kernel<false>, dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial, nrhs);
+  hipLaunchKernelGGL((colred_kernel<T, false>), dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial, nrhs);
   return hipGetLastError();
 }
+template hipError_t launch_colnorm2<double>(const cx<double>*, int64_t, int64_t, int, double*, int, hipStream_t);
+template hipError_t launch_colnorm2<float>(const cx<float>*, int64_t, int64_t, int, double*, int, hipStream_t);
+
 hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial, int nblocks,
                             hipStream_t s) {
   if (nrhs == 0) return hipSuccess;
-  hipLaunchKernelGGL(colred_kernel<true>, dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial, nrhs);
+  hipLaunchKernelGGL((colred_kernel<double, true>), dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial,
+                     nrhs);
   return hipGetLastError();
 }
 
-__global__ void sum_parts_kernel(const double* __restrict__ in, int nparts, int len, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
+// out[i] = sum_p in[p*len + i]: 32 outputs x 8 part-slices per workgroup,
+// the 8 slice sums combined through LDS (fixed order: deterministic).
+__global__ __launch_bounds__(256) void sum_parts_kernel(const double* __restrict__ in, int nparts, int len,
+                                                        double* __restrict__ out) {
+  __shared__ double red[8][32];
+  const int o = threadIdx.x & 31, sl = threadIdx.x >> 5;
+  const int i = blockIdx.x * 32 + o;
   double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += in[(int64_t)p * len + i];
-  out[i] = s;
+  if (i < len)
+    for (int p = sl; p < nparts; p += 8) s += in[(int64_t)p * len + i];
+  red[sl][o] = s;
+  __syncthreads();
+  if (sl == 0 && i < len) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += red[q][o];
+    out[i] = t;
+  }
 }
 
 hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s) {
   if (len <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sum_parts_kernel, dim3((len + 255) / 256), dim3(256), 0, s, in, nparts, len, out);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((len + 31) / 32), dim3(256), 0, s, in, nparts, len, out);
   return hipGetLastError();
 }
 

@@ -29,15 +29,16 @@ template <class T> struct InvDesc {
   int* info;   // 0 or (1-based column of the first zero pivot)
 };
 
-// Y = beta*Y + alpha*(A1*X1 + A2*X2) for skinny X (nrhs <= 16 columns);
-// A column-major m x k.
+// Y = beta*Y + sum_p alpha_p A_p X_p  (p < 3) for skinny X (nrhs <= 16
+// columns); A_p column-major m x k_p; unused products have A_p = null.
 template <class T> struct SkinnyDesc {
-  const cx<T>* A[2];
-  const cx<T>* X[2];
+  const cx<T>* A[3];
+  const cx<T>* X[3];
   cx<T>* Y;
-  int64_t lda[2], ldx[2], ldy;
-  int m, k[2];
-  T alpha, beta;
+  int64_t lda[3], ldx[3], ldy;
+  int m, k[3];
+  T alpha[3];
+  T beta;
 };
 
 template <class T>
@@ -82,8 +83,9 @@ hipError_t launch_scatter(const cx<double>* x, int64_t ldx, cx<double>* y, int64
 template <class Td>
 hipError_t launch_axpy(cx<double>* x, int64_t ldx, const cx<Td>* d, int64_t ldd, int64_t n, int nrhs,
                        double mult, hipStream_t s);
-// partial[b*nrhs + c] = sum over rows in chunk b of |x[:, c]|^2 ; nblocks written to *nb
-hipError_t launch_colnorm2(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial,
+// partial[b*nrhs + c] = sum over rows in chunk b of |x[:, c]|^2 (b < nblocks)
+template <class T>
+hipError_t launch_colnorm2(const cx<T>* x, int64_t ldx, int64_t n, int nrhs, double* partial,
                            int nblocks, hipStream_t s);
 // out[c] = max over chunks of per-chunk max |x| ... (max abs per column)
 hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial,

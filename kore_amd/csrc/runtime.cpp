@@ -276,7 +276,8 @@ static int32_t layout_build(const kore_layout* L, int64_t* sizmat, std::vector<i
 struct FactorBase {
   virtual ~FactorBase() = default;
   int prec = 0;
-  double flops = 0;
+  double flops = 0, flops_inv = 0, flops_gemm = 0, solve_bytes = 0;
+  int solve_launches = 0;
   int64_t bytes = 0;
   int nlev = 0;
 };
@@ -290,15 +291,13 @@ template <class T> struct Level {
   int ng2 = 0, t2 = 0;
   SkinnyDesc<T>* fw = nullptr;
   int nfw = 0, mfw = 0;
-  SkinnyDesc<T>* bw1 = nullptr;
-  int nbw1 = 0, mbw1 = 0;
-  SkinnyDesc<T>* bw2 = nullptr;
-  int nbw2 = 0, mbw2 = 0;
+  SkinnyDesc<T>* bw = nullptr;
+  int nbw = 0, mbw = 0;
 };
 
 template <class T> struct Factor : FactorBase {
   DevBuf mem, aux, desc, vecs;
-  std::vector<cx<T>*> D, L, U;
+  std::vector<cx<T>*> D, L, U, G, H;   // G = Dinv L, H = Dinv U of eliminated blocks
   std::vector<Level<T>> lev;
   InvDesc<T>* root_inv = nullptr;
   SkinnyDesc<T>* root_solve = nullptr;
@@ -349,6 +348,8 @@ struct kore_ws {
 };
 
 static constexpr int MAXR = 16;
+static constexpr int DOT_MAXCHUNK = 512;   // split-K chunks of the workspace inner products
+static constexpr int DOT_ROWS = 512;       // minimum rows per chunk
 
 template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   kore_ctx* c = s->ctx;
@@ -388,6 +389,13 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     }
   }
   const int64_t cntDLU = cnt;
+  std::vector<int64_t> offG(nb, -1), offH(nb, -1);
+  for (int k = 0; k < nb; ++k) {
+    offG[k] = cnt;
+    cnt += nk(k) * nmax;
+    offH[k] = cnt;
+    cnt += nk(k) * nmax;
+  }
   std::vector<std::vector<int64_t>> offE(nlev), offF(nlev);
   for (int l = 0; l < nlev; ++l) {
     const auto& id = ids[l];
@@ -410,10 +418,14 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   F->D.resize(nb);
   F->L.assign(nb, nullptr);
   F->U.assign(nb, nullptr);
+  F->G.assign(nb, nullptr);
+  F->H.assign(nb, nullptr);
   for (int k = 0; k < nb; ++k) {
     F->D[k] = base + offD[k];
     if (offL[k] >= 0) F->L[k] = base + offL[k];
     if (offU[k] >= 0) F->U[k] = base + offU[k];
+    F->G[k] = base + offG[k];
+    F->H[k] = base + offH[k];
   }
   // aux: ipiv (nb * nmax ints), info (nb ints), pointer arrays, solve vectors
   const size_t auxbytes = (size_t)nb * nmax * sizeof(int) + (size_t)nb * sizeof(int) + 3 * (size_t)nb * sizeof(void*) + 256;
@@ -438,7 +450,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   struct Rng {
     size_t off, n;
   };
-  std::vector<Rng> rinv(nlev + 1), rg1(nlev), rg2(nlev), rfw(nlev), rb1(nlev), rb2(nlev);
+  std::vector<Rng> rinv(nlev + 1), rg1(nlev), rg2(nlev), rfw(nlev), rbw(nlev);
   F->lev.resize(nlev);
   double flops = 0;
   auto tiles = [](int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); };
@@ -451,16 +463,28 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     g.tiles_m = (m + 63) / 64; g.tiles_n = (n + 63) / 64;
     g.alpha = alpha; g.beta = beta;
     flops += 8.0 * m * n * ((A1 ? k1 : 0) + (A2 ? k2 : 0));
+    F->flops_gemm += 8.0 * m * n * ((A1 ? k1 : 0) + (A2 ? k2 : 0));
     return g;
   };
   auto vecp = [&](cx<T>* v, int blk) { return v + s->boff[blk]; };
   const int ldv = (int)s->n;
-  auto mkskin = [&](cx<T>* Y, int m, const cx<T>* A1, int lda1, const cx<T>* X1, int k1, const cx<T>* A2, int lda2,
-                    const cx<T>* X2, int k2, T alpha, T beta) {
+  struct Prod {
+    const cx<T>* A;
+    int lda;
+    const cx<T>* X;
+    int k;
+    T alpha;
+  };
+  auto mkskin = [&](cx<T>* Y, int m, std::initializer_list<Prod> prods, T beta) {
     SkinnyDesc<T> d{};
-    d.A[0] = A1; d.X[0] = X1; d.k[0] = A1 ? k1 : 0; d.lda[0] = lda1; d.ldx[0] = ldv;
-    d.A[1] = A2; d.X[1] = X2; d.k[1] = A2 ? k2 : 0; d.lda[1] = lda2; d.ldx[1] = ldv;
-    d.Y = Y; d.ldy = ldv; d.m = m; d.alpha = alpha; d.beta = beta;
+    int p = 0;
+    for (const Prod& q : prods) {
+      if (!q.A) continue;
+      d.A[p] = q.A; d.lda[p] = q.lda; d.X[p] = q.X; d.ldx[p] = ldv; d.k[p] = q.k; d.alpha[p] = q.alpha;
+      F->solve_bytes += (double)sizeof(cx<T>) * m * q.k;
+      ++p;
+    }
+    d.Y = Y; d.ldy = ldv; d.m = m; d.beta = beta;
     return d;
   };
   for (int l = 0; l < nlev; ++l) {
@@ -473,6 +497,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       const int k = id[t];
       vinv.push_back(InvDesc<T>{F->D[k], nk(k), (int)nk(k), ipiv + (size_t)k * nmax, F->info + k});
       flops += 8.0 * nk(k) * nk(k) * nk(k);
+      F->flops_inv += 8.0 * nk(k) * nk(k) * nk(k);
       L.maxn_inv = std::max(L.maxn_inv, (int)nk(k));
     }
     rinv[l].n = vinv.size() - rinv[l].off;
@@ -491,6 +516,21 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
         const int kn = id[t + 1];
         vg.push_back(mkgemm(base + offF[l][t], m, (int)nk(kn), F->U[k], F->D[kn], (int)nk(kn), m, (int)nk(kn),
                             nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+        L.t1 = std::max(L.t1, tiles(m, (int)nk(kn)));
+      }
+    }
+    // G_t = Dinv_t L_t, H_t = Dinv_t U_t for the eliminated blocks (fused backward solve)
+    for (int t = 1; t < T_; t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      const int kp = id[t - 1];
+      vg.push_back(mkgemm(F->G[k], m, (int)nk(kp), F->D[k], F->L[k], m, m, m, nullptr, nullptr, 0, 0, 0, T(1),
+                          T(0)));
+      L.t1 = std::max(L.t1, tiles(m, (int)nk(kp)));
+      if (t + 1 <= T_ - 1) {
+        const int kn = id[t + 1];
+        vg.push_back(mkgemm(F->H[k], m, (int)nk(kn), F->D[k], F->U[k], m, m, m, nullptr, nullptr, 0, 0, 0, T(1),
+                            T(0)));
         L.t1 = std::max(L.t1, tiles(m, (int)nk(kn)));
       }
     }
@@ -530,34 +570,30 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       const int m = (int)nk(k);
       const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
       if (!hasE && !hasF) continue;
-      vs.push_back(mkskin(vecp(F->Bv, k), m, hasE ? base + offE[l][t] : nullptr, m,
-                          hasE ? vecp(F->Bv, id[t - 1]) : nullptr, hasE ? (int)nk(id[t - 1]) : 0,
-                          hasF ? base + offF[l][t] : nullptr, m, hasF ? vecp(F->Bv, id[t + 1]) : nullptr,
-                          hasF ? (int)nk(id[t + 1]) : 0, T(-1), T(1)));
+      vs.push_back(mkskin(vecp(F->Bv, k), m,
+                          {Prod{hasE ? base + offE[l][t] : nullptr, m, hasE ? vecp(F->Bv, id[t - 1]) : nullptr,
+                                hasE ? (int)nk(id[t - 1]) : 0, T(-1)},
+                           Prod{hasF ? base + offF[l][t] : nullptr, m, hasF ? vecp(F->Bv, id[t + 1]) : nullptr,
+                                hasF ? (int)nk(id[t + 1]) : 0, T(-1)}},
+                          T(1)));
       L.mfw = std::max(L.mfw, m);
     }
     rfw[l].n = vs.size() - rfw[l].off;
-    // backward 1: B_t -= L_t X_{t-1} + U_t X_{t+1}   (t odd)
-    rb1[l].off = vs.size();
+    // backward: X_t = Dinv_t B_t - G_t X_{t-1} - H_t X_{t+1}   (t odd)
+    rbw[l].off = vs.size();
     for (int t = 1; t < T_; t += 2) {
       const int k = id[t];
       const int m = (int)nk(k);
       const bool hasR = t + 1 <= T_ - 1;
-      vs.push_back(mkskin(vecp(F->Bv, k), m, F->L[k], m, vecp(F->Xv, id[t - 1]), (int)nk(id[t - 1]),
-                          hasR ? F->U[k] : nullptr, m, hasR ? vecp(F->Xv, id[t + 1]) : nullptr,
-                          hasR ? (int)nk(id[t + 1]) : 0, T(-1), T(1)));
-      L.mbw1 = std::max(L.mbw1, m);
+      vs.push_back(mkskin(vecp(F->Xv, k), m,
+                          {Prod{F->D[k], m, vecp(F->Bv, k), m, T(1)},
+                           Prod{F->G[k], m, vecp(F->Xv, id[t - 1]), (int)nk(id[t - 1]), T(-1)},
+                           Prod{hasR ? F->H[k] : nullptr, m, hasR ? vecp(F->Xv, id[t + 1]) : nullptr,
+                                hasR ? (int)nk(id[t + 1]) : 0, T(-1)}},
+                          T(0)));
+      L.mbw = std::max(L.mbw, m);
     }
-    rb1[l].n = vs.size() - rb1[l].off;
-    // backward 2: X_t = Dinv_t B_t
-    rb2[l].off = vs.size();
-    for (int t = 1; t < T_; t += 2) {
-      const int k = id[t];
-      const int m = (int)nk(k);
-      vs.push_back(mkskin(vecp(F->Xv, k), m, F->D[k], m, vecp(F->Bv, k), m, nullptr, 0, nullptr, 0, T(1), T(0)));
-      L.mbw2 = std::max(L.mbw2, m);
-    }
-    rb2[l].n = vs.size() - rb2[l].off;
+    rbw[l].n = vs.size() - rbw[l].off;
   }
   // root
   const int kr = ids[nlev][0];
@@ -565,11 +601,13 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   vinv.push_back(InvDesc<T>{F->D[kr], nk(kr), (int)nk(kr), ipiv + (size_t)kr * nmax, F->info + kr});
   rinv[nlev].n = 1;
   flops += 8.0 * nk(kr) * nk(kr) * nk(kr);
+  F->flops_inv += 8.0 * nk(kr) * nk(kr) * nk(kr);
   const size_t rroot = vs.size();
-  vs.push_back(mkskin(vecp(F->Xv, kr), (int)nk(kr), F->D[kr], (int)nk(kr), vecp(F->Bv, kr), (int)nk(kr), nullptr, 0,
-                      nullptr, 0, T(1), T(0)));
+  vs.push_back(mkskin(vecp(F->Xv, kr), (int)nk(kr), {Prod{F->D[kr], (int)nk(kr), vecp(F->Bv, kr), (int)nk(kr), T(1)}},
+                      T(0)));
   F->root_n = (int)nk(kr);
   F->flops = flops;
+  F->solve_launches = 2 * nlev + 1;
 
   // upload descriptors
   const size_t bi = vinv.size() * sizeof(InvDesc<T>);
@@ -598,8 +636,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     L.g1 = dg + rg1[l].off; L.ng1 = (int)rg1[l].n;
     L.g2 = dg + rg2[l].off; L.ng2 = (int)rg2[l].n;
     L.fw = dsk + rfw[l].off; L.nfw = (int)rfw[l].n;
-    L.bw1 = dsk + rb1[l].off; L.nbw1 = (int)rb1[l].n;
-    L.bw2 = dsk + rb2[l].off; L.nbw2 = (int)rb2[l].n;
+    L.bw = dsk + rbw[l].off; L.nbw = (int)rbw[l].n;
   }
   F->root_inv = dinv + rinv[nlev].off;
   F->root_solve = dsk + rroot;
@@ -616,6 +653,7 @@ template <class T> static int64_t factor_count_bytes(const std::vector<int>& bsz
     cnt += (int64_t)bsz[k] * bsz[k];
     if (k > 0) cnt += (int64_t)bsz[k] * nmax;
     if (k < nb - 1) cnt += (int64_t)bsz[k] * nmax;
+    cnt += 2 * (int64_t)bsz[k] * nmax;   // G, H
   }
   std::vector<int> cur(nb);
   for (int k = 0; k < nb; ++k) cur[k] = k;
@@ -667,106 +705,119 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
 template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs) {
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
-  Timed tm(c, KORE_TIMER_SKINNY);
   for (int l = 0; l < F->nlev; ++l) {
     Level<T>& L = F->lev[l];
+    Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
   }
-  HIP_TRY(launch_skinny<T>(F->root_solve, 1, F->root_n, nrhs, st));
+  {
+    Timed tm(c, KORE_TIMER_SKINNY);
+    HIP_TRY(launch_skinny<T>(F->root_solve, 1, F->root_n, nrhs, st));
+  }
   for (int l = F->nlev - 1; l >= 0; --l) {
     Level<T>& L = F->lev[l];
-    HIP_TRY(launch_skinny<T>(L.bw1, L.nbw1, L.mbw1, nrhs, st));
-    HIP_TRY(launch_skinny<T>(L.bw2, L.nbw2, L.mbw2, nrhs, st));
+    Timed tm(c, KORE_TIMER_SKINNY);
+    HIP_TRY(launch_skinny<T>(L.bw, L.nbw, L.mbw, nrhs, st));
   }
   return KORE_OK;
 }
 
-// column 2-norms of a complex128 block (degree-major, ld = n) -> host
-static int32_t colnorms(kore_sys* s, const cx<double>* x, int64_t ld, int nrhs, double* out) {
+// Column 2-norms of one or two blocks (degree-major, length n) -> host, with a
+// single synchronisation.  b may be null.
+template <class Ta, class Tb>
+static int32_t colnorms2(kore_sys* s, const cx<Ta>* a, int64_t lda, const cx<Tb>* b, int64_t ldb, int nrhs,
+                         double* outa, double* outb) {
   kore_ctx* c = s->ctx;
-  HIP_TRY(launch_colnorm2(x, ld, s->n, nrhs, s->part, s->npart, c->stream));
-  std::vector<double> h((size_t)s->npart * nrhs);
+  HIP_TRY(launch_colnorm2<Ta>(a, lda, s->n, nrhs, s->part, s->npart, c->stream));
+  if (b) HIP_TRY(launch_colnorm2<Tb>(b, ldb, s->n, nrhs, s->part + (size_t)s->npart * MAXR, s->npart, c->stream));
+  const int nh = b ? 2 : 1;
+  std::vector<double> h((size_t)nh * s->npart * MAXR);
   HIP_TRY(hipMemcpyAsync(h.data(), s->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int j = 0; j < nrhs; ++j) {
-    double a = 0;
-    for (int b = 0; b < s->npart; ++b) a += h[(size_t)b * nrhs + j];
-    out[j] = std::sqrt(a);
+    double sa = 0, sb = 0;
+    for (int q = 0; q < s->npart; ++q) {
+      sa += h[(size_t)q * nrhs + j];
+      if (b) sb += h[(size_t)s->npart * MAXR + (size_t)q * nrhs + j];
+    }
+    outa[j] = std::sqrt(sa);
+    if (b) outb[j] = std::sqrt(sb);
+  }
+  return KORE_OK;
+}
+
+template <class T>
+static int32_t refine_sweep(kore_sys* s, Factor<T>* F, const cx<double>* rhs, int64_t ldr, cx<double>* x,
+                            int64_t ldx, int nrhs, bool first) {
+  kore_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  const int64_t n = s->n;
+  {
+    Timed tm(c, KORE_TIMER_OTHER);
+    HIP_TRY(launch_gather_scale<double, T>(rhs, ldr, F->Bv, n, n, nrhs, nullptr, s->d_scale.as<double>(), 1.0, st));
+  }
+  TRY(bcr_solve<T>(s, F, nrhs));
+  if (first) {
+    Timed tm(c, KORE_TIMER_OTHER);
+    HIP_TRY(launch_gather_scale<T, double>(F->Xv, n, x, ldx, n, nrhs, nullptr, nullptr, 1.0, st));
   }
   return KORE_OK;
 }
 
 // Solve (A - sigma B) x = b for nrhs <= 16 columns; b, x complex128 on the
 // device in degree-major order (leading dimensions ldb, ldx).
+//
+// Refinement (always with a complex64 factor; ir_maxit extra sweeps with an
+// FP64 one): r = b - (A - sigma B) x in FP64 by SpMV, d = factor^{-1} R r,
+// x += d; stop when max_c ||d_c|| / ||x_c|| <= ir_tol (Wilkinson's test) or
+// when the correction stops shrinking (ratio > 1/2 between sweeps).
 static int32_t solve_dev(kore_sys* s, const cx<double>* b, int64_t ldb, cx<double>* x, int64_t ldx, int nrhs,
                          double ir_tol, int ir_maxit, kore_solve_info* info) {
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int64_t n = s->n;
-  const double* scale = s->d_scale.as<double>();
   if (!s->f) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
   if (nrhs < 1 || nrhs > MAXR) return fail(KORE_ERR_ARG, "nrhs must be 1..16 per device solve");
   if (info) {
     info->iterations = 0;
     info->converged = 1;
-    info->backward_error = -1.0;
+    info->rel_correction = -1.0;
   }
   const bool c64 = s->f->prec == KORE_PREC_C64;
-  const bool refine = c64 || ir_maxit > 0;
-  std::vector<double> nb_(nrhs), nx_(nrhs), nr_(nrhs);
-  if (refine) {
-    HIP_TRY(launch_gather_scale<double, double>(b, ldb, s->st, n, n, nrhs, nullptr, scale, 1.0, st));
-    TRY(colnorms(s, s->st, n, nrhs, nb_.data()));
-  }
-  const int maxsweep = c64 ? std::max(1, ir_maxit) : 1 + std::max(0, ir_maxit);
-  double eta_prev = 1e300;
-  int it = 0;
-  for (it = 1; it <= maxsweep; ++it) {
-    const cx<double>* rhs = (it == 1) ? b : s->rr;
-    const int64_t ldr = (it == 1) ? ldb : n;
-    if (c64) {
-      Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
-      {
-        Timed tm(c, KORE_TIMER_OTHER);
-        HIP_TRY(launch_gather_scale<double, float>(rhs, ldr, F->Bv, n, n, nrhs, nullptr, scale, 1.0, st));
-      }
-      TRY(bcr_solve<float>(s, F, nrhs));
-      Timed tm(c, KORE_TIMER_OTHER);
-      if (it == 1) HIP_TRY(launch_gather_scale<float, double>(F->Xv, n, x, ldx, n, nrhs, nullptr, nullptr, 1.0, st));
-      else HIP_TRY(launch_axpy<float>(x, ldx, F->Xv, n, n, nrhs, 1.0, st));
-    } else {
-      Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
-      {
-        Timed tm(c, KORE_TIMER_OTHER);
-        HIP_TRY(launch_gather_scale<double, double>(rhs, ldr, F->Bv, n, n, nrhs, nullptr, scale, 1.0, st));
-      }
-      TRY(bcr_solve<double>(s, F, nrhs));
-      Timed tm(c, KORE_TIMER_OTHER);
-      if (it == 1) HIP_TRY(launch_gather_scale<double, double>(F->Xv, n, x, ldx, n, nrhs, nullptr, nullptr, 1.0, st));
-      else HIP_TRY(launch_axpy<double>(x, ldx, F->Xv, n, n, nrhs, 1.0, st));
-    }
-    if (!refine) break;
-    // r = b - (A - sigma B) x ; eta = ||R r|| / (||RM|| ||x|| + ||R b||)
+  const int maxsweep = c64 ? std::max(2, 1 + ir_maxit) : 1 + std::max(0, ir_maxit);
+  Factor<float>* F32 = c64 ? static_cast<Factor<float>*>(s->f.get()) : nullptr;
+  Factor<double>* F64 = c64 ? nullptr : static_cast<Factor<double>*>(s->f.get());
+  if (c64) TRY(refine_sweep<float>(s, F32, b, ldb, x, ldx, nrhs, true));
+  else TRY(refine_sweep<double>(s, F64, b, ldb, x, ldx, nrhs, true));
+  int it = 1;
+  double prev = 1e300;
+  std::vector<double> nd(nrhs), nx(nrhs);
+  for (it = 2; it <= maxsweep; ++it) {
     {
       Timed tm(c, KORE_TIMER_SPMV);
       HIP_TRY(launch_copy_cols(b, ldb, s->rr, n, n, nrhs, st));
       HIP_TRY(launch_spmv(s->csr, -1.0, 0.0, s->sre, s->sim, x, ldx, s->rr, n, nrhs, 1.0, st));
-      HIP_TRY(launch_gather_scale<double, double>(s->rr, n, s->st, n, n, nrhs, nullptr, scale, 1.0, st));
     }
-    TRY(colnorms(s, s->st, n, nrhs, nr_.data()));
-    TRY(colnorms(s, x, ldx, nrhs, nx_.data()));
-    double eta = 0;
-    for (int j = 0; j < nrhs; ++j) {
-      const double den = s->normRM * nx_[j] + nb_[j];
-      eta = std::max(eta, den > 0 ? nr_[j] / den : 0.0);
+    if (c64) {
+      TRY(refine_sweep<float>(s, F32, s->rr, n, x, ldx, nrhs, false));
+      TRY(colnorms2<float, double>(s, F32->Xv, n, x, ldx, nrhs, nd.data(), nx.data()));
+      Timed tm(c, KORE_TIMER_OTHER);
+      HIP_TRY(launch_axpy<float>(x, ldx, F32->Xv, n, n, nrhs, 1.0, st));
+    } else {
+      TRY(refine_sweep<double>(s, F64, s->rr, n, x, ldx, nrhs, false));
+      TRY(colnorms2<double, double>(s, F64->Xv, n, x, ldx, nrhs, nd.data(), nx.data()));
+      Timed tm(c, KORE_TIMER_OTHER);
+      HIP_TRY(launch_axpy<double>(x, ldx, F64->Xv, n, n, nrhs, 1.0, st));
     }
-    if (info) info->backward_error = eta;
-    if (eta <= ir_tol) break;
-    if (it >= 2 && eta > 0.5 * eta_prev) {  // stalled
+    double ratio = 0;
+    for (int j = 0; j < nrhs; ++j) ratio = std::max(ratio, nx[j] > 0 ? nd[j] / nx[j] : 0.0);
+    if (info) info->rel_correction = ratio;
+    if (ratio <= ir_tol) break;
+    if (it >= 3 && ratio > 0.5 * prev) {  // stagnated
       if (info) info->converged = 0;
       break;
     }
-    eta_prev = eta;
+    prev = ratio;
     if (it == maxsweep && info) info->converged = 0;
   }
   if (info) info->iterations = std::min(it, maxsweep);
@@ -948,7 +999,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->d_boff.alloc(ctx, (s->nblk + 1) * sizeof(int64_t)));
   TRY(s->d_scale.alloc(ctx, n * sizeof(double)));
   TRY(s->d_scr.alloc(ctx, 4 * (size_t)n * MAXR * sizeof(cx<double>)));
-  TRY(s->d_part.alloc(ctx, (size_t)s->npart * MAXR * sizeof(double)));
+  TRY(s->d_part.alloc(ctx, 2 * (size_t)s->npart * MAXR * sizeof(double)));
   hipStream_t st = ctx->stream;
   HIP_TRY(hipMemcpyAsync(s->d_ip.p, s->h_ip.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
   if (nnz) {
@@ -1064,6 +1115,10 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     info->levels = s->f->nlev;
     info->nblk = s->nblk;
     info->flops = s->f->flops;
+    info->flops_inv = s->f->flops_inv;
+    info->flops_gemm = s->f->flops_gemm;
+    info->solve_bytes = s->f->solve_bytes;
+    info->solve_launches = s->f->solve_launches;
     info->bytes = s->f->bytes;
     info->ms = ms;
   }
@@ -1086,7 +1141,7 @@ int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, doub
   if (info) {
     info->iterations = 0;
     info->converged = 1;
-    info->backward_error = -1;
+    info->rel_correction = -1;
   }
   for (int c0 = 0; c0 < nrhs; c0 += MAXR) {
     const int nr = std::min(MAXR, nrhs - c0);
@@ -1105,7 +1160,7 @@ int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, doub
     if (info) {
       info->iterations = std::max(info->iterations, li.iterations);
       info->converged = info->converged && li.converged;
-      info->backward_error = std::max(info->backward_error, li.backward_error);
+      info->rel_correction = std::max(info->rel_correction, li.rel_correction);
     }
   }
   return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
@@ -1120,9 +1175,9 @@ int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
   w->ncol = ncol;
   kore_ctx* c = s->ctx;
   TRY(w->V.alloc(c, (size_t)s->n * ncol * sizeof(zd)));
-  TRY(w->part.alloc(c, (size_t)64 * ncol * ncol * sizeof(zd) + (size_t)ncol * ncol * sizeof(zd)));
+  TRY(w->part.alloc(c, ((size_t)DOT_MAXCHUNK + 1) * ncol * MAXR * sizeof(zd)));
   TRY(w->qbuf.alloc(c, (size_t)ncol * ncol * sizeof(zd)));
-  TRY(w->descb.alloc(c, 64 * sizeof(GemmDesc<double>)));
+  TRY(w->descb.alloc(c, (DOT_MAXCHUNK + 64) * sizeof(GemmDesc<double>)));
   TRY(w->up.init(c, 1 << 22));
   HIP_TRY(launch_zero<double>(w->V.as<cx<double>>(), (int64_t)s->n * ncol, c->stream));
   *out = w.release();
@@ -1187,7 +1242,7 @@ int32_t kore_ws_apply_op(kore_ws* w, int32_t src, int32_t dst, int32_t nc, doubl
   if (info) {
     info->iterations = 0;
     info->converged = 1;
-    info->backward_error = -1;
+    info->rel_correction = -1;
   }
   for (int a = 0; a < nc; a += MAXR) {
     const int nr = std::min(MAXR, nc - a);
@@ -1200,7 +1255,7 @@ int32_t kore_ws_apply_op(kore_ws* w, int32_t src, int32_t dst, int32_t nc, doubl
     if (info) {
       info->iterations = std::max(info->iterations, li.iterations);
       info->converged = info->converged && li.converged;
-      info->backward_error = std::max(info->backward_error, li.backward_error);
+      info->rel_correction = std::max(info->rel_correction, li.rel_correction);
     }
   }
   return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
@@ -1225,36 +1280,43 @@ int32_t kore_ws_dot(kore_ws* w, int32_t a0, int32_t na, int32_t b0, int32_t nb, 
   kore_sys* s = w->sys;
   kore_ctx* c = s->ctx;
   const int64_t n = s->n;
-  int nchunk = (int)std::min<int64_t>(64, (n + 4095) / 4096);
+  // split-K over row chunks so that every CU streams a slice of the basis
+  int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
   if (nchunk < 1) nchunk = 1;
   const int64_t chunk = (n + nchunk - 1) / nchunk;
-  std::vector<GemmDesc<double>> ds;
   cx<double>* part = w->part.as<cx<double>>();
-  cx<double>* res = part + (size_t)64 * w->ncol * w->ncol;
-  for (int q = 0; q < nchunk; ++q) {
-    const int64_t k0 = q * chunk;
-    const int64_t kl = std::min<int64_t>(chunk, n - k0);
-    if (kl <= 0) break;
-    GemmDesc<double> g{};
-    g.A[0] = wcol(w, a0) + k0; g.lda[0] = n;
-    g.B[0] = wcol(w, b0) + k0; g.ldb[0] = n;
-    g.k[0] = (int)kl;
-    g.C = part + (size_t)q * na * nb; g.ldc = na;
-    g.m = na; g.n = nb; g.opA = 1;
-    g.tiles_m = (na + 63) / 64; g.tiles_n = (nb + 63) / 64;
-    g.alpha = 1.0; g.beta = 0.0;
-    ds.push_back(g);
+  cx<double>* res = part + (size_t)DOT_MAXCHUNK * w->ncol * MAXR;
+  std::vector<zd> h((size_t)na * nb);
+  for (int j0 = 0; j0 < nb; j0 += MAXR) {
+    const int nbj = std::min(MAXR, nb - j0);
+    std::vector<GemmDesc<double>> ds;
+    for (int q = 0; q < nchunk; ++q) {
+      const int64_t k0 = q * chunk;
+      const int64_t kl = std::min<int64_t>(chunk, n - k0);
+      if (kl <= 0) break;
+      GemmDesc<double> g{};
+      g.A[0] = wcol(w, a0) + k0; g.lda[0] = n;
+      g.B[0] = wcol(w, b0 + j0) + k0; g.ldb[0] = n;
+      g.k[0] = (int)kl;
+      g.C = part + (size_t)q * na * nbj; g.ldc = na;
+      g.m = na; g.n = nbj; g.opA = 1;
+      g.tiles_m = (na + 63) / 64; g.tiles_n = (nbj + 63) / 64;
+      g.alpha = 1.0; g.beta = 0.0;
+      ds.push_back(g);
+    }
+    GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>();
+    TRY(w->up.put(dd, ds.data(), ds.size() * sizeof(GemmDesc<double>)));
+    {
+      Timed tm(c, KORE_TIMER_ORTHO);
+      HIP_TRY(launch_gemm_batched<double>(dd, (int)ds.size(), ds[0].tiles_m * ds[0].tiles_n, c->stream));
+      HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), (int)ds.size(), 2 * na * nbj,
+                               reinterpret_cast<double*>(res), c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h.data() + (size_t)j0 * na, res, (size_t)na * nbj * sizeof(zd), hipMemcpyDeviceToHost,
+                           c->stream));
   }
-  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>();
-  TRY(w->up.put(dd, ds.data(), ds.size() * sizeof(GemmDesc<double>)));
-  {
-    Timed tm(c, KORE_TIMER_GEMM);
-    HIP_TRY(launch_gemm_batched<double>(dd, (int)ds.size(), ds[0].tiles_m * ds[0].tiles_n, c->stream));
-    HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), (int)ds.size(), 2 * na * nb,
-                             reinterpret_cast<double*>(res), c->stream));
-  }
-  HIP_TRY(hipMemcpyAsync(H, res, (size_t)na * nb * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  memcpy(H, h.data(), h.size() * sizeof(zd));
   return KORE_OK;
 }
 
@@ -1278,9 +1340,9 @@ int32_t kore_ws_gemm(kore_ws* w, int32_t d0, int32_t nd, int32_t s0, int32_t ns,
   g.m = (int)n; g.n = nd; g.opA = 0;
   g.tiles_m = (int)((n + 63) / 64); g.tiles_n = (nd + 63) / 64;
   g.alpha = alpha; g.beta = beta;
-  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>() + 63;
+  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>() + DOT_MAXCHUNK + 1;
   TRY(w->up.put(dd, &g, sizeof(g)));
-  Timed tm(c, KORE_TIMER_GEMM);
+  Timed tm(c, KORE_TIMER_ORTHO);
   HIP_TRY(launch_gemm_batched<double>(dd, 1, std::min(g.tiles_m * g.tiles_n, 65535), c->stream));
   return KORE_OK;
 }

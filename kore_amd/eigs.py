@@ -80,23 +80,26 @@ def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
 
 
 def _triu_eigvecs(S):
+    """Unit-norm eigenvectors of an upper-triangular matrix (columns)."""
     n = S.shape[0]
     d = np.diag(S)
     Z = np.zeros((n, n), dtype=complex)
+    tiny = np.finfo(float).tiny
     for i in range(n):
         z = np.zeros(n, dtype=complex)
         z[i] = 1.0
-        for r in range(i - 1, -1, -1):
-            den = d[r] - d[i]
-            if den == 0:
-                den = np.finfo(float).tiny
-            z[r] = -(S[r, r + 1:i + 1] @ z[r + 1:i + 1]) / den
+        if i > 0:
+            M = S[:i, :i] - d[i] * np.eye(i)
+            dg = np.diag(M).copy()
+            dg[dg == 0] = tiny
+            M[np.diag_indices(i)] = dg
+            z[:i] = sla.solve_triangular(M, -S[:i, i], lower=False, check_finite=False)
         Z[:, i] = z / np.linalg.norm(z)
     return Z
 
 
 def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, seed=0,
-         ir_tol=1e-15, ir_maxit=12, v0=None, log=None, workspace=Workspace) -> EigResult:
+         ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=Workspace) -> EigResult:
     """Eigenpairs nearest ``sigma`` of the pencil held by ``sysm``.
 
     ``sysm`` must have been factored at ``sigma`` (System.factor).
@@ -108,7 +111,8 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
     n = sysm.n
     bs = max(1, min(int(block), 16))
     if ncv is None:
-        ncv = max(2 * nev, nev + 15)
+        # a block space needs more columns to reach the same polynomial degree
+        ncv = max(2 * nev, nev + 15, 10 * bs)
     ncv = max(ncv, nev + 2 * bs)
     ncv = int(np.ceil(ncv / bs) * bs)
     ncv = min(ncv, max(bs, (n // bs) * bs - bs))
@@ -162,19 +166,18 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
             H[:nb_all, j:j + bs] = h
             H[j + bs:j + 2 * bs, j:j + bs] = R
             j += bs
-        # ---- Rayleigh-Ritz on the projected matrix (host, FP64)
+        # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
         T = H[:m, :m]
         Bres = H[m:m + bs, :m]
-        # Schur form ordered by the wanted criterion
         S, Q = sla.schur(T, output="complex")
-        d = np.diag(S)
-        kd = sort_key(which, sigma + 1.0 / d, sigma)
-        perm = np.argsort(kd, kind="stable")
-        S, Q = _reorder(S, Q, np.argsort(perm))
-        theta = np.diag(S).copy()
-        Z = _triu_eigvecs(S)
-        Y = Q @ Z
-        res = np.linalg.norm(Bres @ Y, axis=0)
+        theta_s = np.diag(S).copy()
+        Zs = _triu_eigvecs(S)
+        Ys = Q @ Zs
+        res_s = np.linalg.norm(Bres @ Ys, axis=0)
+        order = np.argsort(sort_key(which, sigma + 1.0 / theta_s, sigma), kind="stable")
+        theta = theta_s[order]
+        Y = Ys[:, order]
+        res = res_s[order]
         conv = res <= tol * np.abs(theta)
         nconv = 0
         while nconv < m and conv[nconv]:
@@ -184,17 +187,22 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
                 float(np.max(res[:nev] / np.abs(theta[:nev])))))
         if nconv >= nev or restarts >= maxit:
             break
-        # ---- thick restart
+        # ---- thick restart: keep the 'keep' best Ritz values' invariant subspace
         keep = nconv + max(1, (m - nconv) // 2)
         keep = max(keep, nev)
         keep = int(np.ceil(keep / bs) * bs)       # the expansion then ends exactly at m
         keep = max(bs, min(keep, m - bs))
-        ws.gemm(cT, keep, 0, m, Q[:, :keep], alpha=1.0, beta=0.0)
+        select = np.zeros(m, dtype=np.int32)
+        select[order[:keep]] = 1
+        St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, S, Q, job="N")
+        if info != 0 or msel != keep:
+            raise RuntimeError("Schur reordering failed (info %d, %d of %d)" % (info, msel, keep))
+        ws.gemm(cT, keep, 0, m, Qt[:, :keep], alpha=1.0, beta=0.0)
         ws.copy(cT, 0, keep)
         ws.copy(m, keep, bs)
         Hn = np.zeros_like(H)
-        Hn[:keep, :keep] = S[:keep, :keep]
-        Hn[keep:keep + bs, :keep] = Bres @ Q[:, :keep]
+        Hn[:keep, :keep] = np.triu(St[:keep, :keep])
+        Hn[keep:keep + bs, :keep] = Bres @ Qt[:, :keep]
         H = Hn
         j = keep
         restarts += 1
@@ -209,33 +217,6 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
     return EigResult(lam=lam, X=X, errest=res[:k] / np.abs(theta[:k]), nconv=k, restarts=restarts,
                      op_columns=op_cols, op_calls=op_calls, seconds=time.perf_counter() - t0,
                      ir_max_iterations=ir_iters)
-
-
-def _reorder(S, Q, target_pos):
-    """Reorder the complex Schur form so that diagonal entry i moves to
-    position target_pos[i] (adjacent Givens swaps, the ztrexc method)."""
-    S = S.copy()
-    Q = Q.copy()
-    n = S.shape[0]
-    order = np.argsort(target_pos)          # order[p] = current index to place at p
-    cur = list(range(n))
-    for p, src in enumerate(order):
-        pos = cur.index(src)
-        while pos > p:
-            i = pos - 1
-            a, b, c = S[i, i], S[i + 1, i + 1], S[i, i + 1]
-            x = np.array([c, b - a])
-            nr = np.linalg.norm(x)
-            if nr != 0:
-                cs, sn = x[0] / nr, x[1] / nr
-                G = np.array([[np.conj(cs), np.conj(sn)], [-sn, cs]])
-                S[[i, i + 1], :] = G @ S[[i, i + 1], :]
-                S[:, [i, i + 1]] = S[:, [i, i + 1]] @ G.conj().T
-                Q[:, [i, i + 1]] = Q[:, [i, i + 1]] @ G.conj().T
-                S[i + 1, i] = 0.0
-            cur[pos - 1], cur[pos] = cur[pos], cur[pos - 1]
-            pos -= 1
-    return S, Q
 
 
 def residuals(A, B, lam, X):

@@ -93,7 +93,7 @@ def test_solve_config1(ctx, config1, precision):
     par, A, B = config1
     say("case: config1 solve precision", precision)
     err, info = _solve_case(ctx, par, A, B, 1.0j, precision, nrhs=3)
-    say("  rel err", err, "IR iterations", info.iterations, "backward error", info.backward_error)
+    say("  rel err", err, "sweeps", info.iterations, "last correction", info.rel_correction)
     assert err.max() <= 1e-10
 
 
