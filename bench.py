@@ -74,21 +74,27 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv):
     ctx.set_timing(True)
     lat = []
     finfo = None
+    host = {}
     ctx.sync()
     t_all = time.perf_counter()
     for k in range(steps):
         t0 = time.perf_counter()
         finfo = S.factor(sigma, precision)
+        t1 = time.perf_counter()
         res = eigs.eigs(S, sigma, nev=nev, ncv=ncv, tol=tol, block=block)
         ctx.sync()
         lat.append(time.perf_counter() - t0)
+        host["factor_call"] = host.get("factor_call", 0.0) + (t1 - t0)
+        for kk, v in res.host_seconds.items():
+            host[kk] = host.get(kk, 0.0) + v
         log("  step %d (%s): %.2f ms, nconv %d, sweeps %d, restarts %d" % (
             k, "c64+IR" if precision else "fp64", 1e3 * lat[-1], res.nconv, res.op_calls, res.restarts))
     ctx.sync()
     total = time.perf_counter() - t_all
     ctx.set_timing(False)
     timing = ctx.timing()
-    return dict(lat=lat, total=total, res=res, finfo=finfo, timing=timing)
+    return dict(lat=lat, total=total, res=res, finfo=finfo, timing=timing,
+                host_ms_per_step={k: 1e3 * v / steps for k, v in host.items()})
 
 
 def roofline(run, precision, steps):
@@ -210,7 +216,7 @@ def main():
             nconv=int(res.nconv), sweeps=int(res.op_calls), restarts=int(res.restarts),
             residual_max=float(resid.max()), factor_ms=float(r["finfo"].ms),
             factor_gflop=float(r["finfo"].flops / 1e9), factor_gbytes=float(r["finfo"].bytes / 1e9),
-            roofline=rl, kernel_classes=classes, lam=[[float(l.real), float(l.imag)] for l in res.lam[:wl["nev"]]])
+            roofline=rl, kernel_classes=classes, host_ms_per_step=r["host_ms_per_step"], lam=[[float(l.real), float(l.imag)] for l in res.lam[:wl["nev"]]])
         log("  %s: %.3f solves/s, p50 %.1f ms, p99 %.1f ms, residual %.2e" % (
             name, paths[name]["solves_per_s"], paths[name]["p50_ms"], paths[name]["p99_ms"], resid.max()))
     # parity between the two paths (relative eigenvalue difference)

@@ -254,6 +254,17 @@ class System:
     def release_factor(self):
         _check(lib().kore_factor_release(self._p))
 
+    def get_workspace(self, ncol: int) -> "Workspace":
+        """A device workspace of at least ncol columns, kept with the system
+        so that repeated eigensolves do not re-allocate it."""
+        ws = getattr(self, "_ws_cached", None)
+        if ws is None or ws._p is None or ws.ncol < ncol:
+            if ws is not None:
+                ws.close()
+            ws = Workspace(self, int(ncol))
+            self._ws_cached = ws
+        return ws
+
     def solve(self, b, ir_tol=1e-14, ir_maxit=0):
         """x = (A - sigma B)^{-1} b for b of shape (n,) or (n, nrhs) (Kore order)."""
         b = np.asarray(b, dtype=np.complex128)

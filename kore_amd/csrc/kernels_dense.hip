@@ -138,17 +138,21 @@ template <class T> __device__ __forceinline__ T tiny_pivot() { return sizeof(T) 
 // ---------------------------------------------------------------------------
 // batched Gauss-Jordan inverse with partial pivoting
 // ---------------------------------------------------------------------------
-// For a panel J = [p0, p0+w):
-//   1. copy the not-yet-pivoted rows of the panel to LDS and factor them
-//      with partial pivoting (pivot = max |re|+|im|, first index on ties);
-//   2. apply the w row swaps to the whole matrix, record ipiv;
-//   3. X = (pivot block)^{-1} from its LU;
-//   4. T = X * M[J, :] with T[:, J] = X, staged in LDS (w x n);
-//   5. every row i not in J:  M[i, j] = [j not in J] M[i, j] - sum_k M[i, p0+k] T[k, j]
-//      rows in J:             M[J, :] = T.
-// Each wave owns whole 32-row tiles in step 5, so the reads of M[i, J]
-// (the multipliers) precede all writes of row i.  At the end, the column
-// interchanges are undone in reverse order:  A^{-1} = (P A)^{-1} P.
+// Rows are never moved while the block is reduced.  Column j is eliminated
+// with pivot row r_j (largest |re|+|im| among the rows not used yet, first
+// index on ties); the in-place Gauss-Jordan update then leaves in column j of
+// M the column r_j of the transformed identity, so that at the end
+//     A^{-1}[k, l] = M[r_k, r^{-1}(l)],
+// which is applied once, through the caller's scratch block.
+// For a panel J = [p0, p0+w) with pivot rows P = (r_p0, ..., r_{p0+w-1}):
+//   1. the w panel columns (all rows) are copied to LDS and factored there
+//      with partial pivoting over the unused rows;
+//   2. X = M[P, J]^{-1} from that LU;
+//   3. T = X * M[P, :], with T[:, J] = X, staged in LDS (w x n);
+//   4. rows i not in P:  M[i, j] = [j not in J] M[i, j] - sum_k M[i, p0+k] T[k, j];
+//      rows P:           M[P, :] = T.
+// Each wave owns whole 32-row tiles in step 4, so the reads of M[i, J] (the
+// multipliers) precede all writes of row i.
 template <class T, int WP, int NMAX>
 __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __restrict__ descs) {
   using A4 = typename acc4<T>::type;
@@ -158,34 +162,34 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
   __shared__ cx<T> sX[WP * WP];          // column-major X
   __shared__ T redv[4];
   __shared__ int redi[4];
-  __shared__ int sPiv[WP];
+  __shared__ int sPiv[WP];               // pivot rows of the current panel
+  __shared__ short sRole[NMAX];          // -1 unused, -2 used earlier, c >= 0: pivot c of this panel
   __shared__ int sSing;
 
   const InvDesc<T> d = descs[blockIdx.x];
   const int n = d.n;
   const int64_t ld = d.ld;
   cx<T>* __restrict__ M = d.M;
+  int* __restrict__ rowof = d.ipiv;      // r_j: pivot row of column j
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
   if (tid == 0) sSing = 0;
+  for (int r = tid; r < n; r += 256) sRole[r] = -1;
 
   for (int p0 = 0; p0 < n; p0 += WP) {
     const int w = min(WP, n - p0);
-    const int rows = n - p0;
     __syncthreads();
-    // 1. panel -> LDS (zero the unused columns)
-    for (int idx = tid; idx < WP * rows; idx += 256) {
-      const int c = idx / rows, r = idx - c * rows;
-      cx<T> v = czero<T>();
-      if (c < w) v = M[(int64_t)(p0 + c) * ld + p0 + r];
-      sW[c * LDP + r] = v;
+    // 1. panel (all rows) -> LDS, zero the unused columns
+    for (int idx = tid; idx < WP * n; idx += 256) {
+      const int c = idx / n, r = idx - c * n;
+      sW[c * LDP + r] = (c < w) ? M[(int64_t)(p0 + c) * ld + r] : czero<T>();
     }
     __syncthreads();
     for (int c = 0; c < w; ++c) {
-      // pivot search over rows c..rows-1 of column c
       T best = T(-1);
       int bidx = 0x7fffffff;
-      for (int r = c + tid; r < rows; r += 256) {
+      for (int r = tid; r < n; r += 256) {
+        if (sRole[r] != -1) continue;
         const T v = cabs1(sW[c * LDP + r]);
         if (v > best) { best = v; bidx = r; }
       }
@@ -197,75 +201,67 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
       }
       if (lane == 0) { redv[wave] = best; redi[wave] = bidx; }
       __syncthreads();
+      // every thread reduces the four wave results itself (no second barrier)
+      T bv = redv[0];
+      int p = redi[0];
+#pragma unroll
+      for (int q = 1; q < 4; ++q)
+        if (redv[q] > bv || (redv[q] == bv && redi[q] < p)) { bv = redv[q]; p = redi[q]; }
+      if (p < 0 || p >= n) {   // no finite candidate left: take the first unused row
+        p = 0;
+        while (p < n && sRole[p] != -1) ++p;
+        if (p >= n) p = 0;
+      }
       if (tid == 0) {
-        T bv = redv[0];
-        int bi = redi[0];
-        for (int q = 1; q < 4; ++q)
-          if (redv[q] > bv || (redv[q] == bv && redi[q] < bi)) { bv = redv[q]; bi = redi[q]; }
-        if (bi < c || bi >= rows) bi = c;
-        sPiv[c] = bi;
+        sPiv[c] = p;
         if (!(bv > T(0)) && sSing == 0) sSing = p0 + c + 1;
       }
-      __syncthreads();
-      const int piv = sPiv[c];
-      if (piv != c && tid < WP) {
-        cx<T> t0 = sW[tid * LDP + c];
-        sW[tid * LDP + c] = sW[tid * LDP + piv];
-        sW[tid * LDP + piv] = t0;
-      }
-      __syncthreads();
-      cx<T> pv = sW[c * LDP + c];
+      cx<T> pv = sW[c * LDP + p];
       T den = pv.re * pv.re + pv.im * pv.im;
       if (!(den > T(0))) { pv = cx<T>{tiny_pivot<T>(), T(0)}; den = pv.re * pv.re; }
       const cx<T> ipv{pv.re / den, -pv.im / den};
-      for (int r = c + 1 + tid; r < rows; r += 256) {
+      for (int r = tid; r < n; r += 256) {
+        if (r == p || sRole[r] != -1) continue;
         const cx<T> l = cmul(sW[c * LDP + r], ipv);
         sW[c * LDP + r] = l;
         for (int j = c + 1; j < w; ++j) {
-          const cx<T> u = sW[j * LDP + c];
+          const cx<T> u = sW[j * LDP + p];
           cx<T> x = sW[j * LDP + r];
           x.re -= l.re * u.re - l.im * u.im;
           x.im -= l.re * u.im + l.im * u.re;
           sW[j * LDP + r] = x;
         }
       }
+      __syncthreads();                 // all reads of sRole for this column done
+      if (tid == 0) sRole[p] = (short)c;
       __syncthreads();
     }
-    // 2. row interchanges on the whole matrix
-    for (int col = tid; col < n; col += 256) {
-      cx<T>* mc = M + (int64_t)col * ld + p0;
-      for (int c = 0; c < w; ++c) {
-        const int piv = sPiv[c];
-        if (piv != c) {
-          cx<T> t0 = mc[c];
-          mc[c] = mc[piv];
-          mc[piv] = t0;
-        }
-      }
-    }
-    if (tid < w) d.ipiv[p0 + tid] = p0 + sPiv[tid];
-    // 3. X = (L U)^{-1}, column tid
+    if (tid < w) rowof[p0 + tid] = sPiv[tid];
+    // 2. X = (pivot block)^{-1}; the LU of M[P, J] is in the panel rows P:
+    //    U(i, k) = sW[k][P_i] (k >= i),  L(i, k) = sW[k][P_i] (k < i, unit diagonal)
     if (tid < w) {
       cx<T> y[WP];
 #pragma unroll
       for (int i = 0; i < WP; ++i) {
         if (i >= w) { y[i] = czero<T>(); continue; }
-        cx<T> s = (i == tid) ? cx<T>{T(1), T(0)} : czero<T>();
+        cx<T> sacc = (i == tid) ? cx<T>{T(1), T(0)} : czero<T>();
+        const int pi = sPiv[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) s = csub(s, cmul(sW[k * LDP + i], y[k]));
-        y[i] = s;
+        for (int k = 0; k < i; ++k) sacc = csub(sacc, cmul(sW[k * LDP + pi], y[k]));
+        y[i] = sacc;
       }
 #pragma unroll
       for (int ii = WP - 1; ii >= 0; --ii) {
         if (ii >= w) continue;
-        cx<T> s = y[ii];
+        const int pi = sPiv[ii];
+        cx<T> sacc = y[ii];
 #pragma unroll
         for (int k = ii + 1; k < WP; ++k)
-          if (k < w) s = csub(s, cmul(sW[k * LDP + ii], y[k]));
-        cx<T> u = sW[ii * LDP + ii];
+          if (k < w) sacc = csub(sacc, cmul(sW[k * LDP + pi], y[k]));
+        cx<T> u = sW[ii * LDP + pi];
         T den = u.re * u.re + u.im * u.im;
         if (!(den > T(0))) { u = cx<T>{tiny_pivot<T>(), T(0)}; den = u.re * u.re; }
-        y[ii] = cmul(s, cx<T>{u.re / den, -u.im / den});
+        y[ii] = cmul(sacc, cx<T>{u.re / den, -u.im / den});
       }
 #pragma unroll
       for (int i = 0; i < WP; ++i) sX[tid * WP + i] = y[i];
@@ -274,26 +270,26 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
       for (int i = 0; i < WP; ++i) sX[tid * WP + i] = czero<T>();
     }
     __syncthreads();
-    // 4. T = X * M[J, :], T[:, J] = X ; rows k >= w of T are zero
+    // 3. T = X * M[P, :], T[:, J] = X ; rows k >= w of T are zero
     for (int j = tid; j < n; j += 256) {
       if (j >= p0 && j < p0 + w) {
         for (int k = 0; k < WP; ++k) sW[k * LDT + j] = (k < w) ? sX[(j - p0) * WP + k] : czero<T>();
       } else {
         cx<T> v[WP];
-        const cx<T>* mc = M + (int64_t)j * ld + p0;
+        const cx<T>* mc = M + (int64_t)j * ld;
 #pragma unroll
-        for (int k = 0; k < WP; ++k) v[k] = (k < w) ? mc[k] : czero<T>();
+        for (int k = 0; k < WP; ++k) v[k] = (k < w) ? mc[sPiv[k]] : czero<T>();
 #pragma unroll
         for (int k = 0; k < WP; ++k) {
-          cx<T> s = czero<T>();
+          cx<T> sacc = czero<T>();
 #pragma unroll
-          for (int q = 0; q < WP; ++q) s = cfma(s, sX[q * WP + k], v[q]);
-          sW[k * LDT + j] = s;
+          for (int q = 0; q < WP; ++q) sacc = cfma(sacc, sX[q * WP + k], v[q]);
+          sW[k * LDT + j] = sacc;
         }
       }
     }
     __syncthreads();
-    // 5. rank-w update, one wave per 32-row tile
+    // 4. rank-w update, one wave per 32-row tile
     const int ntr = (n + 31) / 32, ntc = (n + 31) / 32;
     for (int tr = wave; tr < ntr; tr += 4) {
       const int i0 = tr * 32;
@@ -337,38 +333,40 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
               const int gi = i0 + 16 * rs + l16;
               const int gj = j0 + 16 * cs + mfma_mrow<T>(lq, r);
               if (gi < n && gj < n) {
-                cx<T>* c = M + (int64_t)gj * ld + gi;
+                cx<T>* cp = M + (int64_t)gj * ld + gi;
+                const int role = sRole[gi];
                 cx<T> v;
-                if (gi >= p0 && gi < p0 + w) {
-                  v = sW[(gi - p0) * LDT + gj];
+                if (role >= 0 && role < WP) {
+                  v = sW[role * LDT + gj];
                 } else if (gj >= p0 && gj < p0 + w) {
                   v = cx<T>{-cre[rs][cs][r], -cim[rs][cs][r]};
                 } else {
-                  v = *c;
+                  v = *cp;
                   v.re -= cre[rs][cs][r];
                   v.im -= cim[rs][cs][r];
                 }
-                *c = v;
+                *cp = v;
               }
             }
       }
     }
+    __syncthreads();
+    // this panel's pivot rows become "used earlier"
+    for (int c = tid; c < w; c += 256) sRole[sPiv[c]] = -2;
   }
   __syncthreads();
-  // undo the column interchanges in reverse order
-  for (int p = n - 1; p >= 0; --p) {
-    const int q = d.ipiv[p];
-    if (q != p) {
-      cx<T>* a = M + (int64_t)p * ld;
-      cx<T>* b = M + (int64_t)q * ld;
-      for (int r = tid; r < n; r += 256) {
-        cx<T> t0 = a[r];
-        a[r] = b[r];
-        b[r] = t0;
-      }
-    }
-    __syncthreads();
+  // A^{-1}[k, l] = M[r_k, r^{-1}(l)]: permute into the scratch block, copy back
+  int* rinv = reinterpret_cast<int*>(sX);          // n ints fit in sX (WP*WP complex >= NMAX ints)
+  for (int j = tid; j < n; j += 256) rinv[rowof[j]] = j;
+  __syncthreads();
+  cx<T>* __restrict__ S = d.scratch;
+  for (int l = 0; l < n; ++l) {
+    const cx<T>* src = M + (int64_t)rinv[l] * ld;
+    for (int k = tid; k < n; k += 256) S[(int64_t)l * n + k] = src[rowof[k]];
   }
+  __syncthreads();
+  for (int l = 0; l < n; ++l)
+    for (int k = tid; k < n; k += 256) M[(int64_t)l * ld + k] = S[(int64_t)l * n + k];
   if (tid == 0 && d.info) *d.info = sSing;
 }
 

@@ -162,7 +162,12 @@ template hipError_t launch_build_blocks<float>(const CsrDev&, double, double, co
 // Workgroup: 64 rows x 4 k-slices.  Lanes of a wave share the k index, so the
 // X entries they read from LDS are broadcasts and the A column segment they
 // read from global memory is contiguous.
-template <int NR> struct SkChunk { static constexpr int value = NR >= 16 ? 128 : (NR >= 4 ? 256 : 512); };
+template <int NR> struct SkChunk { static constexpr int value = NR >= 8 ? 128 : (NR >= 4 ? 256 : 512); };
+template <int NR> struct SkLds {
+  static constexpr int x = SkChunk<NR>::value * NR;   // staged X chunk
+  static constexpr int p = 3 * 64 * NR;               // partial sums of k-slices 1..3
+  static constexpr int value = x > p ? x : p;
+};
 
 template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
@@ -172,8 +177,10 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
   if (r0 >= d.m) return;
   const int rl = threadIdx.x & 63, ks = threadIdx.x >> 6;
   const int row = r0 + rl;
-  __shared__ cx<T> sX[KC][NR];
-  __shared__ cx<T> sP[3][64][NR];
+  // one LDS region: the X chunk during the sweep, then the slice partial sums
+  __shared__ cx<T> lds[SkLds<NR>::value];
+  cx<T>(*sX)[NR] = reinterpret_cast<cx<T>(*)[NR]>(lds);
+  cx<T>(*sP)[64][NR] = reinterpret_cast<cx<T>(*)[64][NR]>(lds);
   cx<T> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = czero<T>();

@@ -25,8 +25,9 @@ template <class T> struct InvDesc {
   cx<T>* M;
   int64_t ld;
   int n;
-  int* ipiv;   // n ints of device scratch
-  int* info;   // 0 or (1-based column of the first zero pivot)
+  int* ipiv;        // n ints of device scratch (pivot row of each column)
+  int* info;        // 0 or (1-based column of the first zero pivot)
+  cx<T>* scratch;   // n x n block used for the final permutation
 };
 
 // Y = beta*Y + sum_p alpha_p A_p X_p  (p < 3) for skinny X (nrhs <= 16

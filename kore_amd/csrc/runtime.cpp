@@ -334,8 +334,8 @@ struct kore_sys {
   double* part = nullptr;
   int npart = 64;
   std::unique_ptr<FactorBase> f;
+  bool factored = false;     // f holds a valid factorisation of A - sigma B
   double sre = 0, sim = 0;
-  double normRM = 0;   // ||R (A - sigma B)||_F
   std::vector<double> h_scale;
   Uploader up;
 };
@@ -495,7 +495,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     rinv[l].off = vinv.size();
     for (int t = 1; t < T_; t += 2) {
       const int k = id[t];
-      vinv.push_back(InvDesc<T>{F->D[k], nk(k), (int)nk(k), ipiv + (size_t)k * nmax, F->info + k});
+      vinv.push_back(InvDesc<T>{F->D[k], nk(k), (int)nk(k), ipiv + (size_t)k * nmax, F->info + k, F->G[k]});
       flops += 8.0 * nk(k) * nk(k) * nk(k);
       F->flops_inv += 8.0 * nk(k) * nk(k) * nk(k);
       L.maxn_inv = std::max(L.maxn_inv, (int)nk(k));
@@ -598,7 +598,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   // root
   const int kr = ids[nlev][0];
   rinv[nlev].off = vinv.size();
-  vinv.push_back(InvDesc<T>{F->D[kr], nk(kr), (int)nk(kr), ipiv + (size_t)kr * nmax, F->info + kr});
+  vinv.push_back(InvDesc<T>{F->D[kr], nk(kr), (int)nk(kr), ipiv + (size_t)kr * nmax, F->info + kr, F->G[kr]});
   rinv[nlev].n = 1;
   flops += 8.0 * nk(kr) * nk(kr) * nk(kr);
   F->flops_inv += 8.0 * nk(kr) * nk(kr) * nk(kr);
@@ -776,7 +776,7 @@ static int32_t solve_dev(kore_sys* s, const cx<double>* b, int64_t ldb, cx<doubl
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int64_t n = s->n;
-  if (!s->f) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
+  if (!s->f || !s->factored) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
   if (nrhs < 1 || nrhs > MAXR) return fail(KORE_ERR_ARG, "nrhs must be 1..16 per device solve");
   if (info) {
     info->iterations = 0;
@@ -1044,55 +1044,49 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
   if (precision != KORE_PREC_FP64 && precision != KORE_PREC_C64) return fail(KORE_ERR_ARG, "bad precision");
   kore_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
-  s->f.reset();
+  // the reduction plan and its buffers depend only on the layout: keep them
+  // across shifts of the same system, rebuild only when the precision changes
+  if (s->f && s->f->prec != precision) s->f.reset();
   s->sre = sre;
   s->sim = sim;
-  // power-of-two row equilibration of A - sigma B and ||R(A - sigma B)||_F
-  const zd sg(sre, sim);
-  s->h_scale.assign(s->n, 1.0);
-  double nrm = 0;
-  for (int64_t i = 0; i < s->n; ++i) {
-    double mx = 0;
-    for (int32_t k = s->h_ip[i]; k < s->h_ip[i + 1]; ++k) mx = std::max(mx, std::abs(s->h_va[k] - sg * s->h_vb[k]));
-    double sc = 1.0;
-    if (mx > 0 && std::isfinite(mx)) sc = std::ldexp(1.0, -(int)std::nearbyint(std::log2(mx)));
-    s->h_scale[i] = sc;
-    for (int32_t k = s->h_ip[i]; k < s->h_ip[i + 1]; ++k) nrm += sc * sc * std::norm(s->h_va[k] - sg * s->h_vb[k]);
-  }
-  s->normRM = std::sqrt(nrm);
-  HIP_TRY(hipMemcpyAsync(s->d_scale.p, s->h_scale.data(), s->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // power-of-two row equilibration of A - sigma B (device)
+  HIP_TRY(launch_row_scale(s->csr, sre, sim, s->d_scale.as<double>(), c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   int32_t rc = KORE_OK;
   std::vector<int> hinfo(s->nblk, 0);
+  const int* dinfo = nullptr;
   if (precision == KORE_PREC_FP64) {
-    auto F = std::make_unique<Factor<double>>();
-    F->prec = precision;
-    rc = factor_build<double>(s, F.get());
-    if (rc >= 0) {
-      hipEventRecord(e0, c->stream);
-      rc = factor_run<double>(s, F.get());
-      hipEventRecord(e1, c->stream);
+    if (!s->f) {
+      auto F = std::make_unique<Factor<double>>();
+      F->prec = precision;
+      rc = factor_build<double>(s, F.get());
+      if (rc >= 0) s->f = std::move(F);
     }
     if (rc >= 0) {
-      hipMemcpyAsync(hinfo.data(), F->info, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-      s->f = std::move(F);
+      Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
+      hipEventRecord(e0, c->stream);
+      rc = factor_run<double>(s, F);
+      hipEventRecord(e1, c->stream);
+      dinfo = F->info;
     }
   } else {
-    auto F = std::make_unique<Factor<float>>();
-    F->prec = precision;
-    rc = factor_build<float>(s, F.get());
-    if (rc >= 0) {
-      hipEventRecord(e0, c->stream);
-      rc = factor_run<float>(s, F.get());
-      hipEventRecord(e1, c->stream);
+    if (!s->f) {
+      auto F = std::make_unique<Factor<float>>();
+      F->prec = precision;
+      rc = factor_build<float>(s, F.get());
+      if (rc >= 0) s->f = std::move(F);
     }
     if (rc >= 0) {
-      hipMemcpyAsync(hinfo.data(), F->info, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-      s->f = std::move(F);
+      Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
+      hipEventRecord(e0, c->stream);
+      rc = factor_run<float>(s, F);
+      hipEventRecord(e1, c->stream);
+      dinfo = F->info;
     }
   }
+  if (rc >= 0 && dinfo) hipMemcpyAsync(hinfo.data(), dinfo, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
   hipError_t se = hipStreamSynchronize(c->stream);
   float ms = 0.f;
   if (rc >= 0 && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
@@ -1106,9 +1100,10 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     s->f.reset();
     return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
   }
+  s->factored = true;
   for (int k = 0; k < s->nblk; ++k)
     if (hinfo[k] != 0) {
-      s->f.reset();
+      s->factored = false;
       return fail(KORE_ERR_SINGULAR, "zero pivot in degree block %d (column %d) during cyclic reduction", k, hinfo[k]);
     }
   if (info) {
@@ -1129,6 +1124,7 @@ int32_t kore_factor_release(kore_sys* s) {
   if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
   hipStreamSynchronize(s->ctx->stream);
   s->f.reset();
+  s->factored = false;
   return KORE_OK;
 }
 

@@ -62,6 +62,29 @@ class EigResult:
     op_calls: int              # block applications
     seconds: float
     ir_max_iterations: int = 0
+    host_seconds: dict = dataclasses.field(default_factory=dict)   # wall time per workspace primitive
+
+
+class _Timed:
+    """Wraps a workspace so that the wall time of each primitive is summed."""
+
+    def __init__(self, ws, acc):
+        self._ws = ws
+        self._acc = acc
+
+    def __getattr__(self, name):
+        f = getattr(self._ws, name)
+        if not callable(f):
+            return f
+        acc = self._acc
+
+        def g(*a, **k):
+            t = time.perf_counter()
+            try:
+                return f(*a, **k)
+            finally:
+                acc[name] = acc.get(name, 0.0) + time.perf_counter() - t
+        return g
 
 
 def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
@@ -98,13 +121,26 @@ def _triu_eigvecs(S):
     return Z
 
 
-def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, seed=0,
-         ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=Workspace) -> EigResult:
+def eigs(*args, **kwargs) -> EigResult:
+    """See ``_eigs``.  The projected problem is tiny (at most a few hundred
+    columns): multithreaded BLAS only adds start-up cost to each call, so the
+    host side runs with one BLAS thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:            # pragma: no cover
+        return _eigs(*args, **kwargs)
+    with threadpool_limits(limits=1):
+        return _eigs(*args, **kwargs)
+
+
+def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, seed=0,
+          ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None) -> EigResult:
     """Eigenpairs nearest ``sigma`` of the pencil held by ``sysm``.
 
     ``sysm`` must have been factored at ``sigma`` (System.factor).
-    ``workspace`` constructs the vector block (the device Workspace; tests of
-    the host logic pass a stand-in with the same methods).
+    ``workspace`` constructs the vector block; by default the system's cached
+    device Workspace is used (tests of the host logic pass a stand-in class
+    with the same methods).
     """
     t0 = time.perf_counter()
     sigma = complex(sigma)
@@ -120,7 +156,14 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
     # columns: [0, m+bs) basis + residual block ; [m+bs, m+2bs) new block ; [m+2bs, 2m+2bs) restart temp
     cW = m + bs
     cT = m + 2 * bs
-    ws = workspace(sysm, 2 * m + 2 * bs)
+    if workspace is None:
+        ws_raw = sysm.get_workspace(2 * m + 2 * bs)
+        own_ws = False
+    else:
+        ws_raw = workspace(sysm, 2 * m + 2 * bs)
+        own_ws = True
+    host = {}
+    ws = _Timed(ws_raw, host)
     rng = np.random.default_rng(seed)
     if v0 is None:
         X0 = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
@@ -213,10 +256,13 @@ def eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, 
         X = ws.get(cT, k)
         X = X / np.linalg.norm(X, axis=0)
     lam = sigma + 1.0 / theta[:k]
-    ws.close()
+    if own_ws:
+        ws.close()
+    total = time.perf_counter() - t0
+    host["projected"] = total - sum(host.values())
     return EigResult(lam=lam, X=X, errest=res[:k] / np.abs(theta[:k]), nconv=k, restarts=restarts,
-                     op_columns=op_cols, op_calls=op_calls, seconds=time.perf_counter() - t0,
-                     ir_max_iterations=ir_iters)
+                     op_columns=op_cols, op_calls=op_calls, seconds=total,
+                     ir_max_iterations=ir_iters, host_seconds=host)
 
 
 def residuals(A, B, lam, X):

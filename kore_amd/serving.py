@@ -1,0 +1,253 @@
+"""Request loop for a stream of Kore solves on one MI355X (continuous batching).
+
+A request is one Kore problem (m, E, N, lmax, ...) plus either a forced solve
+at frequency w (the system A(w) = A0 - i w B0, bin/assemble.py:623 vs 445,
+with a seeded unit-norm complex right-hand side) or an eigensolve with target
+sigma (nev pairs nearest sigma).  Both factor A0 - sigma B0 (sigma = i w for
+forced solves), so they share one kernel set.
+
+Pipeline:
+
+* CPU assembly workers build (A0, B0) for queued requests ahead of the GPU;
+* admission: a request enters a device slot when a slot is free and its
+  predicted device footprint (library estimate of the factor bytes plus the
+  system and Krylov workspace) fits in the HBM budget;
+* each slot owns a device context (its own HIP stream): requests in flight
+  run concurrently and their kernels overlap on the device; a finished
+  request frees its slot and memory at once and the next admissible request
+  starts (continuous batching, not a FIFO of whole batches);
+* results are checked (forced: ||A x - b|| / (||A||_F ||x|| + ||b||);
+  eigen: ||(A - lam B) x|| / (||A||_F ||x||)), and latency is measured
+  from arrival to result on the host.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import queue
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from typing import List, Optional
+
+import numpy as np
+import scipy.sparse as ss
+
+from . import assembly, backend, eigs
+from .params import Params
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str                  # "forced" or "eigen"
+    m: int
+    Ek: float
+    N: int
+    lmax: int
+    omega: float               # forcing frequency; eigen target is sigma = i*omega
+    nev: int = 6
+    seed: int = 0
+    arrival: float = 0.0       # seconds after the trace start
+
+    @property
+    def sigma(self) -> complex:
+        return 1j * self.omega
+
+    def params(self) -> Params:
+        return Params(m=self.m, symm=1, ricb=0.35, bci=1, bco=1, Ek=self.Ek, N=self.N, lmax=self.lmax)
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    kind: str
+    latency: float
+    service: float
+    residual: float
+    ok: bool
+    lam: Optional[np.ndarray] = None
+    x: Optional[np.ndarray] = None
+    precision: int = 0
+    ir_sweeps: int = 0
+    error: str = ""
+
+
+def serving_trace(nreq=1024, seed=1234, e_range=(1e-7, 1e-5), n_cap=500, forced_frac=0.7, rate=None, nev=6,
+                  m_values=(0, 1, 2, 3, 4)) -> List[Request]:
+    """The serving workload: m ~ U{0..4}, log10 E ~ U, N = min(ceil(8 E^-1/3), cap),
+    lmax = ceil(1.2 N) rounded up so that lmax - m + 1 is even, w ~ U[-2, 2],
+    70 % forced / 30 % eigen, Poisson arrivals at ``rate`` (None: all at t=0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(nreq):
+        m = int(rng.choice(m_values))
+        E = float(10 ** rng.uniform(math.log10(e_range[0]), math.log10(e_range[1])))
+        N = int(min(math.ceil(8 * E ** (-1.0 / 3.0)), n_cap))
+        N += N % 2
+        lmax = int(math.ceil(1.2 * N))
+        lmax += (lmax - m + 1) % 2
+        w = float(rng.uniform(-2, 2))
+        kind = "forced" if rng.uniform() < forced_frac else "eigen"
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        out.append(Request(i, kind, m, E, N, lmax, w, nev=nev, seed=int(rng.integers(1 << 31)), arrival=t))
+    return out
+
+
+def rhs_for(req: Request, n: int) -> np.ndarray:
+    rng = np.random.default_rng(req.seed)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2.0)
+    return b / np.linalg.norm(b)
+
+
+def footprint(par: Params, precision: int, nev: int = 6, block: int = 8) -> int:
+    """Predicted device bytes of one request (factor + system + workspace)."""
+    lay = backend.layout_from_params(par)
+    fb = backend.factor_bytes(lay, precision)
+    n = par.sizmat
+    nnz_est = 48 * n
+    sysb = nnz_est * (4 + 32) + n * (8 + 4 + 8) + 4 * n * 16 * 16
+    ncv = max(2 * nev, nev + 15, 10 * block)
+    wsb = n * (2 * ncv + 4 * block) * 16
+    return int(fb + sysb + wsb)
+
+
+def assemble_request(req: Request):
+    par = req.params()
+    A0, B0 = assembly.assemble(par, which="unscaled")
+    return par, A0, B0
+
+
+def residual(A0, B0, req: Request, x=None, lam=None, X=None, b=None):
+    nA = np.sqrt(np.sum(np.abs((A0 - req.sigma * B0).data) ** 2)) if req.kind == "forced" else \
+        np.sqrt(np.sum(np.abs(A0.data) ** 2))
+    if req.kind == "forced":
+        M = A0 - req.sigma * B0
+        r = M @ x - b
+        return float(np.linalg.norm(r) / (nA * np.linalg.norm(x) + np.linalg.norm(b)))
+    return float(eigs.residuals(A0, B0, lam, X).max()) if len(lam) else float("inf")
+
+
+class Server:
+    """Continuous-batching request loop over ``slots`` device contexts."""
+
+    def __init__(self, device=0, slots=4, mem_budget=200 << 30, precision=backend.PREC_C64, block=8,
+                 assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None):
+        self.device = device
+        self.slots = slots
+        self.mem_budget = int(mem_budget)
+        self.precision = precision
+        self.block = block
+        self.tol = tol
+        self.ir_tol = ir_tol
+        self.keep_results = keep_results
+        self.log = log
+        self.ctxs = [backend.Context(device) for _ in range(slots)]
+        self.free_slots = queue.Queue()
+        for i in range(slots):
+            self.free_slots.put(i)
+        self.mem_in_use = 0
+        self.mem_peak = 0
+        self.max_inflight = 0
+        self._inflight = 0
+        self._cv = threading.Condition()
+        self.asm_pool = ThreadPoolExecutor(max_workers=assembly_workers)
+
+    def close(self):
+        self.asm_pool.shutdown(wait=True)
+        for c in self.ctxs:
+            c.close()
+
+    # -- one request on one slot ------------------------------------------------
+    def _execute(self, slot, req: Request, par, A0, B0, t0) -> Result:
+        ctx = self.ctxs[slot]
+        t_start = time.perf_counter()
+        lay = backend.layout_from_params(par)
+        S = backend.System(ctx, lay, A0, B0)
+        try:
+            S.factor(req.sigma, self.precision)
+            if req.kind == "forced":
+                b = rhs_for(req, S.n)
+                x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=20 if self.precision else 0)
+                res = residual(A0, B0, req, x=x, b=b)
+                out = Result(req.rid, req.kind, 0.0, 0.0, res, res <= 1e-10, x=x if self.keep_results else None,
+                             precision=self.precision, ir_sweeps=info.iterations)
+            else:
+                r = eigs.eigs(S, req.sigma, nev=req.nev, tol=self.tol, block=self.block, ir_tol=self.ir_tol)
+                k = min(req.nev, r.nconv)
+                res = residual(A0, B0, req, lam=r.lam[:k], X=r.X[:, :k])
+                out = Result(req.rid, req.kind, 0.0, 0.0, res, (res <= 1e-10) and r.nconv >= req.nev,
+                             lam=r.lam[:k] if True else None, x=r.X[:, :k] if self.keep_results else None,
+                             precision=self.precision, ir_sweeps=r.ir_max_iterations)
+        finally:
+            S.close()
+        now = time.perf_counter()
+        out.service = now - t_start
+        out.latency = now - (t0 + req.arrival)
+        return out
+
+    def run(self, trace: List[Request]) -> List[Result]:
+        """Serve the trace (arrival times relative to now); returns results by id."""
+        t0 = time.perf_counter()
+        results = {}
+        lock = threading.Lock()
+        # assembly runs ahead: submit in arrival order
+        futs = {}
+        for req in sorted(trace, key=lambda r: r.arrival):
+            futs[req.rid] = self.asm_pool.submit(assemble_request, req)
+        threads = []
+
+        def worker(slot, req, par, A0, B0, need):
+            try:
+                res = self._execute(slot, req, par, A0, B0, t0)
+            except Exception as e:  # report, keep serving
+                res = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0, float("inf"), False,
+                             error=repr(e))
+            with lock:
+                results[req.rid] = res
+            with self._cv:
+                self.mem_in_use -= need
+                self._inflight -= 1
+                self.free_slots.put(slot)
+                self._cv.notify_all()
+            if self.log:
+                self.log("req %d %s done: latency %.3f s, service %.3f s, residual %.1e" % (
+                    req.rid, req.kind, res.latency, res.service, res.residual))
+
+        for req in sorted(trace, key=lambda r: r.arrival):
+            # wait for the arrival time
+            dt = t0 + req.arrival - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            par, A0, B0 = futs[req.rid].result()
+            need = footprint(par, self.precision, req.nev, self.block)
+            if need > self.mem_budget:
+                results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
+                                          error="request needs %d bytes, budget %d" % (need, self.mem_budget))
+                continue
+            with self._cv:
+                while self.free_slots.empty() or self.mem_in_use + need > self.mem_budget:
+                    self._cv.wait()
+                slot = self.free_slots.get()
+                self.mem_in_use += need
+                self.mem_peak = max(self.mem_peak, self.mem_in_use)
+                self._inflight += 1
+                self.max_inflight = max(self.max_inflight, self._inflight)
+            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need), daemon=True)
+            th.start()
+            threads.append(th)
+            futs.pop(req.rid, None)
+        for th in threads:
+            th.join()
+        return [results[r.rid] for r in trace]
+
+
+def summarize(results: List[Result], wall: float):
+    lat = np.array([r.latency for r in results])
+    ok = np.array([r.ok for r in results])
+    return dict(requests=len(results), ok=int(ok.sum()), solves_per_s=len(results) / wall,
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_service_s=float(np.mean([r.service for r in results])),
+                worst_residual=float(max(r.residual for r in results)))
