@@ -274,7 +274,7 @@ class System:
         info = SolveInfo()
         rc = lib().kore_solve(self._p, bb.shape[1], _dptr(bb), self.n,
                               _dptr(x), self.n, float(ir_tol), int(ir_maxit), C.byref(info))
-        _check(rc)
+        _check(rc)      # 1 (refinement not converged) is reported through info.converged
         return (x[:, 0] if one else x), info
 
     def close(self):

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
         const cx<T>* mc = M + (int64_t)j * ld;
 #pragma unroll
         for (int k = 0; k < WP; ++k) v[k] = (k < w) ? mc[sPiv[k]] : czero<T>();
-#pragma unroll
+#pragma unroll 2
         for (int k = 0; k < WP; ++k) {
           cx<T> sacc = czero<T>();
 #pragma unroll

@@ -159,13 +159,16 @@ template hipError_t launch_build_blocks<float>(const CsrDev&, double, double, co
 // ---------------------------------------------------------------------------
 // batched skinny products for the solve sweeps
 // ---------------------------------------------------------------------------
-// Workgroup: 64 rows x 4 k-slices.  Lanes of a wave share the k index, so the
-// X entries they read from LDS are broadcasts and the A column segment they
-// read from global memory is contiguous.
+// Workgroup: 16 rows x 16 k-slices.  A wave covers 16 consecutive rows of 4
+// k-columns, so its A loads are four 256-byte segments; the X entries are LDS
+// broadcasts (the 16 lanes of a k-slice share k).  Short k-ranges per thread
+// keep the dependent load chains short, which matters for the deep reduction
+// levels where a launch holds only a handful of blocks.
+constexpr int SK_ROWS = 16, SK_KS = 16;
 template <int NR> struct SkChunk { static constexpr int value = NR >= 8 ? 128 : (NR >= 4 ? 256 : 512); };
 template <int NR> struct SkLds {
-  static constexpr int x = SkChunk<NR>::value * NR;   // staged X chunk
-  static constexpr int p = 3 * 64 * NR;               // partial sums of k-slices 1..3
+  static constexpr int x = SkChunk<NR>::value * NR;           // staged X chunk
+  static constexpr int p = (SK_KS - 1) * SK_ROWS * NR;         // partial sums of k-slices 1..15
   static constexpr int value = x > p ? x : p;
 };
 
@@ -173,14 +176,14 @@ template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
   const SkinnyDesc<T> d = descs[blockIdx.y];
-  const int r0 = blockIdx.x * 64;
+  const int r0 = blockIdx.x * SK_ROWS;
   if (r0 >= d.m) return;
-  const int rl = threadIdx.x & 63, ks = threadIdx.x >> 6;
+  const int rl = threadIdx.x & (SK_ROWS - 1), ks = threadIdx.x / SK_ROWS;
   const int row = r0 + rl;
   // one LDS region: the X chunk during the sweep, then the slice partial sums
   __shared__ cx<T> lds[SkLds<NR>::value];
   cx<T>(*sX)[NR] = reinterpret_cast<cx<T>(*)[NR]>(lds);
-  cx<T>(*sP)[64][NR] = reinterpret_cast<cx<T>(*)[64][NR]>(lds);
+  cx<T>(*sP)[SK_ROWS][NR] = reinterpret_cast<cx<T>(*)[SK_ROWS][NR]>(lds);
   cx<T> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = czero<T>();
@@ -208,21 +211,20 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
       if (row < d.m) {
         const cx<T>* __restrict__ Ac = A + (int64_t)kc0 * lda + row;
         int kk = ks;
-        // four independent loads in flight per thread
-        for (; kk + 12 < kend; kk += 16) {
+        for (; kk + 3 * SK_KS < kend; kk += 4 * SK_KS) {
           const cx<T> a0 = Ac[(int64_t)kk * lda];
-          const cx<T> a1 = Ac[(int64_t)(kk + 4) * lda];
-          const cx<T> a2 = Ac[(int64_t)(kk + 8) * lda];
-          const cx<T> a3 = Ac[(int64_t)(kk + 12) * lda];
+          const cx<T> a1 = Ac[(int64_t)(kk + SK_KS) * lda];
+          const cx<T> a2 = Ac[(int64_t)(kk + 2 * SK_KS) * lda];
+          const cx<T> a3 = Ac[(int64_t)(kk + 3 * SK_KS) * lda];
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
             acc[c] = cfma(acc[c], a0, sX[kk][c]);
-            acc[c] = cfma(acc[c], a1, sX[kk + 4][c]);
-            acc[c] = cfma(acc[c], a2, sX[kk + 8][c]);
-            acc[c] = cfma(acc[c], a3, sX[kk + 12][c]);
+            acc[c] = cfma(acc[c], a1, sX[kk + SK_KS][c]);
+            acc[c] = cfma(acc[c], a2, sX[kk + 2 * SK_KS][c]);
+            acc[c] = cfma(acc[c], a3, sX[kk + 3 * SK_KS][c]);
           }
         }
-        for (; kk < kend; kk += 4) {
+        for (; kk < kend; kk += SK_KS) {
           const cx<T> a = Ac[(int64_t)kk * lda];
 #pragma unroll
           for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a, sX[kk][c]);
@@ -241,9 +243,8 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
     for (int c = 0; c < NR; ++c) {
       if (c < nrhs) {
         cx<T> v = acc[c];
-        v = cadd(v, sP[0][rl][c]);
-        v = cadd(v, sP[1][rl][c]);
-        v = cadd(v, sP[2][rl][c]);
+#pragma unroll
+        for (int q = 0; q < SK_KS - 1; ++q) v = cadd(v, sP[q][rl][c]);
         cx<T>* y = d.Y + (int64_t)c * d.ldy + row;
         cx<T> o = v;
         if (d.beta != T(0)) {
@@ -261,7 +262,7 @@ template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s) {
   if (ndesc <= 0 || max_m <= 0) return hipSuccess;
   if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)((max_m + 63) / 64), (unsigned)ndesc);
+  const dim3 grid((unsigned)((max_m + SK_ROWS - 1) / SK_ROWS), (unsigned)ndesc);
   if (nrhs == 1)
     hipLaunchKernelGGL((skinny_kernel<T, 1>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 2)
@@ -395,6 +396,92 @@ hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrh
   if (nrhs == 0) return hipSuccess;
   hipLaunchKernelGGL((colred_kernel<double, true>), dim3(nblocks, nrhs), dim3(256), 0, s, x, ldx, n, partial,
                      nrhs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// tall-skinny inner products  P[chunk] = V[rows, i0:i0+64]^H W[rows, 0:nb]
+// ---------------------------------------------------------------------------
+// Workgroup = 64 V columns x 4 row slices; 32-row tiles of V are staged
+// through LDS with coalesced loads (a column's rows are contiguous), W tiles
+// likewise; each thread keeps nb <= NB complex accumulators.  The chunk
+// partials are summed by sum_parts_kernel.
+template <int NB>
+__global__ __launch_bounds__(256) void tsdot_kernel(const cx<double>* __restrict__ V, int64_t ldv, int na,
+                                                    const cx<double>* __restrict__ W, int64_t ldw, int nb,
+                                                    int64_t n, int64_t rows_per_chunk,
+                                                    cx<double>* __restrict__ partial) {
+  constexpr int TR = 32;
+  constexpr int NV = TR * (64 + 1), NR3 = 3 * 64 * NB;
+  // the V tile and, after the sweep, the slice partial sums share one region
+  __shared__ cx<double> lds[NV > NR3 ? NV : NR3];
+  __shared__ cx<double> sWt[TR][NB];
+  cx<double>(*sV)[64 + 1] = reinterpret_cast<cx<double>(*)[64 + 1]>(lds);
+  cx<double>(*sRed)[64][NB] = reinterpret_cast<cx<double>(*)[64][NB]>(lds);
+  const int tid = threadIdx.x;
+  const int ci = tid & 63, rs = tid >> 6;
+  const int i0 = blockIdx.y * 64;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+  const int64_t r1 = r0 + rows_per_chunk < n ? r0 + rows_per_chunk : n;
+  cx<double> acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = czero<double>();
+  for (int64_t rt = r0; rt < r1; rt += TR) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (TR * 64) / 256; ++q) {
+      const int idx = tid + 256 * q;
+      const int col = idx / TR, row = idx - col * TR;
+      cx<double> v = czero<double>();
+      if (rt + row < r1 && i0 + col < na) v = V[(int64_t)(i0 + col) * ldv + rt + row];
+      sV[row][col] = v;
+    }
+    for (int idx = tid; idx < TR * NB; idx += 256) {
+      const int col = idx / TR, row = idx - col * TR;
+      cx<double> v = czero<double>();
+      if (rt + row < r1 && col < nb) v = W[(int64_t)col * ldw + rt + row];
+      sWt[row][col] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = rs; r < TR; r += 4) {
+      const cx<double> v = sV[r][ci];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[j] = cfma_conj(acc[j], v, sWt[r][j]);
+    }
+  }
+  __syncthreads();
+  if (rs > 0) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) sRed[rs - 1][ci][j] = acc[j];
+  }
+  __syncthreads();
+  if (rs == 0 && i0 + ci < na) {
+    cx<double>* out = partial + (int64_t)blockIdx.x * na * nb;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < nb) {
+        cx<double> t = acc[j];
+        t = cadd(t, sRed[0][ci][j]);
+        t = cadd(t, sRed[1][ci][j]);
+        t = cadd(t, sRed[2][ci][j]);
+        out[(int64_t)j * na + i0 + ci] = t;   // column-major na x nb
+      }
+    }
+  }
+}
+
+hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<double>* W, int64_t ldw, int nb,
+                        int64_t n, int nchunk, int64_t rows_per_chunk, cx<double>* partial, hipStream_t s) {
+  if (na <= 0 || nb <= 0) return hipSuccess;
+  if (nb > 16) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nchunk, (unsigned)((na + 63) / 64));
+  if (nb <= 4)
+    hipLaunchKernelGGL(tsdot_kernel<4>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);
+  else if (nb <= 8)
+    hipLaunchKernelGGL(tsdot_kernel<8>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);
+  else
+    hipLaunchKernelGGL(tsdot_kernel<16>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);
   return hipGetLastError();
 }
 

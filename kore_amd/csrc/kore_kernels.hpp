@@ -91,6 +91,9 @@ hipError_t launch_colnorm2(const cx<T>* x, int64_t ldx, int64_t n, int nrhs, dou
 // out[c] = max over chunks of per-chunk max |x| ... (max abs per column)
 hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrhs, double* partial,
                             int nblocks, hipStream_t s);
+// partial[c] (na x nb, column-major) = V[chunk c rows, :na]^H W[chunk c rows, :nb], nb <= 16
+hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<double>* W, int64_t ldw, int nb,
+                        int64_t n, int nchunk, int64_t rows_per_chunk, cx<double>* partial, hipStream_t s);
 // sum nparts consecutive arrays of len doubles: out[i] = sum_p in[p*len + i]
 hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s);
 // fill complex buffer with zeros (graph-capture-safe clear)

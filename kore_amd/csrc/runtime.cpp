@@ -1285,27 +1285,10 @@ int32_t kore_ws_dot(kore_ws* w, int32_t a0, int32_t na, int32_t b0, int32_t nb, 
   std::vector<zd> h((size_t)na * nb);
   for (int j0 = 0; j0 < nb; j0 += MAXR) {
     const int nbj = std::min(MAXR, nb - j0);
-    std::vector<GemmDesc<double>> ds;
-    for (int q = 0; q < nchunk; ++q) {
-      const int64_t k0 = q * chunk;
-      const int64_t kl = std::min<int64_t>(chunk, n - k0);
-      if (kl <= 0) break;
-      GemmDesc<double> g{};
-      g.A[0] = wcol(w, a0) + k0; g.lda[0] = n;
-      g.B[0] = wcol(w, b0 + j0) + k0; g.ldb[0] = n;
-      g.k[0] = (int)kl;
-      g.C = part + (size_t)q * na * nbj; g.ldc = na;
-      g.m = na; g.n = nbj; g.opA = 1;
-      g.tiles_m = (na + 63) / 64; g.tiles_n = (nbj + 63) / 64;
-      g.alpha = 1.0; g.beta = 0.0;
-      ds.push_back(g);
-    }
-    GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>();
-    TRY(w->up.put(dd, ds.data(), ds.size() * sizeof(GemmDesc<double>)));
     {
       Timed tm(c, KORE_TIMER_ORTHO);
-      HIP_TRY(launch_gemm_batched<double>(dd, (int)ds.size(), ds[0].tiles_m * ds[0].tiles_n, c->stream));
-      HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), (int)ds.size(), 2 * na * nbj,
+      HIP_TRY(launch_tsdot(wcol(w, a0), n, na, wcol(w, b0 + j0), n, nbj, n, nchunk, chunk, part, c->stream));
+      HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * na * nbj,
                                reinterpret_cast<double*>(res), c->stream));
     }
     HIP_TRY(hipMemcpyAsync(h.data() + (size_t)j0 * na, res, (size_t)na * nbj * sizeof(zd), hipMemcpyDeviceToHost,

@@ -63,6 +63,8 @@ class EigResult:
     seconds: float
     ir_max_iterations: int = 0
     host_seconds: dict = dataclasses.field(default_factory=dict)   # wall time per workspace primitive
+    precision: int = 0         # factor precision the converged pairs were computed with
+    fallback: bool = False     # True if a complex64 factor was replaced by an FP64 one
 
 
 class _Timed:
@@ -104,21 +106,27 @@ def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
 
 def _triu_eigvecs(S):
     """Unit-norm eigenvectors of an upper-triangular matrix (columns)."""
+    # Z is upper triangular with a unit diagonal: row k of (S - d_j I) z_j = 0
+    # gives Z[k, j] = -S[k, k+1:] Z[k+1:, j] / (d_k - d_j) for j > k, so all
+    # eigenvectors advance together, one row at a time from the bottom.
+    # Near-equal diagonal entries are separated by at least eps |d_j| (as
+    # LAPACK's ztrevc perturbs them), and a column that grows past 1e100 is
+    # rescaled on the spot so that no entry overflows.
     n = S.shape[0]
     d = np.diag(S)
-    Z = np.zeros((n, n), dtype=complex)
-    tiny = np.finfo(float).tiny
-    for i in range(n):
-        z = np.zeros(n, dtype=complex)
-        z[i] = 1.0
-        if i > 0:
-            M = S[:i, :i] - d[i] * np.eye(i)
-            dg = np.diag(M).copy()
-            dg[dg == 0] = tiny
-            M[np.diag_indices(i)] = dg
-            z[:i] = sla.solve_triangular(M, -S[:i, i], lower=False, check_finite=False)
-        Z[:, i] = z / np.linalg.norm(z)
-    return Z
+    Z = np.eye(n, dtype=complex)
+    smin = np.maximum(np.finfo(float).eps * np.abs(d), np.finfo(float).tiny / np.finfo(float).eps)
+    for k in range(n - 2, -1, -1):
+        num = S[k, k + 1:] @ Z[k + 1:, k + 1:]
+        den = d[k] - d[k + 1:]
+        small = np.abs(den) < smin[k + 1:]
+        den[small] = smin[k + 1:][small]
+        Z[k, k + 1:] = -num / den
+        big = np.abs(Z[k, k + 1:]) > 1e100
+        if big.any():
+            cols = np.nonzero(big)[0] + k + 1
+            Z[:, cols] /= np.abs(Z[k, cols])
+    return Z / np.linalg.norm(Z, axis=0)
 
 
 def eigs(*args, **kwargs) -> EigResult:
@@ -134,7 +142,26 @@ def eigs(*args, **kwargs) -> EigResult:
 
 
 def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, seed=0,
-          ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None) -> EigResult:
+          ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None, fallback_sweeps=4) -> EigResult:
+    """fallback_sweeps: with a complex64 factor, if one OP application needs
+    more than this many refinement sweeps (or does not converge), the system
+    is refactored in FP64 at the same shift and the iteration restarts: every
+    OP application costs ~one FP64 sweep instead of many complex64 ones."""
+    res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
+                     fallback_sweeps if sysm.precision != PREC_FP64 else None)
+    if res is None:
+        if log:
+            log("complex64 refinement too slow: refactoring in FP64")
+        sysm.factor(sigma, PREC_FP64)
+        res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log,
+                         workspace, None)
+        res.fallback = True
+    res.precision = sysm.precision
+    return res
+
+
+def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
+               fallback_sweeps):
     """Eigenpairs nearest ``sigma`` of the pencil held by ``sysm``.
 
     ``sysm`` must have been factored at ``sigma`` (System.factor).
@@ -187,6 +214,11 @@ def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8,
         while j < m:
             ws_info = ws.apply_op(j, cW, bs, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0)
             ir_iters = max(ir_iters, ws_info.iterations)
+            if fallback_sweeps is not None and (ws_info.iterations > fallback_sweeps
+                                                or not getattr(ws_info, "converged", 1)):
+                if own_ws:
+                    ws.close()
+                return None
             op_cols += bs
             op_calls += 1
             nb_all = j + bs

@@ -171,16 +171,20 @@ class Server:
             if req.kind == "forced":
                 b = rhs_for(req, S.n)
                 x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=20 if self.precision else 0)
+                if self.precision != backend.PREC_FP64 and not info.converged:
+                    # refinement stalled (low Ekman number): FP64 refactorisation
+                    S.factor(req.sigma, backend.PREC_FP64)
+                    x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=0)
                 res = residual(A0, B0, req, x=x, b=b)
                 out = Result(req.rid, req.kind, 0.0, 0.0, res, res <= 1e-10, x=x if self.keep_results else None,
-                             precision=self.precision, ir_sweeps=info.iterations)
+                             precision=S.precision, ir_sweeps=info.iterations)
             else:
                 r = eigs.eigs(S, req.sigma, nev=req.nev, tol=self.tol, block=self.block, ir_tol=self.ir_tol)
                 k = min(req.nev, r.nconv)
                 res = residual(A0, B0, req, lam=r.lam[:k], X=r.X[:, :k])
                 out = Result(req.rid, req.kind, 0.0, 0.0, res, (res <= 1e-10) and r.nconv >= req.nev,
-                             lam=r.lam[:k] if True else None, x=r.X[:, :k] if self.keep_results else None,
-                             precision=self.precision, ir_sweeps=r.ir_max_iterations)
+                             lam=r.lam[:k], x=r.X[:, :k] if self.keep_results else None,
+                             precision=r.precision, ir_sweeps=r.ir_max_iterations)
         finally:
             S.close()
         now = time.perf_counter()

@@ -1,0 +1,60 @@
+"""Serving loop on the device (MI355X only): every result is the same when the
+request is served alone and when it is served in the mix with others in
+flight (relative difference <= 1e-10), all residuals meet the bar, and
+admission respects the slot count and the memory budget."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from kore_amd import backend, serving
+
+pytestmark = pytest.mark.gpu
+
+
+def small_trace():
+    tr = serving.serving_trace(nreq=8, seed=7, e_range=(1e-4, 1e-3), n_cap=32, forced_frac=0.6, nev=4)
+    return [dataclasses.replace(r, arrival=0.0) for r in tr]
+
+
+@pytest.mark.parametrize("precision", [backend.PREC_FP64, backend.PREC_C64])
+def test_alone_equals_mix(precision):
+    trace = small_trace()
+    print("case: serving alone vs mix, precision", precision, [(r.kind, r.N, r.m) for r in trace], flush=True)
+    mix = serving.Server(slots=4, precision=precision, keep_results=True)
+    res_mix = {r.rid: r for r in mix.run(trace)}
+    inflight = mix.max_inflight
+    mix.close()
+    alone = serving.Server(slots=1, precision=precision, keep_results=True)
+    res_alone = {}
+    for r in trace:
+        res_alone[r.rid] = alone.run([r])[0]
+    alone.close()
+    assert inflight >= 2            # requests really overlapped
+    for r in trace:
+        a, b = res_alone[r.rid], res_mix[r.rid]
+        assert a.ok and b.ok, (r, a.error, b.error, a.residual, b.residual)
+        if r.kind == "forced":
+            d = np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x)
+        else:
+            d = max(np.min(np.abs(b.lam - l)) / abs(l) for l in a.lam)
+        assert d <= 1e-10, (r.rid, r.kind, d)
+
+
+def test_admission_respects_budget():
+    trace = small_trace()[:4]
+    need = max(serving.footprint(r.params(), backend.PREC_FP64, r.nev) for r in trace)
+    srv = serving.Server(slots=4, precision=backend.PREC_FP64, mem_budget=int(need * 1.5))
+    out = srv.run(trace)
+    assert all(r.ok for r in out)
+    assert srv.max_inflight == 1        # the budget admits one request at a time
+    assert srv.mem_peak <= int(need * 1.5)
+    srv.close()
+
+
+def test_oversized_request_is_refused_not_crashed():
+    trace = small_trace()[:2]
+    srv = serving.Server(slots=2, precision=backend.PREC_FP64, mem_budget=1 << 20)
+    out = srv.run(trace)
+    assert all((not r.ok) and "budget" in r.error for r in out)
+    srv.close()

@@ -147,3 +147,23 @@ def test_restart_path_is_exercised():
     assert r1.restarts >= 1
     for i in range(4):
         assert abs(r1.lam[i] - r2.lam[i]) / abs(r2.lam[i]) <= 1e-10
+
+
+def test_triangular_eigenvectors():
+    """S z_j = d_j z_j for every column, unit norm, and the same directions as
+    LAPACK's eigenvectors of the triangular matrix (equal up to a phase)."""
+    rng = np.random.default_rng(5)
+    n = 40
+    S = np.triu(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    Z = eigs._triu_eigvecs(S)
+    d = np.diag(S)
+    assert np.abs(S @ Z - Z * d).max() <= 1e-10 * np.abs(S).max() * np.abs(Z).max() * n
+    assert np.allclose(np.linalg.norm(Z, axis=0), 1.0)
+    w, V = np.linalg.eig(S)
+    for j in range(n):
+        i = int(np.argmin(np.abs(w - d[j])))
+        assert abs(abs(np.vdot(V[:, i], Z[:, j])) - 1.0) <= 1e-8
+    # a repeated diagonal entry must not produce inf / nan
+    S2 = S.copy()
+    S2[3, 3] = S2[7, 7]
+    assert np.all(np.isfinite(eigs._triu_eigvecs(S2)))
