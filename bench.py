@@ -63,7 +63,10 @@ def pct(x, q):
     return float(x[f] + (x[c] - x[f]) * (k - f))
 
 
-def run_path(S, ctx, wl, precision, steps, warmup, block, ncv):
+def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
+    """Warm-up, then ``steps`` timed eigensolves (no instrumentation), then
+    ``timing_steps`` more with HIP-event timers around every kernel (the
+    per-class device times of the roofline; kept out of the timed region)."""
     from kore_amd import backend, eigs
     sigma, nev, tol = wl["sigma"], wl["nev"], wl["tol"]
     res = None
@@ -71,8 +74,6 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv):
         S.factor(sigma, precision)
         res = eigs.eigs(S, sigma, nev=nev, ncv=ncv, tol=tol, block=block)
     ctx.sync()
-    ctx.timing_reset()
-    ctx.set_timing(True)
     lat = []
     finfo = None
     host = {}
@@ -92,14 +93,21 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv):
             k, "c64+IR" if precision else "fp64", 1e3 * lat[-1], res.nconv, res.op_calls, res.restarts))
     ctx.sync()
     total = time.perf_counter() - t_all
+    ctx.timing_reset()
+    ctx.set_timing(True)
+    for _ in range(timing_steps):
+        S.factor(sigma, precision)
+        eigs.eigs(S, sigma, nev=nev, ncv=ncv, tol=tol, block=block)
+    ctx.sync()
     ctx.set_timing(False)
     timing = ctx.timing()
-    return dict(lat=lat, total=total, res=res, finfo=finfo, timing=timing,
+    return dict(lat=lat, total=total, res=res, finfo=finfo, timing=timing, timing_steps=timing_steps,
                 host_ms_per_step={k: 1e3 * v / steps for k, v in host.items()})
 
 
-def roofline(run, precision, steps):
+def roofline(run, precision):
     t = run["timing"]
+    steps = run["timing_steps"]
     f = run["finfo"]
     cand = {}
     # solve sweeps: factor bytes streamed per sweep
@@ -125,7 +133,7 @@ def roofline(run, precision, steps):
     r = dict(cand[dom][1])
     r["frac"] = r["achieved"] / r["peak"]
     r["traffic"] = None
-    classes = {k: dict(launches=v[0], ms=v[1]) for k, v in t.items()}
+    classes = {k: dict(launches_per_step=v[0] / steps, device_ms_per_step=v[1] / steps) for k, v in t.items()}
     return r, classes
 
 
@@ -253,7 +261,7 @@ def main():
     ap.add_argument("--e-range", default="1e-5,1e-4")
     ap.add_argument("--n-cap", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--block", type=int, default=8)
+    ap.add_argument("--block", type=int, default=4)
     ap.add_argument("--ncv", type=int, default=None)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -281,7 +289,7 @@ def main():
         r = run_path(S, ctx, wl, prec, args.steps, args.warmup, args.block, args.ncv)
         res = r["res"]
         resid = eigs.residuals(A, B, res.lam[:wl["nev"]], res.X[:, :wl["nev"]])
-        rl, classes = roofline(r, prec, args.steps)
+        rl, classes = roofline(r, prec)
         paths[name] = dict(
             solves_per_s=args.steps / r["total"], ms_per_step=1e3 * r["total"] / args.steps,
             p50_ms=1e3 * pct(r["lat"], 50), p99_ms=1e3 * pct(r["lat"], 99),

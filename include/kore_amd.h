@@ -23,6 +23,7 @@ extern "C" {
 
 #define KORE_OK 0
 #define KORE_WARN_IR_NOT_CONVERGED 1
+#define KORE_WARN_BREAKDOWN 2        /* Krylov block is numerically rank deficient */
 #define KORE_ERR_ARG (-1)
 #define KORE_ERR_HIP (-2)
 #define KORE_ERR_OOM (-3)
@@ -153,6 +154,23 @@ int32_t kore_ws_gemm(kore_ws* ws, int32_t d0, int32_t nd, int32_t s0, int32_t ns
                      double alpha, double beta);
 /* V[:, dst+j] = V[:, src+j]. */
 int32_t kore_ws_copy(kore_ws* ws, int32_t src, int32_t dst, int32_t nc);
+/* Block Arnoldi expansion of the Krylov-Schur basis from column j0 to j1 in
+ * blocks of bs: for j = j0, j0+bs, ...: W = OP V[:, j:j+bs] (in the scratch
+ * block V[:, cw:cw+bs], cw >= j1+bs), two passes of classical Gram-Schmidt of
+ * W against V[:, 0:j+bs], two passes of Cholesky QR giving V[:, j+bs:j+2bs] R
+ * = W.  H (host, column-major, ldh >= j1+bs) receives the projected columns:
+ * H[0:j+bs, j:j+bs] = Gram-Schmidt coefficients, H[j+bs:j+2bs, j:j+bs] = R.
+ * ir_tol / ir_maxit as in kore_ws_apply_op; ir_stop > 0 ends the expansion
+ * when one block needs more than ir_stop refinement sweeps or does not
+ * converge (KORE_WARN_IR_NOT_CONVERGED).  KORE_WARN_BREAKDOWN: the Cholesky
+ * factor of block j_done broke down (its Gram-Schmidt coefficients are in H,
+ * the orthogonalised block in V[:, cw:cw+bs]).  *j_done = first column not
+ * completed (j1 on success).
+ * Replaces: the Arnoldi expansion inside SLEPc's Krylov-Schur (STApply +
+ * BVOrthogonalize per step) under E.solve() (bin/solve.py:123). */
+int32_t kore_ws_expand(kore_ws* ws, int32_t j0, int32_t j1, int32_t bs, int32_t cw, double ir_tol,
+                       int32_t ir_maxit, int32_t ir_stop, double* H, int64_t ldh, int32_t* j_done,
+                       kore_solve_info* info);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ LIBPATH = os.environ.get("KORE_AMD_LIB", os.path.join(HERE, "libkore_amd.so"))
 
 KORE_OK = 0
 KORE_WARN_IR_NOT_CONVERGED = 1
+KORE_WARN_BREAKDOWN = 2
 PREC_FP64 = 0
 PREC_C64 = 1
 TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER = range(6)
@@ -87,6 +88,8 @@ SIGNATURES = [
     ("kore_ws_dot", _I32, [_P, _I32, _I32, _I32, _I32, _PD]),
     ("kore_ws_gemm", _I32, [_P, _I32, _I32, _I32, _I32, _PD, _D, _D]),
     ("kore_ws_copy", _I32, [_P, _I32, _I32, _I32]),
+    ("kore_ws_expand", _I32, [_P, _I32, _I32, _I32, _I32, _D, _I32, _I32, _PD, _I64, C.POINTER(C.c_int32),
+                              C.POINTER(SolveInfo)]),
 ]
 
 _lib = None
@@ -335,6 +338,19 @@ class Workspace:
 
     def copy(self, src, dst, nc):
         _check(lib().kore_ws_copy(self._p, int(src), int(dst), int(nc)))
+
+    def expand(self, j0, j1, bs, cw, H, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
+        """Block Arnoldi expansion on the device (kore_ws_expand); H is a
+        Fortran-ordered complex128 array whose columns j0..j1 are filled.
+        Returns (status, j_done, SolveInfo)."""
+        if not (isinstance(H, np.ndarray) and H.dtype == np.complex128 and H.flags.f_contiguous):
+            raise ValueError("H must be a Fortran-ordered complex128 array")
+        info = SolveInfo()
+        jd = C.c_int32(0)
+        rc = lib().kore_ws_expand(self._p, int(j0), int(j1), int(bs), int(cw), float(ir_tol), int(ir_maxit),
+                                  int(ir_stop), _dptr(H), H.shape[0], C.byref(jd), C.byref(info))
+        _check(rc)
+        return rc, int(jd.value), info
 
     def close(self):
         if getattr(self, "_p", None):

@@ -511,6 +511,87 @@ hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, 
   return hipGetLastError();
 }
 
+// Cholesky factor of a small Hermitian positive definite matrix and the
+// inverse of the factor, one wavefront (n <= 16).  G (n x n, column-major, ld
+// ldg) enters symmetrised as (G + G^H)/2.  On return R (upper triangular,
+// G = R^H R) and Rinv = R^{-1}, both n x n column-major with ld n and zeros
+// below the diagonal; *flag = 1 when a pivot is not positive (the block is
+// numerically rank deficient), and then R = Rinv = I.  Used by the Cholesky
+// QR of the Krylov expansion.
+__global__ __launch_bounds__(64) void chol_inv_kernel(const cx<double>* __restrict__ G, int ldg, int n,
+                                                      cx<double>* __restrict__ R, cx<double>* __restrict__ Rinv,
+                                                      int* __restrict__ flag) {
+  __shared__ cx<double> A[16][17];
+  __shared__ cx<double> X[16][17];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < n * n; idx += 64) {
+    const int i = idx % n, j = idx / n;
+    const cx<double> gij = G[(int64_t)j * ldg + i], gji = G[(int64_t)i * ldg + j];
+    A[i][j] = {0.5 * (gij.re + gji.re), 0.5 * (gij.im - gji.im)};
+  }
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      const double p = A[k][k].re;
+      if (!(p > 0.0)) bad = 1;
+      else A[k][k] = {sqrt(p), 0.0};
+    }
+    __syncthreads();
+    if (bad) break;
+    const double rkk = A[k][k].re;
+    for (int j = k + 1 + tid; j < n; j += 64) {
+      A[k][j].re /= rkk;
+      A[k][j].im /= rkk;
+    }
+    __syncthreads();
+    const int mk = n - k - 1;
+    for (int idx = tid; idx < mk * mk; idx += 64) {
+      const int i = k + 1 + idx % mk, j = k + 1 + idx / mk;
+      if (i <= j) {
+        // A[i][j] -= conj(R[k][i]) R[k][j]
+        const cx<double> a = A[k][i], b = A[k][j];
+        A[i][j].re -= a.re * b.re + a.im * b.im;
+        A[i][j].im -= a.re * b.im - a.im * b.re;
+      }
+    }
+    __syncthreads();
+  }
+  const bool fail = bad != 0;
+  // column j of R^{-1} by back substitution, one thread per column
+  if (!fail && tid < n) {
+    const int j = tid;
+    X[j][j] = {1.0 / A[j][j].re, 0.0};
+    for (int i = j - 1; i >= 0; --i) {
+      cx<double> s = czero<double>();
+      for (int p = i + 1; p <= j; ++p) s = cfma(s, A[i][p], X[p][j]);
+      X[i][j] = {-s.re / A[i][i].re, -s.im / A[i][i].re};
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < n * n; idx += 64) {
+    const int i = idx % n, j = idx / n;
+    cx<double> r = czero<double>(), ri = czero<double>();
+    if (fail) {
+      if (i == j) r = ri = {1.0, 0.0};
+    } else if (i <= j) {
+      r = A[i][j];
+      ri = X[i][j];
+    }
+    R[(int64_t)j * n + i] = r;
+    Rinv[(int64_t)j * n + i] = ri;
+  }
+  if (tid == 0) *flag = fail ? 1 : 0;
+}
+
+hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, cx<double>* Rinv, int* flag,
+                           hipStream_t s) {
+  if (n < 1 || n > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, G, ldg, n, R, Rinv, flag);
+  return hipGetLastError();
+}
+
 template <class T>
 __global__ void zero_kernel(cx<T>* __restrict__ p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)

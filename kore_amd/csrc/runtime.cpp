@@ -344,12 +344,20 @@ struct kore_ws {
   kore_sys* sys = nullptr;
   int ncol = 0;
   DevBuf V, part, qbuf, descb;
+  DevBuf hb;                 // device-side Gram-Schmidt / Cholesky-QR coefficients of the expansion
+  cx<double>* hh = nullptr;  // pinned host copies of them, two slots (blocks in flight)
+  hipEvent_t hev[2] = {nullptr, nullptr};
   Uploader up;
+  ~kore_ws() {
+    if (hh) hipHostFree(hh);
+    for (hipEvent_t e : hev)
+      if (e) hipEventDestroy(e);
+  }
 };
 
 static constexpr int MAXR = 16;
 static constexpr int DOT_MAXCHUNK = 512;   // split-K chunks of the workspace inner products
-static constexpr int DOT_ROWS = 512;       // minimum rows per chunk
+static constexpr int DOT_ROWS = 64;        // minimum rows per chunk
 
 template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   kore_ctx* c = s->ctx;
@@ -1163,6 +1171,75 @@ int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, doub
 }
 
 // ---- workspace ---------------------------------------------------------------
+}  // extern "C"
+
+static inline cx<double>* wcol(kore_ws* w, int j) { return w->V.as<cx<double>>() + (size_t)j * w->sys->n; }
+
+// Coefficient block of the expansion (complex entries): h1, h2 (ncol x MAXR
+// each), G, R1, Rinv1, R2, Rinv2 (MAXR x MAXR each), then two int flags.
+static size_t ws_hb_elems(int ncol) { return 2 * (size_t)ncol * MAXR + 5 * MAXR * MAXR + 1; }
+
+// out (na x nb, column-major, ld na, device) = V[:, a0:a0+na]^H V[:, b0:b0+nb], nb <= MAXR.
+static int32_t ws_dot_dev(kore_ws* w, int a0, int na, int b0, int nb, cx<double>* out) {
+  kore_sys* s = w->sys;
+  kore_ctx* c = s->ctx;
+  const int64_t n = s->n;
+  // split-K over row chunks so that every CU streams a slice of the basis
+  int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
+  if (nchunk < 1) nchunk = 1;
+  const int64_t chunk = (n + nchunk - 1) / nchunk;
+  cx<double>* part = w->part.as<cx<double>>();
+  Timed tm(c, KORE_TIMER_ORTHO);
+  HIP_TRY(launch_tsdot(wcol(w, a0), n, na, wcol(w, b0), n, nb, n, nchunk, chunk, part, c->stream));
+  HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * na * nb, reinterpret_cast<double*>(out),
+                           c->stream));
+  return KORE_OK;
+}
+
+// V[:, d0:d0+nd] = beta V[:, d0:d0+nd] + alpha V[:, s0:s0+ns] Q, Q on the device (ldq).
+static int32_t ws_gemm_dev(kore_ws* w, int d0, int nd, int s0, int ns, const cx<double>* Q, int ldq, double alpha,
+                           double beta) {
+  kore_sys* s = w->sys;
+  kore_ctx* c = s->ctx;
+  const int64_t n = s->n;
+  GemmDesc<double> g{};
+  g.A[0] = wcol(w, s0); g.lda[0] = n;
+  g.B[0] = Q; g.ldb[0] = ldq;
+  g.k[0] = ns;
+  g.C = wcol(w, d0); g.ldc = n;
+  g.m = (int)n; g.n = nd; g.opA = 0;
+  g.tiles_m = (int)((n + 63) / 64); g.tiles_n = (nd + 63) / 64;
+  g.alpha = alpha; g.beta = beta;
+  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>() + DOT_MAXCHUNK + 1;
+  TRY(w->up.put(dd, &g, sizeof(g)));
+  Timed tm(c, KORE_TIMER_ORTHO);
+  HIP_TRY(launch_gemm_batched<double>(dd, 1, std::min(g.tiles_m * g.tiles_n, 65535), c->stream));
+  return KORE_OK;
+}
+
+static int32_t ws_apply_op_dev(kore_ws* w, int src, int dst, int nc, double ir_tol, int ir_maxit,
+                               kore_solve_info* info) {
+  kore_sys* s = w->sys;
+  const int64_t n = s->n;
+  info->iterations = 0;
+  info->converged = 1;
+  info->rel_correction = -1;
+  for (int a = 0; a < nc; a += MAXR) {
+    const int nr = std::min(MAXR, nc - a);
+    {
+      Timed tm(s->ctx, KORE_TIMER_SPMV);
+      HIP_TRY(launch_spmv(s->csr, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
+    }
+    kore_solve_info li{};
+    TRY(solve_dev(s, s->bb, n, wcol(w, dst + a), n, nr, ir_tol, ir_maxit, &li));
+    info->iterations = std::max(info->iterations, li.iterations);
+    info->converged = info->converged && li.converged;
+    info->rel_correction = std::max(info->rel_correction, li.rel_correction);
+  }
+  return KORE_OK;
+}
+
+extern "C" {
 int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
   if (!s || !out || ncol <= 0) return fail(KORE_ERR_ARG, "bad workspace arguments");
   *out = nullptr;
@@ -1174,6 +1251,9 @@ int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
   TRY(w->part.alloc(c, ((size_t)DOT_MAXCHUNK + 1) * ncol * MAXR * sizeof(zd)));
   TRY(w->qbuf.alloc(c, (size_t)ncol * ncol * sizeof(zd)));
   TRY(w->descb.alloc(c, (DOT_MAXCHUNK + 64) * sizeof(GemmDesc<double>)));
+  TRY(w->hb.alloc(c, ws_hb_elems(ncol) * sizeof(zd)));
+  HIP_TRY(hipHostMalloc((void**)&w->hh, 2 * ws_hb_elems(ncol) * sizeof(zd), hipHostMallocDefault));
+  for (hipEvent_t& e : w->hev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   TRY(w->up.init(c, 1 << 22));
   HIP_TRY(launch_zero<double>(w->V.as<cx<double>>(), (int64_t)s->n * ncol, c->stream));
   *out = w.release();
@@ -1185,8 +1265,6 @@ void kore_ws_destroy(kore_ws* w) {
   hipStreamSynchronize(w->sys->ctx->stream);
   delete w;
 }
-
-static inline cx<double>* wcol(kore_ws* w, int j) { return w->V.as<cx<double>>() + (size_t)j * w->sys->n; }
 
 static int32_t ws_check(kore_ws* w, int a, int na) {
   if (!w) return fail(KORE_ERR_ARG, "workspace is NULL");
@@ -1234,27 +1312,10 @@ int32_t kore_ws_apply_op(kore_ws* w, int32_t src, int32_t dst, int32_t nc, doubl
   kore_sys* s = w->sys;
   if (!s->has_b) return fail(KORE_ERR_STATE, "system has no B matrix");
   if (src < dst + nc && dst < src + nc) return fail(KORE_ERR_ARG, "source and destination overlap");
-  const int64_t n = s->n;
-  if (info) {
-    info->iterations = 0;
-    info->converged = 1;
-    info->rel_correction = -1;
-  }
-  for (int a = 0; a < nc; a += MAXR) {
-    const int nr = std::min(MAXR, nc - a);
-    {
-      Timed tm(s->ctx, KORE_TIMER_SPMV);
-      HIP_TRY(launch_spmv(s->csr, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
-    }
-    kore_solve_info li{};
-    TRY(solve_dev(s, s->bb, n, wcol(w, dst + a), n, nr, ir_tol, ir_maxit, &li));
-    if (info) {
-      info->iterations = std::max(info->iterations, li.iterations);
-      info->converged = info->converged && li.converged;
-      info->rel_correction = std::max(info->rel_correction, li.rel_correction);
-    }
-  }
-  return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
+  kore_solve_info li{};
+  TRY(ws_apply_op_dev(w, src, dst, nc, ir_tol, ir_maxit, &li));
+  if (info) *info = li;
+  return li.converged ? KORE_OK : KORE_WARN_IR_NOT_CONVERGED;
 }
 
 int32_t kore_ws_spmv(kore_ws* w, int32_t src, int32_t dst, int32_t nc, double ca_re, double ca_im, double cb_re,
@@ -1273,24 +1334,12 @@ int32_t kore_ws_dot(kore_ws* w, int32_t a0, int32_t na, int32_t b0, int32_t nb, 
   TRY(ws_check(w, a0, na));
   TRY(ws_check(w, b0, nb));
   if (!H) return fail(KORE_ERR_ARG, "H is NULL");
-  kore_sys* s = w->sys;
-  kore_ctx* c = s->ctx;
-  const int64_t n = s->n;
-  // split-K over row chunks so that every CU streams a slice of the basis
-  int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
-  if (nchunk < 1) nchunk = 1;
-  const int64_t chunk = (n + nchunk - 1) / nchunk;
-  cx<double>* part = w->part.as<cx<double>>();
-  cx<double>* res = part + (size_t)DOT_MAXCHUNK * w->ncol * MAXR;
+  kore_ctx* c = w->sys->ctx;
+  cx<double>* res = w->part.as<cx<double>>() + (size_t)DOT_MAXCHUNK * w->ncol * MAXR;
   std::vector<zd> h((size_t)na * nb);
   for (int j0 = 0; j0 < nb; j0 += MAXR) {
     const int nbj = std::min(MAXR, nb - j0);
-    {
-      Timed tm(c, KORE_TIMER_ORTHO);
-      HIP_TRY(launch_tsdot(wcol(w, a0), n, na, wcol(w, b0 + j0), n, nbj, n, nchunk, chunk, part, c->stream));
-      HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * na * nbj,
-                               reinterpret_cast<double*>(res), c->stream));
-    }
+    TRY(ws_dot_dev(w, a0, na, b0 + j0, nbj, res));
     HIP_TRY(hipMemcpyAsync(h.data() + (size_t)j0 * na, res, (size_t)na * nbj * sizeof(zd), hipMemcpyDeviceToHost,
                            c->stream));
   }
@@ -1306,24 +1355,9 @@ int32_t kore_ws_gemm(kore_ws* w, int32_t d0, int32_t nd, int32_t s0, int32_t ns,
   if (!Q) return fail(KORE_ERR_ARG, "Q is NULL");
   if (s0 < d0 + nd && d0 < s0 + ns) return fail(KORE_ERR_ARG, "source and destination overlap");
   if (nd == 0) return KORE_OK;
-  kore_sys* s = w->sys;
-  kore_ctx* c = s->ctx;
-  const int64_t n = s->n;
   cx<double>* q = w->qbuf.as<cx<double>>();
   TRY(w->up.put(q, Q, (size_t)ns * nd * sizeof(zd)));
-  GemmDesc<double> g{};
-  g.A[0] = wcol(w, s0); g.lda[0] = n;
-  g.B[0] = q; g.ldb[0] = ns;
-  g.k[0] = ns;
-  g.C = wcol(w, d0); g.ldc = n;
-  g.m = (int)n; g.n = nd; g.opA = 0;
-  g.tiles_m = (int)((n + 63) / 64); g.tiles_n = (nd + 63) / 64;
-  g.alpha = alpha; g.beta = beta;
-  GemmDesc<double>* dd = w->descb.as<GemmDesc<double>>() + DOT_MAXCHUNK + 1;
-  TRY(w->up.put(dd, &g, sizeof(g)));
-  Timed tm(c, KORE_TIMER_ORTHO);
-  HIP_TRY(launch_gemm_batched<double>(dd, 1, std::min(g.tiles_m * g.tiles_n, 65535), c->stream));
-  return KORE_OK;
+  return ws_gemm_dev(w, d0, nd, s0, ns, q, ns, alpha, beta);
 }
 
 int32_t kore_ws_copy(kore_ws* w, int32_t src, int32_t dst, int32_t nc) {
@@ -1331,6 +1365,127 @@ int32_t kore_ws_copy(kore_ws* w, int32_t src, int32_t dst, int32_t nc) {
   TRY(ws_check(w, dst, nc));
   kore_sys* s = w->sys;
   HIP_TRY(launch_copy_cols(wcol(w, src), s->n, wcol(w, dst), s->n, s->n, nc, s->ctx->stream));
+  return KORE_OK;
+}
+
+// Block Arnoldi expansion with everything but the bookkeeping on the device:
+// per block, OP, two classical Gram-Schmidt passes and two Cholesky-QR
+// passes.  The coefficients of block j travel to a pinned slot behind an
+// event; block j+1 is queued before the host waits for block j, so the device
+// never drains between blocks.  A breakdown found at block j discards block
+// j+1 (everything it wrote lies in columns the recovery overwrites).
+static int32_t ws_expand_finish(kore_ws* w, int j, int bs, int slot, double* H, int64_t ldh, bool* breakdown) {
+  const int ncol = w->ncol;
+  HIP_TRY(hipEventSynchronize(w->hev[slot]));
+  const zd* h1 = reinterpret_cast<const zd*>(w->hh) + (size_t)slot * ws_hb_elems(ncol);
+  const zd* h2 = h1 + (size_t)ncol * MAXR;
+  const zd* R1 = h2 + (size_t)ncol * MAXR + MAXR * MAXR;
+  const zd* R2 = R1 + 2 * MAXR * MAXR;
+  const int* flag = reinterpret_cast<const int*>(h1 + (size_t)2 * ncol * MAXR + 5 * MAXR * MAXR);
+  const int nb = j + bs;
+  zd* Hz = reinterpret_cast<zd*>(H);
+  for (int q = 0; q < bs; ++q)
+    for (int i = 0; i < nb; ++i) Hz[(size_t)(j + q) * ldh + i] = h1[(size_t)q * nb + i] + h2[(size_t)q * nb + i];
+  *breakdown = flag[0] || flag[1];
+  if (*breakdown) return KORE_OK;
+  // R = R2 R1 (both upper triangular, ld bs)
+  for (int q = 0; q < bs; ++q)
+    for (int i = 0; i < bs; ++i) {
+      zd acc = 0;
+      for (int p = i; p <= q; ++p) acc += R2[(size_t)p * bs + i] * R1[(size_t)q * bs + p];
+      Hz[(size_t)(j + q) * ldh + nb + i] = acc;
+    }
+  return KORE_OK;
+}
+
+int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t cw, double ir_tol, int32_t ir_maxit,
+                       int32_t ir_stop, double* H, int64_t ldh, int32_t* j_done, kore_solve_info* info) {
+  if (!w || !H || !j_done) return fail(KORE_ERR_ARG, "NULL argument");
+  *j_done = j0;
+  kore_sys* s = w->sys;
+  if (!s->has_b) return fail(KORE_ERR_STATE, "system has no B matrix");
+  if (bs < 1 || bs > MAXR) return fail(KORE_ERR_ARG, "block size must be 1..%d", MAXR);
+  if (j0 < 0 || j1 < j0 || (j1 - j0) % bs != 0)
+    return fail(KORE_ERR_ARG, "expansion range [%d, %d) is not a whole number of blocks of %d", j0, j1, bs);
+  TRY(ws_check(w, 0, j1 + bs));
+  TRY(ws_check(w, cw, bs));
+  if (cw < j1 + bs) return fail(KORE_ERR_ARG, "scratch block at %d overlaps the basis [0, %d)", cw, j1 + bs);
+  if (ldh < j1 + bs) return fail(KORE_ERR_ARG, "ldh < %d", j1 + bs);
+  kore_ctx* c = s->ctx;
+  const int ncol = w->ncol;
+  cx<double>* d_h1 = w->hb.as<cx<double>>();
+  cx<double>* d_h2 = d_h1 + (size_t)ncol * MAXR;
+  cx<double>* d_G = d_h2 + (size_t)ncol * MAXR;
+  cx<double>* d_R1 = d_G + MAXR * MAXR;
+  cx<double>* d_Ri1 = d_R1 + MAXR * MAXR;
+  cx<double>* d_R2 = d_Ri1 + MAXR * MAXR;
+  cx<double>* d_Ri2 = d_R2 + MAXR * MAXR;
+  int* d_flag = reinterpret_cast<int*>(d_Ri2 + MAXR * MAXR);
+  if (info) {
+    info->iterations = 0;
+    info->converged = 1;
+    info->rel_correction = -1;
+  }
+  int pending = -1, pslot = 0;   // block queued on the device whose coefficients are not yet read
+  bool bd = false;
+  for (int j = j0, slot = 0; j < j1; j += bs, slot ^= 1) {
+    kore_solve_info li{};
+    TRY(ws_apply_op_dev(w, j, cw, bs, ir_tol, ir_maxit, &li));
+    if (info) {
+      info->iterations = std::max(info->iterations, li.iterations);
+      info->converged = info->converged && li.converged;
+      info->rel_correction = std::max(info->rel_correction, li.rel_correction);
+    }
+    if (ir_stop > 0 && (li.iterations > ir_stop || !li.converged)) {
+      if (pending >= 0) {
+        TRY(ws_expand_finish(w, pending, bs, pslot, H, ldh, &bd));
+        if (bd) {
+          HIP_TRY(hipStreamSynchronize(c->stream));
+          *j_done = pending;
+          return KORE_WARN_BREAKDOWN;
+        }
+      }
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      *j_done = j;
+      return KORE_WARN_IR_NOT_CONVERGED;
+    }
+    const int nb = j + bs;
+    // two passes of classical Gram-Schmidt against V[:, 0:nb]
+    TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h1));
+    TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h1, nb, -1.0, 1.0));
+    TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h2));
+    TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h2, nb, -1.0, 1.0));
+    // two passes of Cholesky QR: V[:, nb:nb+bs] (R2 R1) = W
+    TRY(ws_dot_dev(w, cw, bs, cw, bs, d_G));
+    HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R1, d_Ri1, d_flag, c->stream));
+    TRY(ws_gemm_dev(w, nb, bs, cw, bs, d_Ri1, bs, 1.0, 0.0));
+    TRY(ws_dot_dev(w, nb, bs, nb, bs, d_G));
+    HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R2, d_Ri2, d_flag + 1, c->stream));
+    TRY(ws_gemm_dev(w, cw, bs, nb, bs, d_Ri2, bs, 1.0, 0.0));
+    HIP_TRY(launch_copy_cols(wcol(w, cw), s->n, wcol(w, nb), s->n, s->n, bs, c->stream));
+    HIP_TRY(hipMemcpyAsync(w->hh + (size_t)slot * ws_hb_elems(ncol), d_h1, ws_hb_elems(ncol) * sizeof(zd),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(w->hev[slot], c->stream));
+    if (pending >= 0) {
+      TRY(ws_expand_finish(w, pending, bs, pslot, H, ldh, &bd));
+      if (bd) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        *j_done = pending;
+        return KORE_WARN_BREAKDOWN;
+      }
+      *j_done = pending + bs;
+    }
+    pending = j;
+    pslot = slot;
+  }
+  if (pending >= 0) {
+    TRY(ws_expand_finish(w, pending, bs, pslot, H, ldh, &bd));
+    if (bd) {
+      *j_done = pending;
+      return KORE_WARN_BREAKDOWN;
+    }
+    *j_done = pending + bs;
+  }
   return KORE_OK;
 }
 

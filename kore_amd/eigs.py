@@ -12,8 +12,11 @@ vectors at a time (block Krylov-Schur, Zhou & Saad 2008; Stewart 2001 for
 bs = 1).  Everything of length sizmat stays on the device: the OP
 applications, the block classical Gram-Schmidt with one reorthogonalisation
 (two passes of V^H W then W -= V h), the Cholesky-QR of each new block and
-the restart combination V <- V Q.  The host holds only the small projected
-matrix and computes its ordered Schur form in FP64 (scipy/LAPACK).
+the restart combination V <- V Q.  A whole expansion (OP, Gram-Schmidt and
+Cholesky-QR of every new block) is one library call (kore_ws_expand) whose
+coefficients stay on the device until the block is done.  The host holds
+only the small projected matrix and computes its ordered Schur form in FP64
+(scipy/LAPACK).
 
 Krylov decomposition kept throughout:  OP V[:, :j] = V[:, :j+bs] H[:j+bs, :j].
 Convergence: a Ritz pair (theta, y) is converged when ||Bres y|| <= tol |theta|
@@ -28,7 +31,7 @@ import time
 import numpy as np
 import scipy.linalg as sla
 
-from .backend import PREC_FP64, Workspace
+from .backend import KORE_WARN_BREAKDOWN, KORE_WARN_IR_NOT_CONVERGED, PREC_FP64, Workspace
 
 
 def sort_key(which: str, lam, tau):
@@ -65,6 +68,7 @@ class EigResult:
     host_seconds: dict = dataclasses.field(default_factory=dict)   # wall time per workspace primitive
     precision: int = 0         # factor precision the converged pairs were computed with
     fallback: bool = False     # True if a complex64 factor was replaced by an FP64 one
+    breakdowns: int = 0        # blocks that hit an invariant subspace (replaced by random blocks)
 
 
 class _Timed:
@@ -104,6 +108,38 @@ def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
     return R2 @ R1
 
 
+def expand_host(ws, j0, j1, bs, cw, H, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
+    """Block Arnoldi expansion driven from the host through the workspace
+    primitives; the same contract and arithmetic as kore_ws_expand (OP, two
+    classical Gram-Schmidt passes, two Cholesky-QR passes per block).
+    Returns (status, j_done, info)."""
+    iters, conv = 0, True
+    for j in range(j0, j1, bs):
+        inf = ws.apply_op(j, cw, bs, ir_tol=ir_tol, ir_maxit=ir_maxit)
+        iters = max(iters, inf.iterations)
+        ok = bool(getattr(inf, "converged", 1))
+        conv = conv and ok
+        if ir_stop > 0 and (inf.iterations > ir_stop or not ok):
+            return KORE_WARN_IR_NOT_CONVERGED, j, _Info(iters, conv)
+        nb = j + bs
+        h = ws.dot(0, nb, cw, bs)
+        ws.gemm(cw, bs, 0, nb, h, alpha=-1.0, beta=1.0)
+        h2 = ws.dot(0, nb, cw, bs)
+        ws.gemm(cw, bs, 0, nb, h2, alpha=-1.0, beta=1.0)
+        H[:nb, j:j + bs] = h + h2
+        try:
+            H[nb:nb + bs, j:j + bs] = _cholqr(ws, cw, nb, bs)
+        except np.linalg.LinAlgError:
+            return KORE_WARN_BREAKDOWN, j, _Info(iters, conv)
+    return 0, j1, _Info(iters, conv)
+
+
+@dataclasses.dataclass
+class _Info:
+    iterations: int
+    converged: bool
+
+
 def _triu_eigvecs(S):
     """Unit-norm eigenvectors of an upper-triangular matrix (columns)."""
     # Z is upper triangular with a unit diagonal: row k of (S - d_j I) z_j = 0
@@ -141,7 +177,7 @@ def eigs(*args, **kwargs) -> EigResult:
         return _eigs(*args, **kwargs)
 
 
-def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=8, seed=0,
+def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=4, seed=0,
           ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None, fallback_sweeps=4) -> EigResult:
     """fallback_sweeps: with a complex64 factor, if one OP application needs
     more than this many refinement sweeps (or does not converge), the system
@@ -174,8 +210,10 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     n = sysm.n
     bs = max(1, min(int(block), 16))
     if ncv is None:
-        # a block space needs more columns to reach the same polynomial degree
-        ncv = max(2 * nev, nev + 15, 10 * bs)
+        # a block space needs more columns to reach the same polynomial degree;
+        # 16 blocks of 4 gave the fewest device sweeps plus host Schur time
+        # on the 60 000-unknown m=1, E=1e-6 inertial-mode problem (nev=10)
+        ncv = max(2 * nev, nev + 15, 16 * bs)
     ncv = max(ncv, nev + 2 * bs)
     ncv = int(np.ceil(ncv / bs) * bs)
     ncv = min(ncv, max(bs, (n // bs) * bs - bs))
@@ -201,46 +239,47 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
         X0 = X0[:, :bs]
     ws.set(cW, X0)
     _cholqr(ws, cW, 0, bs)
-    H = np.zeros((m + bs, m), dtype=complex)
+    H = np.zeros((m + bs, m), dtype=complex, order="F")
     j = 0
     restarts = 0
     op_cols = 0
     op_calls = 0
     ir_iters = 0
     nconv = 0
+    breakdowns = 0
     mixed = sysm.precision != PREC_FP64
+    expand = getattr(ws, "expand", None) if hasattr(ws_raw, "expand") else None
+    if expand is None:
+        def expand(j0, j1, bs_, cw, Hm, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
+            return expand_host(ws, j0, j1, bs_, cw, Hm, ir_tol, ir_maxit, ir_stop)
     while True:
         # ---- expand the decomposition to m columns
         while j < m:
-            ws_info = ws.apply_op(j, cW, bs, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0)
+            st, jd, ws_info = expand(j, m, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
+                                     ir_stop=fallback_sweeps or 0)
             ir_iters = max(ir_iters, ws_info.iterations)
-            if fallback_sweeps is not None and (ws_info.iterations > fallback_sweeps
-                                                or not getattr(ws_info, "converged", 1)):
+            done_blocks = (jd - j) // bs
+            if st == KORE_WARN_IR_NOT_CONVERGED and fallback_sweeps is not None:
                 if own_ws:
                     ws.close()
                 return None
-            op_cols += bs
-            op_calls += 1
-            nb_all = j + bs
-            h = ws.dot(0, nb_all, cW, bs)
-            ws.gemm(cW, bs, 0, nb_all, h, alpha=-1.0, beta=1.0)
-            h2 = ws.dot(0, nb_all, cW, bs)
-            ws.gemm(cW, bs, 0, nb_all, h2, alpha=-1.0, beta=1.0)
-            h = h + h2
-            try:
-                R = _cholqr(ws, cW, j + bs, bs)
-            except np.linalg.LinAlgError:
+            if st == KORE_WARN_BREAKDOWN:
+                done_blocks += 1
+            op_calls += done_blocks
+            op_cols += bs * done_blocks
+            if st == KORE_WARN_BREAKDOWN:
                 # invariant subspace: continue with a fresh random block
+                breakdowns += 1
+                nb_all = jd + bs
                 Xr = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
                 ws.set(cW, Xr)
                 for _ in range(2):
                     hh = ws.dot(0, nb_all, cW, bs)
                     ws.gemm(cW, bs, 0, nb_all, hh, alpha=-1.0, beta=1.0)
-                _cholqr(ws, cW, j + bs, bs)
-                R = np.zeros((bs, bs), dtype=complex)
-            H[:nb_all, j:j + bs] = h
-            H[j + bs:j + 2 * bs, j:j + bs] = R
-            j += bs
+                _cholqr(ws, cW, nb_all, bs)
+                H[nb_all:nb_all + bs, jd:jd + bs] = 0.0
+                jd = nb_all
+            j = jd
         # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
         T = H[:m, :m]
         Bres = H[m:m + bs, :m]
@@ -294,7 +333,7 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     host["projected"] = total - sum(host.values())
     return EigResult(lam=lam, X=X, errest=res[:k] / np.abs(theta[:k]), nconv=k, restarts=restarts,
                      op_columns=op_cols, op_calls=op_calls, seconds=total,
-                     ir_max_iterations=ir_iters, host_seconds=host)
+                     ir_max_iterations=ir_iters, host_seconds=host, breakdowns=breakdowns)
 
 
 def residuals(A, B, lam, X):

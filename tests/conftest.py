@@ -67,3 +67,20 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def pencil_with_null_block(A, B, k=2):
+    """(A, B') with B' = B minus k of its nonzero columns, and the unit
+    vectors on those columns: OP = (A - sigma B')^{-1} B' maps them to exactly
+    zero, so a Krylov start block made of them breaks down at once (the
+    invariant-subspace path of the expansion)."""
+    B = ss.csc_matrix(B)
+    nzcols = np.nonzero(np.diff(B.indptr))[0]
+    cols = nzcols[len(nzcols) // 3: len(nzcols) // 3 + k]
+    keep = np.ones(B.shape[1])
+    keep[cols] = 0.0
+    Bn = ss.csr_matrix(B @ ss.diags(keep))
+    Bn.eliminate_zeros()
+    E = np.zeros((A.shape[0], k), dtype=complex)
+    E[cols, np.arange(k)] = 1.0
+    return ss.csr_matrix(A), Bn, E

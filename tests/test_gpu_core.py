@@ -141,3 +141,63 @@ def test_spinover_golden(ctx, spinover, precision):
     np.testing.assert_allclose([got.real, got.imag], [SPINOVER_LAMBDA.real, SPINOVER_LAMBDA.imag], rtol=1e-8,
                                atol=1e-20)
     S.close()
+
+
+@pytest.mark.parametrize("bs", [1, 5, 8, 16])
+def test_expand_matches_host_driven(ctx, config1, bs):
+    """kore_ws_expand (coefficients kept on the device, Cholesky on the
+    device) against the same expansion driven from the host through the
+    workspace primitives (scipy Cholesky): projected matrix and basis agree."""
+    from kore_amd import backend, eigs
+    par, A, B = config1
+    say("case: expand vs host-driven, bs", bs)
+    S = _system(ctx, par, A, B)
+    S.factor(par.tau, 0)
+    nblk = 4
+    m = nblk * bs
+    ncol = m + 2 * bs
+    rng = np.random.default_rng(11)
+    X0 = rng.standard_normal((S.n, bs)) + 1j * rng.standard_normal((S.n, bs))
+    X0, _ = np.linalg.qr(X0)
+    out = []
+    for use_device in (True, False):
+        ws = backend.Workspace(S, ncol)
+        ws.set(0, X0)
+        H = np.zeros((m + bs, m), dtype=complex, order="F")
+        if use_device:
+            st, jd, _ = ws.expand(0, m, bs, m + bs, H)
+        else:
+            st, jd, _ = eigs.expand_host(ws, 0, m, bs, m + bs, H)
+        assert st == 0 and jd == m
+        out.append((H.copy(), ws.get(0, m + bs)))
+        ws.close()
+    (Hd, Vd), (Hh, Vh) = out
+    assert np.abs(Hd - Hh).max() <= 1e-10 * np.abs(Hh).max()
+    assert np.abs(Vd - Vh).max() <= 1e-10 * np.abs(Vh).max()
+    # the basis is orthonormal
+    G = Vd.conj().T @ Vd
+    assert np.abs(G - np.eye(m + bs)).max() <= 1e-12
+    S.close()
+
+
+def test_expand_breakdown_recovers(ctx, config1):
+    """Start block in the null space of B': the device Cholesky flags the
+    breakdown, the host continues with a random block, and the eigenvalues
+    match the oracle on the same pencil."""
+    from conftest import pencil_with_null_block
+    from kore_amd import eigs
+    from oracle import krylov_schur
+    par, A, B = config1
+    A2, Bn, E = pencil_with_null_block(A, B)
+    sigma = par.tau
+    say("case: expand breakdown, null columns", np.nonzero(E)[0])
+    S = _system(ctx, par, A2, Bn)
+    S.factor(sigma, 0)
+    r = eigs.eigs(S, sigma, nev=4, tol=1e-12, block=2, v0=E)
+    assert r.breakdowns >= 1 and r.nconv >= 4
+    lu = sla.splu((A2 - sigma * Bn).tocsc())
+    lam_o, _, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(Bn @ x), A2.shape[0], sigma, 4, tol=1e-13,
+                                               maxit=100)
+    for lo in lam_o[:4]:
+        assert np.min(np.abs(r.lam - lo)) / abs(lo) <= 1e-10
+    S.close()

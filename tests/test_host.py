@@ -167,3 +167,21 @@ def test_triangular_eigenvectors():
     S2 = S.copy()
     S2[3, 3] = S2[7, 7]
     assert np.all(np.isfinite(eigs._triu_eigvecs(S2)))
+
+
+def test_breakdown_path_recovers(config1):
+    """A start block in the null space of B' breaks down at the first
+    expansion (W = 0 exactly); the random replacement block must lead to the
+    pencil's true eigenvalues nearest sigma (dense QZ)."""
+    from conftest import pencil_with_null_block
+    par, A, B = config1
+    A2, Bn, E = pencil_with_null_block(A, B)
+    sigma = 1.0j
+    S = NumpySystem(A2, Bn, sigma)
+    r = eigs.eigs(S, sigma, nev=4, tol=1e-12, block=2, v0=E, workspace=NumpyWorkspace)
+    assert r.breakdowns >= 1 and r.nconv >= 4
+    w = la.eigvals(A2.toarray(), Bn.toarray())
+    w = w[np.isfinite(w)]
+    near = w[np.argsort(np.abs(w - sigma))[:4]]
+    for lam in r.lam[:4]:
+        assert np.min(np.abs(near - lam)) / abs(lam) <= 1e-9
