@@ -173,9 +173,8 @@ template <int NR> struct SkLds {
 };
 
 template <class T, int NR>
-__global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
+__device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
-  const SkinnyDesc<T> d = descs[blockIdx.y];
   const int r0 = blockIdx.x * SK_ROWS;
   if (r0 >= d.m) return;
   const int rl = threadIdx.x & (SK_ROWS - 1), ks = threadIdx.x / SK_ROWS;
@@ -257,6 +256,37 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
     }
   }
 }
+
+template <class T, int NR>
+__global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
+  const SkinnyDesc<T> d = descs[blockIdx.y];
+  skinny_tile<T, NR>(d, nrhs);
+}
+
+// one product, descriptor passed by value (no descriptor upload)
+template <class T, int NR>
+__global__ __launch_bounds__(256) void skinny1_kernel(const SkinnyDesc<T> d, int nrhs) {
+  skinny_tile<T, NR>(d, nrhs);
+}
+
+template <class T>
+hipError_t launch_skinny1(const SkinnyDesc<T>& d, int nrhs, hipStream_t s) {
+  if (d.m <= 0) return hipSuccess;
+  if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((d.m + SK_ROWS - 1) / SK_ROWS), 1);
+  if (nrhs == 1)
+    hipLaunchKernelGGL((skinny1_kernel<T, 1>), grid, dim3(256), 0, s, d, nrhs);
+  else if (nrhs <= 2)
+    hipLaunchKernelGGL((skinny1_kernel<T, 2>), grid, dim3(256), 0, s, d, nrhs);
+  else if (nrhs <= 4)
+    hipLaunchKernelGGL((skinny1_kernel<T, 4>), grid, dim3(256), 0, s, d, nrhs);
+  else if (nrhs <= 8)
+    hipLaunchKernelGGL((skinny1_kernel<T, 8>), grid, dim3(256), 0, s, d, nrhs);
+  else
+    hipLaunchKernelGGL((skinny1_kernel<T, 16>), grid, dim3(256), 0, s, d, nrhs);
+  return hipGetLastError();
+}
+template hipError_t launch_skinny1<double>(const SkinnyDesc<double>&, int, hipStream_t);
 
 template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s) {
@@ -485,17 +515,31 @@ hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<doubl
   return hipGetLastError();
 }
 
-// out[i] = sum_p in[p*len + i]: 32 outputs x 8 part-slices per workgroup,
-// the 8 slice sums combined through LDS (fixed order: deterministic).
+// out[i] = sum_p in[p*len + i]: 8 outputs x 32 part-slices per workgroup, so
+// that each thread adds at most nparts/32 values (short load chains even for
+// a handful of outputs); the slice sums are combined through LDS in a fixed
+// order (deterministic).
 __global__ __launch_bounds__(256) void sum_parts_kernel(const double* __restrict__ in, int nparts, int len,
                                                         double* __restrict__ out) {
-  __shared__ double red[8][32];
-  const int o = threadIdx.x & 31, sl = threadIdx.x >> 5;
-  const int i = blockIdx.x * 32 + o;
+  __shared__ double red[32][9];
+  const int o = threadIdx.x & 7, sl = threadIdx.x >> 3;
+  const int i = blockIdx.x * 8 + o;
   double s = 0.0;
-  if (i < len)
-    for (int p = sl; p < nparts; p += 8) s += in[(int64_t)p * len + i];
+  if (i < len) {
+    int p = sl;
+    for (; p + 96 < nparts; p += 128) {
+      const double a = in[(int64_t)p * len + i], b = in[(int64_t)(p + 32) * len + i];
+      const double c = in[(int64_t)(p + 64) * len + i], e = in[(int64_t)(p + 96) * len + i];
+      s += (a + b) + (c + e);
+    }
+    for (; p < nparts; p += 32) s += in[(int64_t)p * len + i];
+  }
   red[sl][o] = s;
+  __syncthreads();
+  if (sl < 8) {
+    double t = red[sl][o] + red[sl + 8][o] + red[sl + 16][o] + red[sl + 24][o];
+    red[sl][o] = t;
+  }
   __syncthreads();
   if (sl == 0 && i < len) {
     double t = 0.0;
@@ -507,7 +551,7 @@ __global__ __launch_bounds__(256) void sum_parts_kernel(const double* __restrict
 
 hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s) {
   if (len <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sum_parts_kernel, dim3((len + 31) / 32), dim3(256), 0, s, in, nparts, len, out);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((len + 7) / 8), dim3(256), 0, s, in, nparts, len, out);
   return hipGetLastError();
 }
 

@@ -48,6 +48,9 @@ template <class T>
 hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hipStream_t s);
 template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s);
+// single product with the descriptor passed by value
+template <class T>
+hipError_t launch_skinny1(const SkinnyDesc<T>& d, int nrhs, hipStream_t s);
 
 // ---- sparse / vector kernels (kernels_vec.hip) -----------------------------
 // CSR on the union pattern of A and B, rows and columns already in l-major

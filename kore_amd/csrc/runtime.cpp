@@ -1202,6 +1202,20 @@ static int32_t ws_gemm_dev(kore_ws* w, int d0, int nd, int s0, int ns, const cx<
   kore_sys* s = w->sys;
   kore_ctx* c = s->ctx;
   const int64_t n = s->n;
+  if (nd <= MAXR) {
+    // tall-skinny product (Gram-Schmidt update, Cholesky-QR): the row-sliced
+    // streaming kernel reads V once and wastes no matrix-core tile on
+    // padding; the descriptor travels as a kernel argument
+    SkinnyDesc<double> d{};
+    d.A[0] = wcol(w, s0); d.lda[0] = n;
+    d.X[0] = Q; d.ldx[0] = ldq;
+    d.k[0] = ns; d.alpha[0] = alpha;
+    d.Y = wcol(w, d0); d.ldy = n;
+    d.m = (int)n; d.beta = beta;
+    Timed tm(c, KORE_TIMER_ORTHO);
+    HIP_TRY(launch_skinny1<double>(d, nd, c->stream));
+    return KORE_OK;
+  }
   GemmDesc<double> g{};
   g.A[0] = wcol(w, s0); g.lda[0] = n;
   g.B[0] = Q; g.ldb[0] = ldq;
