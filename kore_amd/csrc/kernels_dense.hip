@@ -138,32 +138,75 @@ template <class T> __device__ __forceinline__ T tiny_pivot() { return sizeof(T) 
 // ---------------------------------------------------------------------------
 // batched Gauss-Jordan inverse with partial pivoting
 // ---------------------------------------------------------------------------
-// Rows are never moved while the block is reduced.  Column j is eliminated
-// with pivot row r_j (largest |re|+|im| among the rows not used yet, first
-// index on ties); the in-place Gauss-Jordan update then leaves in column j of
-// M the column r_j of the transformed identity, so that at the end
+// One workgroup of 512 threads per block.  Rows are never moved: column j is
+// eliminated with pivot row r_j (largest |re|+|im| among the rows not used
+// yet, first index on ties), so that at the end
 //     A^{-1}[k, l] = M[r_k, r^{-1}(l)],
-// which is applied once, through the caller's scratch block.
-// For a panel J = [p0, p0+w) with pivot rows P = (r_p0, ..., r_{p0+w-1}):
-//   1. the w panel columns (all rows) are copied to LDS and factored there
-//      with partial pivoting over the unused rows;
-//   2. X = M[P, J]^{-1} from that LU;
-//   3. T = X * M[P, :], with T[:, J] = X, staged in LDS (w x n);
-//   4. rows i not in P:  M[i, j] = [j not in J] M[i, j] - sum_k M[i, p0+k] T[k, j];
-//      rows P:           M[P, :] = T.
-// Each wave owns whole 32-row tiles in step 4, so the reads of M[i, J] (the
-// multipliers) precede all writes of row i.
+// applied once through the caller's scratch block.
+//
+// Panel J = [p0, p0+w) with pivot rows P:
+//   1. Gauss-Jordan on the panel columns over all rows, in registers (each
+//      row owned by TPR adjacent lanes, CPT columns each).  One column costs
+//      one barrier: the pivot is a packed 64-bit magnitude|row key reduced
+//      within each wave by DPP, and each wave's best row travels with its
+//      key through LDS.
+//      Afterwards the panel holds C: rows P hold X = M[P, J]^{-1} (permuted),
+//      the other rows hold -M[i, J] X -- the final columns J of M.
+//   2. R = M[P, :] (old pivot rows) is staged in LDS.
+//   3. Outside J:  M[i, j] <- [i not in P] M[i, j] + sum_k C[i, k] R[k, j]
+//      (matrix cores; C read back from the block, R from LDS).
+// Cross-lane moves within rows of 16 lanes (DPP) and lane reads.
+template <int CTRL> __device__ __forceinline__ unsigned dpp_u32(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+  return ((unsigned long long)dpp_u32<CTRL>((unsigned)(x >> 32)) << 32) | dpp_u32<CTRL>((unsigned)x);
+}
+template <int CTRL> __device__ __forceinline__ double dpp_t(double x) {
+  return __hiloint2double((int)dpp_u32<CTRL>((unsigned)__double2hiint(x)), (int)dpp_u32<CTRL>((unsigned)__double2loint(x)));
+}
+template <int CTRL> __device__ __forceinline__ float dpp_t(float x) {
+  return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(x)));
+}
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+// Maximum over the wavefront, returned to every lane: quad swaps and row
+// mirrors leave each row of 16 lanes with its maximum, then the four rows
+// are combined through lanes 15, 31, 47 and 63.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  v = umax64(v, dpp_u64<0xB1>(v));     // quad_perm [1,0,3,2]
+  v = umax64(v, dpp_u64<0x4E>(v));     // quad_perm [2,3,0,1]
+  v = umax64(v, dpp_u64<0x141>(v));    // row_half_mirror
+  v = umax64(v, dpp_u64<0x140>(v));    // row_mirror
+  return umax64(umax64(readlane_u64(v, 15), readlane_u64(v, 31)), umax64(readlane_u64(v, 47), readlane_u64(v, 63)));
+}
+
+// Orderable pivot key: valid flag, magnitude (as double; the low 9 mantissa
+// bits give way to the row so that ties go to the lower row), row.
+__device__ __forceinline__ unsigned long long pivot_key(double mag, int r) {
+  if (!(mag <= 1.7976931348623157e308)) mag = 0.0;   // NaN / inf never win
+  const unsigned long long b = (unsigned long long)__double_as_longlong(mag);
+  return (1ULL << 63) | (b & ~0x1FFULL) | (unsigned long long)(511 - r);
+}
+
 template <class T, int WP, int NMAX>
-__global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __restrict__ descs) {
+__global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __restrict__ descs) {
   using A4 = typename acc4<T>::type;
-  constexpr int LDP = NMAX + 4;          // panel: column c at sW[c*LDP + r]
-  constexpr int LDT = NMAX + 4;          // T:     row   k at sW[k*LDT + j]
-  __shared__ cx<T> sW[WP * LDP];
-  __shared__ cx<T> sX[WP * WP];          // column-major X
-  __shared__ T redv[4];
-  __shared__ int redi[4];
-  __shared__ int sPiv[WP];               // pivot rows of the current panel
+  constexpr int NT = 512;
+  constexpr int NW = NT / 64;
+  constexpr int TPR = NT / NMAX;         // threads per row
+  constexpr int CPT = WP / TPR;          // panel columns per thread
+  constexpr int LDR = NMAX + 4;          // R: row k at sR[k*LDR + j]
+  static_assert(TPR >= 1 && CPT * TPR == WP, "panel split");
+  __shared__ cx<T> sR[WP * LDR];
+  __shared__ cx<T> sCand[2][NW][WP];      // each wave's best candidate row (double-buffered)
+  __shared__ unsigned long long sKey[2][NW];
   __shared__ short sRole[NMAX];          // -1 unused, -2 used earlier, c >= 0: pivot c of this panel
+  __shared__ int sPiv[WP];
   __shared__ int sSing;
 
   const InvDesc<T> d = descs[blockIdx.x];
@@ -173,125 +216,119 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
   int* __restrict__ rowof = d.ipiv;      // r_j: pivot row of column j
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
+  const int r = tid / TPR, h = tid % TPR;   // owned row and column slice
+  const bool live = r < n;
+  int role = -1;                          // register copy of sRole[r]
   if (tid == 0) sSing = 0;
-  for (int r = tid; r < n; r += 256) sRole[r] = -1;
+  for (int q = tid; q < NMAX; q += NT) sRole[q] = -1;
 
   for (int p0 = 0; p0 < n; p0 += WP) {
     const int w = min(WP, n - p0);
-    __syncthreads();
-    // 1. panel (all rows) -> LDS, zero the unused columns
-    for (int idx = tid; idx < WP * n; idx += 256) {
-      const int c = idx / n, r = idx - c * n;
-      sW[c * LDP + r] = (c < w) ? M[(int64_t)(p0 + c) * ld + r] : czero<T>();
+    // 1. panel -> registers
+    cx<T> v[CPT];
+#pragma unroll
+    for (int jj = 0; jj < CPT; ++jj) {
+      const int j = h * CPT + jj;
+      v[jj] = (live && j < w) ? M[(int64_t)(p0 + j) * ld + r] : czero<T>();
     }
-    __syncthreads();
-    for (int c = 0; c < w; ++c) {
-      T best = T(-1);
-      int bidx = 0x7fffffff;
-      for (int r = tid; r < n; r += 256) {
-        if (sRole[r] != -1) continue;
-        const T v = cabs1(sW[c * LDP + r]);
-        if (v > best) { best = v; bidx = r; }
-      }
+    // One barrier per column: every wave publishes its best candidate row
+    // with its key, and after the barrier all threads pick the best of the
+    // eight and read the pivot row from that wave's slot.
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const T ov = shfl_xor_t(best, off);
-        const int oi = shfl_xor_t(bidx, off);
-        if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
-      }
-      if (lane == 0) { redv[wave] = best; redi[wave] = bidx; }
-      __syncthreads();
-      // every thread reduces the four wave results itself (no second barrier)
-      T bv = redv[0];
-      int p = redi[0];
+    for (int c = 0; c < WP; ++c) {
+      if (c < w) {                        // uniform
+        const int par = c & 1;
+        unsigned long long key = 0;
+        if (live && h == c / CPT && role == -1) key = pivot_key((double)cabs1(v[c % CPT]), r);
+        key = wave_max_u64(key);
+        if (key != 0 && r == 511 - (int)(key & 0x1FFULL)) {
 #pragma unroll
-      for (int q = 1; q < 4; ++q)
-        if (redv[q] > bv || (redv[q] == bv && redi[q] < p)) { bv = redv[q]; p = redi[q]; }
-      if (p < 0 || p >= n) {   // no finite candidate left: take the first unused row
-        p = 0;
-        while (p < n && sRole[p] != -1) ++p;
-        if (p >= n) p = 0;
-      }
-      if (tid == 0) {
-        sPiv[c] = p;
-        if (!(bv > T(0)) && sSing == 0) sSing = p0 + c + 1;
-      }
-      cx<T> pv = sW[c * LDP + p];
-      T den = pv.re * pv.re + pv.im * pv.im;
-      if (!(den > T(0))) { pv = cx<T>{tiny_pivot<T>(), T(0)}; den = pv.re * pv.re; }
-      const cx<T> ipv{pv.re / den, -pv.im / den};
-      for (int r = tid; r < n; r += 256) {
-        if (r == p || sRole[r] != -1) continue;
-        const cx<T> l = cmul(sW[c * LDP + r], ipv);
-        sW[c * LDP + r] = l;
-        for (int j = c + 1; j < w; ++j) {
-          const cx<T> u = sW[j * LDP + p];
-          cx<T> x = sW[j * LDP + r];
-          x.re -= l.re * u.re - l.im * u.im;
-          x.im -= l.re * u.im + l.im * u.re;
-          sW[j * LDP + r] = x;
+          for (int jj = 0; jj < CPT; ++jj) sCand[par][wave][h * CPT + jj] = v[jj];
+        }
+        if (lane == 0) sKey[par][wave] = key;
+        __syncthreads();
+        unsigned long long best = sKey[par][0];
+        int q = 0;
+#pragma unroll
+        for (int qq = 1; qq < NW; ++qq) {
+          const unsigned long long kq = sKey[par][qq];
+          if (kq > best) {
+            best = kq;
+            q = qq;
+          }
+        }
+        // every column has an unused row left, so a valid key always exists
+        const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
+        if (tid == 0) {
+          sRole[p] = (short)c;
+          sPiv[c] = p;
+        }
+        if (r == p) role = c;
+        const cx<T>* __restrict__ prow = sCand[par][q];
+        cx<T> pv = prow[c];
+        T den = pv.re * pv.re + pv.im * pv.im;
+        if (!(den > T(0))) {
+          if (tid == 0 && sSing == 0) sSing = p0 + c + 1;
+          pv = cx<T>{tiny_pivot<T>(), T(0)};
+          den = pv.re * pv.re;
+        }
+        const T iden = T(1) / den;
+        const cx<T> ipv{pv.re * iden, -pv.im * iden};
+        // this row's entry in column c, held by lane slice c / CPT of the row
+        cx<T> vc = v[c % CPT];
+        if constexpr (TPR == 2) {
+          if (c / CPT == 0) {
+            vc.re = dpp_t<0xA0>(vc.re);   // quad_perm [0,0,2,2]
+            vc.im = dpp_t<0xA0>(vc.im);
+          } else {
+            vc.re = dpp_t<0xF5>(vc.re);   // quad_perm [1,1,3,3]
+            vc.im = dpp_t<0xF5>(vc.im);
+          }
+        }
+        if (live) {
+          if (r == p) {
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) {
+              const int j = h * CPT + jj;
+              v[jj] = (j == c) ? ipv : cmul(v[jj], ipv);
+            }
+          } else {
+            const cx<T> l = cmul(vc, ipv);
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) {
+              const int j = h * CPT + jj;
+              if (j == c) {
+                v[jj] = cx<T>{-l.re, -l.im};
+              } else if (j < w) {
+                const cx<T> u = prow[j];
+                v[jj].re -= l.re * u.re - l.im * u.im;
+                v[jj].im -= l.re * u.im + l.im * u.re;
+              }
+            }
+          }
         }
       }
-      __syncthreads();                 // all reads of sRole for this column done
-      if (tid == 0) sRole[p] = (short)c;
-      __syncthreads();
+    }
+    // C -> columns J of the block (final values)
+    if (live) {
+#pragma unroll
+      for (int jj = 0; jj < CPT; ++jj) {
+        const int j = h * CPT + jj;
+        if (j < w) M[(int64_t)(p0 + j) * ld + r] = v[jj];
+      }
     }
     if (tid < w) rowof[p0 + tid] = sPiv[tid];
-    // 2. X = (pivot block)^{-1}; the LU of M[P, J] is in the panel rows P:
-    //    U(i, k) = sW[k][P_i] (k >= i),  L(i, k) = sW[k][P_i] (k < i, unit diagonal)
-    if (tid < w) {
-      cx<T> y[WP];
-#pragma unroll
-      for (int i = 0; i < WP; ++i) {
-        if (i >= w) { y[i] = czero<T>(); continue; }
-        cx<T> sacc = (i == tid) ? cx<T>{T(1), T(0)} : czero<T>();
-        const int pi = sPiv[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) sacc = csub(sacc, cmul(sW[k * LDP + pi], y[k]));
-        y[i] = sacc;
-      }
-#pragma unroll
-      for (int ii = WP - 1; ii >= 0; --ii) {
-        if (ii >= w) continue;
-        const int pi = sPiv[ii];
-        cx<T> sacc = y[ii];
-#pragma unroll
-        for (int k = ii + 1; k < WP; ++k)
-          if (k < w) sacc = csub(sacc, cmul(sW[k * LDP + pi], y[k]));
-        cx<T> u = sW[ii * LDP + pi];
-        T den = u.re * u.re + u.im * u.im;
-        if (!(den > T(0))) { u = cx<T>{tiny_pivot<T>(), T(0)}; den = u.re * u.re; }
-        y[ii] = cmul(sacc, cx<T>{u.re / den, -u.im / den});
-      }
-#pragma unroll
-      for (int i = 0; i < WP; ++i) sX[tid * WP + i] = y[i];
-    }
-    if (tid >= w && tid < WP) {
-      for (int i = 0; i < WP; ++i) sX[tid * WP + i] = czero<T>();
+    // 2. R = M[P, :] outside J (rows k >= w are zero)
+    for (int idx = tid; idx < WP * n; idx += NT) {
+      const int k = idx % WP, j = idx / WP;
+      cx<T> x = czero<T>();
+      if (k < w && (j < p0 || j >= p0 + w)) x = M[(int64_t)j * ld + sPiv[k]];
+      sR[k * LDR + j] = x;
     }
     __syncthreads();
-    // 3. T = X * M[P, :], T[:, J] = X ; rows k >= w of T are zero
-    for (int j = tid; j < n; j += 256) {
-      if (j >= p0 && j < p0 + w) {
-        for (int k = 0; k < WP; ++k) sW[k * LDT + j] = (k < w) ? sX[(j - p0) * WP + k] : czero<T>();
-      } else {
-        cx<T> v[WP];
-        const cx<T>* mc = M + (int64_t)j * ld;
-#pragma unroll
-        for (int k = 0; k < WP; ++k) v[k] = (k < w) ? mc[sPiv[k]] : czero<T>();
-#pragma unroll 2
-        for (int k = 0; k < WP; ++k) {
-          cx<T> sacc = czero<T>();
-#pragma unroll
-          for (int q = 0; q < WP; ++q) sacc = cfma(sacc, sX[q * WP + k], v[q]);
-          sW[k * LDT + j] = sacc;
-        }
-      }
-    }
-    __syncthreads();
-    // 4. rank-w update, one wave per 32-row tile
+    // 3. rank-w update outside J, one wave per 32-row tile
     const int ntr = (n + 31) / 32, ntc = (n + 31) / 32;
-    for (int tr = wave; tr < ntr; tr += 4) {
+    for (int tr = wave; tr < ntr; tr += NW) {
       const int i0 = tr * 32;
       cx<T> g[2][WP / 4];
 #pragma unroll
@@ -301,8 +338,15 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
           const int gi = i0 + 16 * rs + l16, gk = 4 * kk + lq;
           g[rs][kk] = (gi < n && gk < w) ? M[(int64_t)(p0 + gk) * ld + gi] : czero<T>();
         }
+      int prole[2];
+#pragma unroll
+      for (int rs = 0; rs < 2; ++rs) {
+        const int gi = i0 + 16 * rs + l16;
+        prole[rs] = gi < n ? (int)sRole[gi] : -1;
+      }
       for (int tc = 0; tc < ntc; ++tc) {
         const int j0 = tc * 32;
+        if (j0 >= p0 && j0 + 32 <= p0 + w) continue;   // tile inside J: already final
         A4 cre[2][2], cim[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -317,7 +361,7 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
 #pragma unroll
           for (int cs = 0; cs < 2; ++cs) {
             const int gj = j0 + 16 * cs + l16;
-            b[cs] = (gj < n) ? sW[(4 * kk + lq) * LDT + gj] : czero<T>();
+            b[cs] = (gj < n) ? sR[(4 * kk + lq) * LDR + gj] : czero<T>();
           }
 #pragma unroll
           for (int rs = 0; rs < 2; ++rs)
@@ -329,44 +373,40 @@ __global__ __launch_bounds__(256) void gj_inverse_kernel(const InvDesc<T>* __res
 #pragma unroll
           for (int cs = 0; cs < 2; ++cs)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int rr = 0; rr < 4; ++rr) {
               const int gi = i0 + 16 * rs + l16;
-              const int gj = j0 + 16 * cs + mfma_mrow<T>(lq, r);
-              if (gi < n && gj < n) {
+              const int gj = j0 + 16 * cs + mfma_mrow<T>(lq, rr);
+              if (gi < n && gj < n && (gj < p0 || gj >= p0 + w)) {
                 cx<T>* cp = M + (int64_t)gj * ld + gi;
-                const int role = sRole[gi];
-                cx<T> v;
-                if (role >= 0 && role < WP) {
-                  v = sW[role * LDT + gj];
-                } else if (gj >= p0 && gj < p0 + w) {
-                  v = cx<T>{-cre[rs][cs][r], -cim[rs][cs][r]};
-                } else {
-                  v = *cp;
-                  v.re -= cre[rs][cs][r];
-                  v.im -= cim[rs][cs][r];
+                cx<T> x{cre[rs][cs][rr], cim[rs][cs][rr]};
+                if (!(prole[rs] >= 0)) {
+                  const cx<T> o = *cp;
+                  x.re += o.re;
+                  x.im += o.im;
                 }
-                *cp = v;
+                *cp = x;
               }
             }
       }
     }
     __syncthreads();
     // this panel's pivot rows become "used earlier"
-    for (int c = tid; c < w; c += 256) sRole[sPiv[c]] = -2;
+    if (role >= 0) role = -2;
+    for (int c = tid; c < w; c += NT) sRole[sPiv[c]] = -2;
   }
   __syncthreads();
   // A^{-1}[k, l] = M[r_k, r^{-1}(l)]: permute into the scratch block, copy back
-  int* rinv = reinterpret_cast<int*>(sX);          // n ints fit in sX (WP*WP complex >= NMAX ints)
-  for (int j = tid; j < n; j += 256) rinv[rowof[j]] = j;
+  int* rinv = reinterpret_cast<int*>(sR);
+  for (int j = tid; j < n; j += NT) rinv[rowof[j]] = j;
   __syncthreads();
   cx<T>* __restrict__ S = d.scratch;
   for (int l = 0; l < n; ++l) {
     const cx<T>* src = M + (int64_t)rinv[l] * ld;
-    for (int k = tid; k < n; k += 256) S[(int64_t)l * n + k] = src[rowof[k]];
+    for (int k = tid; k < n; k += NT) S[(int64_t)l * n + k] = src[rowof[k]];
   }
   __syncthreads();
   for (int l = 0; l < n; ++l)
-    for (int k = tid; k < n; k += 256) M[(int64_t)l * ld + k] = S[(int64_t)l * n + k];
+    for (int k = tid; k < n; k += NT) M[(int64_t)l * ld + k] = S[(int64_t)l * n + k];
   if (tid == 0 && d.info) *d.info = sSing;
 }
 
@@ -386,14 +426,16 @@ hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hip
   if (ndesc <= 0) return hipSuccess;
   if constexpr (sizeof(T) == 8) {
     if (nmax <= 256)
-      hipLaunchKernelGGL((gj_inverse_kernel<T, 32, 256>), dim3(ndesc), dim3(256), 0, s, d_descs);
+      hipLaunchKernelGGL((gj_inverse_kernel<T, 32, 256>), dim3(ndesc), dim3(512), 0, s, d_descs);
     else if (nmax <= 512)
-      hipLaunchKernelGGL((gj_inverse_kernel<T, 16, 512>), dim3(ndesc), dim3(256), 0, s, d_descs);
+      hipLaunchKernelGGL((gj_inverse_kernel<T, 16, 512>), dim3(ndesc), dim3(512), 0, s, d_descs);
     else
       return hipErrorInvalidValue;
   } else {
-    if (nmax <= 512)
-      hipLaunchKernelGGL((gj_inverse_kernel<T, 32, 512>), dim3(ndesc), dim3(256), 0, s, d_descs);
+    if (nmax <= 256)
+      hipLaunchKernelGGL((gj_inverse_kernel<T, 32, 256>), dim3(ndesc), dim3(512), 0, s, d_descs);
+    else if (nmax <= 512)
+      hipLaunchKernelGGL((gj_inverse_kernel<T, 16, 512>), dim3(ndesc), dim3(512), 0, s, d_descs);
     else
       return hipErrorInvalidValue;
   }

@@ -283,7 +283,9 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
         # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
         T = H[:m, :m]
         Bres = H[m:m + bs, :m]
+        tq = time.perf_counter()
         S, Q = sla.schur(T, output="complex")
+        host["schur"] = host.get("schur", 0.0) + time.perf_counter() - tq
         theta_s = np.diag(S).copy()
         Zs = _triu_eigvecs(S)
         Ys = Q @ Zs
@@ -308,7 +310,9 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
         keep = max(bs, min(keep, m - bs))
         select = np.zeros(m, dtype=np.int32)
         select[order[:keep]] = 1
+        tq = time.perf_counter()
         St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, S, Q, job="N")
+        host["reorder"] = host.get("reorder", 0.0) + time.perf_counter() - tq
         if info != 0 or msel != keep:
             raise RuntimeError("Schur reordering failed (info %d, %d of %d)" % (info, msel, keep))
         ws.gemm(cT, keep, 0, m, Qt[:, :keep], alpha=1.0, beta=0.0)
