@@ -128,6 +128,29 @@ int32_t kore_factor_release(kore_sys* sys);
 int32_t kore_solve(kore_sys* sys, int32_t nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
                    double ir_tol, int32_t ir_maxit, kore_solve_info* info);
 
+/* Solves (A - sigma B) x = b for one right-hand side (host arrays, Kore
+ * order) by restarted FP64 GMRES right-preconditioned with the factor:
+ * every Krylov step is one factor solve and one FP64 sparse product, so a
+ * complex64 factor acts as a preconditioner and the answer carries FP64
+ * accuracy where plain refinement would stall.  Stops when
+ * ||b - (A - sigma B) x|| <= tol (||A - sigma B||_F ||x|| + ||b||), or after
+ * maxit factor solves (KORE_WARN_IR_NOT_CONVERGED); restart <= 30.
+ * info: iterations = factor solves, converged, rel_correction = final
+ * normwise backward error.
+ * Replaces: KSPSolve of bin/solve.py:227 (forced problems), with GMRES
+ * (-ksp_type gmres, tools/ramp.sh) preconditioned by the factor. */
+int32_t kore_solve_gmres(kore_sys* sys, const double* b, double* x, double tol, int32_t maxit, int32_t restart,
+                         kore_solve_info* info);
+
+/* Residual norms of nrhs candidate solutions from the stored (unscaled)
+ * matrices, on the device: r_c = (A - (sigma + lam_c) B) x_c - b_c and
+ * rnorm[c] = ||r_c||_2.  x and b are host arrays in Kore order (column-major,
+ * ldx / ldb); lam (nrhs interleaved complex) and b may be NULL (zero).
+ * Replaces: the host-side true residuals behind -eps_true_residual and the
+ * error table of bin/solve.py:125-129, and the check of a forced solution. */
+int32_t kore_residual(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
+                      const double* b, int64_t ldb, double sre, double sim, double* rnorm);
+
 /* ---- Krylov workspace (device, degree-major order) ------------------------- */
 /* A block of ncol device vectors of length sizmat (complex128).
  * Replaces: SLEPc's BV basis object inside EPS Krylov-Schur. */

@@ -79,6 +79,8 @@ SIGNATURES = [
     ("kore_factor", _I32, [_P, _D, _D, _I32, C.POINTER(FactorInfo)]),
     ("kore_factor_release", _I32, [_P]),
     ("kore_solve", _I32, [_P, _I32, _PD, _I64, _PD, _I64, _D, _I32, C.POINTER(SolveInfo)]),
+    ("kore_residual", _I32, [_P, _I32, _PD, _I64, _PD, _PD, _I64, _D, _D, _PD]),
+    ("kore_solve_gmres", _I32, [_P, _PD, _PD, _D, _I32, _I32, C.POINTER(SolveInfo)]),
     ("kore_ws_create", _I32, [_P, _I32, C.POINTER(_P)]),
     ("kore_ws_destroy", None, [_P]),
     ("kore_ws_set", _I32, [_P, _I32, _I32, _PD, _I64]),
@@ -215,14 +217,14 @@ class System:
         self.n = n
         self.ctx = ctx
         self.lay = lay
-        self._a = (A.indptr.astype(np.int32), A.indices.astype(np.int32),
-                   np.ascontiguousarray(A.data.astype(np.complex128)))
+        self._a = (A.indptr.astype(np.int32, copy=False), A.indices.astype(np.int32, copy=False),
+                   np.ascontiguousarray(A.data.astype(np.complex128, copy=False)))
         args_b = [None, None, None, 0]
         if B is not None:
             B = ss.csr_matrix(B)
             bc = np.iscomplexobj(B.data)
-            bdata = np.ascontiguousarray(B.data.astype(np.complex128 if bc else np.float64))
-            self._b = (B.indptr.astype(np.int32), B.indices.astype(np.int32), bdata)
+            bdata = np.ascontiguousarray(B.data.astype(np.complex128 if bc else np.float64, copy=False))
+            self._b = (B.indptr.astype(np.int32, copy=False), B.indices.astype(np.int32, copy=False), bdata)
             args_b = [self._b[0].ctypes.data_as(_PI32), self._b[1].ctypes.data_as(_PI32),
                       self._b[2].ctypes.data_as(_PD), 1 if bc else 0]
         p = _P()
@@ -230,6 +232,7 @@ class System:
                                  self._a[1].ctypes.data_as(_PI32), _dptr(self._a[2]),
                                  args_b[0], args_b[1], args_b[2], args_b[3], C.byref(p)))
         self._p = p
+        self._a = self._b = None           # the library keeps its own copies
         self._children = weakref.WeakSet()
         ctx._children.add(self)
         self.sigma = None
@@ -279,6 +282,35 @@ class System:
                               _dptr(x), self.n, float(ir_tol), int(ir_maxit), C.byref(info))
         _check(rc)      # 1 (refinement not converged) is reported through info.converged
         return (x[:, 0] if one else x), info
+
+    def solve_gmres(self, b, tol=1e-14, maxit=60, restart=20):
+        """x = (A - sigma B)^{-1} b (one column, Kore order) by FP64 GMRES
+        right-preconditioned with the factor (kore_solve_gmres)."""
+        bb = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(self.n))
+        x = np.zeros(self.n, dtype=np.complex128)
+        info = SolveInfo()
+        rc = lib().kore_solve_gmres(self._p, _dptr(bb), _dptr(x), float(tol), int(maxit), int(restart),
+                                    C.byref(info))
+        _check(rc)      # 1: not converged, reported through info.converged
+        return x, info
+
+    def residual_norms(self, X, lam=None, b=None, shift=0j):
+        """||(A - (shift + lam_c) B) x_c - b_c||_2 per column, on the device
+        from the stored matrices (X, b in Kore order; lam, b optional)."""
+        X = np.asfortranarray(np.asarray(X, dtype=np.complex128).reshape(self.n, -1))
+        k = X.shape[1]
+        lp = None
+        if lam is not None:
+            lam = np.ascontiguousarray(np.asarray(lam, dtype=np.complex128).reshape(k))
+            lp = _dptr(lam)
+        bp = None
+        if b is not None:
+            b = np.asfortranarray(np.asarray(b, dtype=np.complex128).reshape(self.n, k))
+            bp = _dptr(b)
+        out = np.zeros(k)
+        shift = complex(shift)
+        _check(lib().kore_residual(self._p, k, _dptr(X), self.n, lp, bp, self.n, shift.real, shift.imag, _dptr(out)))
+        return out
 
     def close(self):
         if getattr(self, "_p", None):

@@ -259,8 +259,7 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
 
 template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
-  const SkinnyDesc<T> d = descs[blockIdx.y];
-  skinny_tile<T, NR>(d, nrhs);
+  skinny_tile<T, NR>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
 }
 
 // one product, descriptor passed by value (no descriptor upload)
@@ -633,6 +632,33 @@ hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, c
                            hipStream_t s) {
   if (n < 1 || n > 16) return hipErrorInvalidValue;
   hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, G, ldg, n, R, Rinv, flag);
+  return hipGetLastError();
+}
+
+// partial[b] = sum over a grid-stride slice of |va[k] - sigma vb[k]|^2
+// (Frobenius norm of A - sigma B on the union pattern; one value per block).
+__global__ __launch_bounds__(256) void csr_fro2_kernel(const cx<double>* __restrict__ va,
+                                                       const cx<double>* __restrict__ vb, int64_t nnz, double sre,
+                                                       double sim, double* __restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * 256) {
+    const cx<double> a = va[k], b = vb[k];
+    const double re = a.re - (sre * b.re - sim * b.im), im = a.im - (sre * b.im + sim * b.re);
+    acc += re * re + im * im;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+hipError_t launch_csr_fro2(const cx<double>* va, const cx<double>* vb, int64_t nnz, double sre, double sim,
+                           double* partial, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(csr_fro2_kernel, dim3(nblocks), dim3(256), 0, s, va, vb, nnz, sre, sim, partial);
   return hipGetLastError();
 }
 

@@ -99,6 +99,9 @@ hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<doubl
                         int64_t n, int nchunk, int64_t rows_per_chunk, cx<double>* partial, hipStream_t s);
 // sum nparts consecutive arrays of len doubles: out[i] = sum_p in[p*len + i]
 hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s);
+// partial[b] (b < nblocks) = slice sums of |va - sigma vb|^2 over nnz entries
+hipError_t launch_csr_fro2(const cx<double>* va, const cx<double>* vb, int64_t nnz, double sre, double sim,
+                           double* partial, int nblocks, hipStream_t s);
 hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, cx<double>* Rinv, int* flag,
                            hipStream_t s);
 // fill complex buffer with zeros (graph-capture-safe clear)

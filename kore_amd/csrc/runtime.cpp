@@ -19,6 +19,7 @@
 #include <string>
 #include <unordered_map>
 #include <vector>
+#include <thread>
 
 #include "kore_amd.h"
 #include "kore_device.hpp"
@@ -320,8 +321,8 @@ struct kore_sys {
   std::vector<int> bsz;
   std::vector<int64_t> boff;
   std::vector<int64_t> perm;
-  std::vector<int32_t> h_ip, h_ix, h_blk;
-  std::vector<zd> h_va, h_vb;
+  std::vector<int32_t> h_ip, h_blk;
+  int64_t nnz = 0;                 // entries of the union pattern (device arrays only)
   bool has_b = false;
   DevBuf d_ip, d_ix, d_va, d_vb, d_perm, d_blk, d_boff, d_scale, d_scr, d_part;
   CsrDev csr{};
@@ -338,6 +339,8 @@ struct kore_sys {
   double sre = 0, sim = 0;
   std::vector<double> h_scale;
   Uploader up;
+  DevBuf gm;                 // GMRES basis (restart+1 vectors), dot partials, coefficients
+  int gm_cols = 0;
 };
 
 struct kore_ws {
@@ -833,6 +836,171 @@ static int32_t solve_dev(kore_sys* s, const cx<double>* b, int64_t ldb, cx<doubl
 }
 
 // ---------------------------------------------------------------------------
+// GMRES-based iterative refinement (one right-hand side)
+// ---------------------------------------------------------------------------
+// FP64 restarted GMRES on (A - sigma B) x = b, right-preconditioned by the
+// factor: every Krylov step is one factor solve (complex64 or FP64) and one
+// FP64 sparse product; the basis lives on the device, the (restart+1) x
+// restart Hessenberg matrix and its Givens rotations on the host.  Converges
+// where plain refinement with a complex64 factor stalls (condition numbers up
+// to about 1/eps_fp32), because the factor only has to be a preconditioner.
+static constexpr int GM_MAXRESTART = 30, GM_CHUNKS = 512;
+
+static int32_t prec_apply(kore_sys* s, const cx<double>* v, cx<double>* z) {
+  if (s->f->prec == KORE_PREC_C64)
+    return refine_sweep<float>(s, static_cast<Factor<float>*>(s->f.get()), v, s->n, z, s->n, 1, true);
+  return refine_sweep<double>(s, static_cast<Factor<double>*>(s->f.get()), v, s->n, z, s->n, 1, true);
+}
+
+// out (na complex, device) = V[:, 0:na]^H w
+static int32_t gm_dot(kore_sys* s, const cx<double>* V, int na, const cx<double>* w, cx<double>* part,
+                      cx<double>* out) {
+  const int64_t n = s->n;
+  const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(GM_CHUNKS, (n + 63) / 64));
+  const int64_t chunk = (n + nchunk - 1) / nchunk;
+  Timed tm(s->ctx, KORE_TIMER_ORTHO);
+  HIP_TRY(launch_tsdot(V, n, na, w, n, 1, n, nchunk, chunk, part, s->ctx->stream));
+  HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * na, reinterpret_cast<double*>(out),
+                           s->ctx->stream));
+  return KORE_OK;
+}
+
+// y = beta y + alpha V[:, 0:k] c   (c: k complex on the device)
+static int32_t gm_comb(kore_sys* s, const cx<double>* V, int k, const cx<double>* c, cx<double>* y, double alpha,
+                       double beta) {
+  SkinnyDesc<double> d{};
+  d.A[0] = V; d.lda[0] = s->n;
+  d.X[0] = c; d.ldx[0] = k;
+  d.k[0] = k; d.alpha[0] = alpha;
+  d.Y = y; d.ldy = s->n;
+  d.m = (int)s->n; d.beta = beta;
+  Timed tm(s->ctx, KORE_TIMER_ORTHO);
+  HIP_TRY(launch_skinny1<double>(d, 1, s->ctx->stream));
+  return KORE_OK;
+}
+
+// b, x: device vectors (degree-major); x receives the solution.
+static int32_t gmres_dev(kore_sys* s, const cx<double>* b, cx<double>* x, double tol, int maxit, int m, double nA,
+                         kore_solve_info* info) {
+  kore_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  const int64_t n = s->n;
+  if (m < 1 || m > GM_MAXRESTART) return fail(KORE_ERR_ARG, "GMRES restart length must be 1..%d", GM_MAXRESTART);
+  if (s->gm_cols < m + 1) {
+    TRY(s->gm.alloc(c, ((size_t)(GM_MAXRESTART + 1) * n + (size_t)GM_CHUNKS * (GM_MAXRESTART + 1) + 4 * 64) *
+                           sizeof(zd)));
+    s->gm_cols = GM_MAXRESTART + 1;
+  }
+  cx<double>* V = s->gm.as<cx<double>>();
+  cx<double>* part = V + (size_t)(GM_MAXRESTART + 1) * n;
+  cx<double>* h1 = part + (size_t)GM_CHUNKS * (GM_MAXRESTART + 1);
+  cx<double>* h2 = h1 + 64;
+  cx<double>* yv = h2 + 64;
+  cx<double>* z = s->st;       // preconditioned vector
+  cx<double>* r = s->rr;       // residual
+  double nb = 0;
+  TRY(colnorms2<double, double>(s, b, n, nullptr, 0, 1, &nb, nullptr));
+  // x0 = M^{-1} b
+  TRY(prec_apply(s, b, x));
+  int its = 1;
+  bool conv = false;
+  double berr = 1.0;
+  std::vector<zd> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), hh(2 * 64);
+  for (;;) {
+    // r = b - (A - sigma B) x
+    {
+      Timed tm(c, KORE_TIMER_SPMV);
+      HIP_TRY(launch_copy_cols(b, n, r, n, n, 1, st));
+      HIP_TRY(launch_spmv(s->csr, -1.0, 0.0, s->sre, s->sim, x, n, r, n, 1, 1.0, st));
+    }
+    double beta = 0, nx = 0;
+    TRY(colnorms2<double, double>(s, r, n, x, n, 1, &beta, &nx));
+    const double thr = tol * (nA * nx + nb);
+    berr = (nA * nx + nb) > 0 ? beta / (nA * nx + nb) : 0.0;
+    if (beta <= thr) {
+      conv = true;
+      break;
+    }
+    if (its >= maxit) break;
+    HIP_TRY(launch_gather_scale<double, double>(r, n, V, n, n, 1, nullptr, nullptr, 1.0 / beta, st));
+    std::fill(H.begin(), H.end(), zd(0));
+    std::fill(g.begin(), g.end(), zd(0));
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && its < maxit; ++j) {
+      cx<double>* vj = V + (size_t)j * n;
+      cx<double>* w = V + (size_t)(j + 1) * n;
+      TRY(prec_apply(s, vj, z));
+      ++its;
+      {
+        Timed tm(c, KORE_TIMER_SPMV);
+        HIP_TRY(launch_spmv(s->csr, 1.0, 0.0, -s->sre, -s->sim, z, n, w, n, 1, 0.0, st));
+      }
+      // classical Gram-Schmidt, twice
+      TRY(gm_dot(s, V, j + 1, w, part, h1));
+      TRY(gm_comb(s, V, j + 1, h1, w, -1.0, 1.0));
+      TRY(gm_dot(s, V, j + 1, w, part, h2));
+      TRY(gm_comb(s, V, j + 1, h2, w, -1.0, 1.0));
+      HIP_TRY(hipMemcpyAsync(hh.data(), h1, 2 * 64 * sizeof(zd), hipMemcpyDeviceToHost, st));
+      double hn = 0;
+      TRY(colnorms2<double, double>(s, w, n, nullptr, 0, 1, &hn, nullptr));   // synchronises
+      zd* Hj = H.data() + (size_t)j * (m + 1);
+      for (int i = 0; i <= j; ++i) Hj[i] = hh[i] + hh[64 + i];
+      Hj[j + 1] = hn;
+      if (hn > 0) HIP_TRY(launch_gather_scale<double, double>(w, n, w, n, n, 1, nullptr, nullptr, 1.0 / hn, st));
+      // previous rotations, then a new one that zeroes H[j+1, j]
+      for (int i = 0; i < j; ++i) {
+        const zd t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+        Hj[i + 1] = -std::conj(sn[i]) * Hj[i] + cs[i] * Hj[i + 1];
+        Hj[i] = t;
+      }
+      const double a = std::abs(Hj[j]), bb = std::abs(Hj[j + 1]);
+      const double rho = std::hypot(a, bb);
+      if (rho == 0) {
+        cs[j] = 1;
+        sn[j] = 0;
+      } else if (a == 0) {
+        cs[j] = 0;
+        sn[j] = std::conj(Hj[j + 1]) / bb;
+      } else {
+        cs[j] = a / rho;
+        sn[j] = (Hj[j] / a) * std::conj(Hj[j + 1]) / rho;
+      }
+      Hj[j] = cs[j] * Hj[j] + sn[j] * Hj[j + 1];
+      Hj[j + 1] = 0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] = cs[j] * g[j];
+      if (std::abs(g[j + 1]) <= 0.5 * thr || hn == 0) {
+        ++j;
+        break;
+      }
+    }
+    // y = R^{-1} g, u = V y (into r), x += M^{-1} u
+    const int k = j;
+    std::vector<zd> y(k);
+    for (int i = k - 1; i >= 0; --i) {
+      zd t = g[i];
+      for (int q = i + 1; q < k; ++q) t -= H[(size_t)q * (m + 1) + i] * y[q];
+      const zd d = H[(size_t)i * (m + 1) + i];
+      y[i] = d != zd(0) ? t / d : zd(0);
+    }
+    if (k > 0) {
+      HIP_TRY(hipMemcpyAsync(yv, y.data(), k * sizeof(zd), hipMemcpyHostToDevice, st));
+      TRY(gm_comb(s, V, k, yv, r, 1.0, 0.0));
+      TRY(prec_apply(s, r, z));
+      HIP_TRY(launch_axpy<double>(x, n, z, n, n, 1, 1.0, st));
+      HIP_TRY(hipStreamSynchronize(st));   // y leaves scope
+    }
+  }
+  if (info) {
+    info->iterations = its;
+    info->converged = conv ? 1 : 0;
+    info->rel_correction = berr;
+  }
+  return KORE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -961,43 +1129,78 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     if (!b_indices || !b_data) return fail(KORE_ERR_ARG, "B arrays incomplete");
     TRY(check_csr(b_indptr, b_indices, "B"));
   }
-  // union pattern in degree-major order
-  s->h_ip.assign(n + 1, 0);
-  std::vector<std::pair<int32_t, std::pair<zd, zd>>> row;
-  s->h_ix.reserve((size_t)a_indptr[n] + (s->has_b ? b_indptr[n] : 0));
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t r = s->perm[i];
-    row.clear();
-    for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
-      row.push_back({(int32_t)inv[a_indices[k]], {zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)}});
-    if (s->has_b)
-      for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) {
-        const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
-        row.push_back({(int32_t)inv[b_indices[k]], {zd(0, 0), bv}});
+  // union pattern in degree-major order, built by row ranges on host threads
+  // (each range into its own arrays, then concatenated in row order)
+  struct Ent {
+    int32_t col;
+    zd a, b;
+  };
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                               std::max<int64_t>(1, n / 4096)}));
+  std::vector<std::vector<int32_t>> tix(nth);
+  std::vector<std::vector<zd>> tva(nth), tvb(nth);
+  std::vector<int32_t> rowcnt(n, 0);
+  std::vector<int64_t> bad_row(nth, -1);
+  std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
+  auto build_rows = [&](int t) {
+    const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
+    std::vector<Ent> row;
+    const int64_t est = (int64_t)a_indptr[n] / nth + 1024;
+    tix[t].reserve(est);
+    tva[t].reserve(est);
+    tvb[t].reserve(est);
+    for (int64_t i = r0; i < r1; ++i) {
+      const int64_t r = s->perm[i];
+      row.clear();
+      for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
+        row.push_back({(int32_t)inv[a_indices[k]], zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)});
+      if (s->has_b)
+        for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) {
+          const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
+          row.push_back({(int32_t)inv[b_indices[k]], zd(0, 0), bv});
+        }
+      auto lt = [](const Ent& x, const Ent& y) { return x.col < y.col; };
+      if (!std::is_sorted(row.begin(), row.end(), lt)) std::stable_sort(row.begin(), row.end(), lt);
+      const int kb = s->h_blk[i];
+      int32_t cnt = 0;
+      for (size_t q = 0; q < row.size();) {
+        const int32_t col = row[q].col;
+        zd a = 0, b = 0;
+        while (q < row.size() && row[q].col == col) {
+          a += row[q].a;
+          b += row[q].b;
+          ++q;
+        }
+        if (a == zd(0, 0) && b == zd(0, 0)) continue;
+        const int kc = s->h_blk[col];
+        if ((kc < kb - 1 || kc > kb + 1) && bad_row[t] < 0) {
+          bad_row[t] = i;
+          bad_kb[t] = kb;
+          bad_kc[t] = kc;
+        }
+        tix[t].push_back(col);
+        tva[t].push_back(a);
+        tvb[t].push_back(b);
+        ++cnt;
       }
-    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-    const int kb = s->h_blk[i];
-    for (size_t q = 0; q < row.size();) {
-      int32_t col = row[q].first;
-      zd a = 0, b = 0;
-      while (q < row.size() && row[q].first == col) {
-        a += row[q].second.first;
-        b += row[q].second.second;
-        ++q;
-      }
-      if (a == zd(0, 0) && b == zd(0, 0)) continue;
-      const int kc = s->h_blk[col];
-      if (kc < kb - 1 || kc > kb + 1)
-        return fail(KORE_ERR_UNSUPPORTED,
-                    "matrix is not block-tridiagonal in degree order (row %lld couples block %d to %d)",
-                    (long long)i, kb, kc);
-      s->h_ix.push_back(col);
-      s->h_va.push_back(a);
-      s->h_vb.push_back(b);
+      rowcnt[i] = cnt;
     }
-    s->h_ip[i + 1] = (int32_t)s->h_ix.size();
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(build_rows, t);
+    build_rows(0);
+    for (auto& th : pool) th.join();
   }
-  const size_t nnz = s->h_ix.size();
+  for (int t = 0; t < nth; ++t)
+    if (bad_row[t] >= 0)
+      return fail(KORE_ERR_UNSUPPORTED,
+                  "matrix is not block-tridiagonal in degree order (row %lld couples block %d to %d)",
+                  (long long)bad_row[t], bad_kb[t], bad_kc[t]);
+  s->h_ip.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) s->h_ip[i + 1] = s->h_ip[i] + rowcnt[i];
+  const size_t nnz = (size_t)s->h_ip[n];
+  s->nnz = (int64_t)nnz;
   TRY(s->d_ip.alloc(ctx, (n + 1) * sizeof(int32_t)));
   TRY(s->d_ix.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(int32_t)));
   TRY(s->d_va.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(zd)));
@@ -1010,10 +1213,19 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->d_part.alloc(ctx, 2 * (size_t)s->npart * MAXR * sizeof(double)));
   hipStream_t st = ctx->stream;
   HIP_TRY(hipMemcpyAsync(s->d_ip.p, s->h_ip.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  if (nnz) {
-    HIP_TRY(hipMemcpyAsync(s->d_ix.p, s->h_ix.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(s->d_va.p, s->h_va.data(), nnz * sizeof(zd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(s->d_vb.p, s->h_vb.data(), nnz * sizeof(zd), hipMemcpyHostToDevice, st));
+  // each row range goes straight from its builder's arrays to its place on the device
+  {
+    size_t off = 0;
+    for (int t = 0; t < nth; ++t) {
+      const size_t cnt = tix[t].size();
+      if (cnt) {
+        HIP_TRY(hipMemcpyAsync(s->d_ix.as<int32_t>() + off, tix[t].data(), cnt * sizeof(int32_t),
+                               hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->d_va.as<zd>() + off, tva[t].data(), cnt * sizeof(zd), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->d_vb.as<zd>() + off, tvb[t].data(), cnt * sizeof(zd), hipMemcpyHostToDevice, st));
+      }
+      off += cnt;
+    }
   }
   HIP_TRY(hipMemcpyAsync(s->d_perm.p, s->perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_blk.p, s->h_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1040,9 +1252,17 @@ void kore_sys_destroy(kore_sys* sys) {
 
 int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
   if (!s || !out) return fail(KORE_ERR_ARG, "NULL argument");
-  const zd sg(sre, sim);
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int nblk = std::min(2 * s->npart * MAXR, 1024);
+  if (s->nnz > 0) HIP_TRY(launch_csr_fro2(s->csr.va, s->csr.vb, s->nnz, sre, sim, s->part, nblk, c->stream));
+  std::vector<double> h(nblk, 0.0);
+  if (s->nnz > 0) {
+    HIP_TRY(hipMemcpyAsync(h.data(), s->part, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   double acc = 0;
-  for (size_t k = 0; k < s->h_va.size(); ++k) acc += std::norm(s->h_va[k] - sg * s->h_vb[k]);
+  for (double v : h) acc += v;
   *out = std::sqrt(acc);
   return KORE_OK;
 }
@@ -1168,6 +1388,63 @@ int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, doub
     }
   }
   return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
+}
+
+int32_t kore_residual(kore_sys* s, int32_t nrhs, const double* x, int64_t ldx, const double* lam, const double* b,
+                      int64_t ldb, double sre, double sim, double* rnorm) {
+  if (!s || !x || !rnorm || nrhs < 0 || ldx < s->n || (b && ldb < s->n))
+    return fail(KORE_ERR_ARG, "bad residual arguments");
+  if (!s->has_b && (lam || sre != 0.0 || sim != 0.0)) return fail(KORE_ERR_STATE, "system has no B matrix");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int64_t n = s->n;
+  for (int c0 = 0; c0 < nrhs; c0 += MAXR) {
+    const int nr = std::min(MAXR, nrhs - c0);
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(s->st + (size_t)j * n, x + 2 * (ldx * (int64_t)(c0 + j)), n * sizeof(zd),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_gather_scale<double, double>(s->st, n, s->bb, n, n, nr, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                                c->stream));
+    {
+      Timed tm(c, KORE_TIMER_SPMV);
+      for (int j = 0; j < nr; ++j) {
+        const double lre = lam ? lam[2 * (c0 + j)] : 0.0, lim = lam ? lam[2 * (c0 + j) + 1] : 0.0;
+        HIP_TRY(launch_spmv(s->csr, 1.0, 0.0, -(sre + lre), -(sim + lim), s->bb + (size_t)j * n, n,
+                            s->rr + (size_t)j * n, n, 1, 0.0, c->stream));
+      }
+    }
+    if (b) {
+      for (int j = 0; j < nr; ++j)
+        HIP_TRY(hipMemcpyAsync(s->st + (size_t)j * n, b + 2 * (ldb * (int64_t)(c0 + j)), n * sizeof(zd),
+                               hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(launch_gather_scale<double, double>(s->st, n, s->bb, n, n, nr, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                                  c->stream));
+      HIP_TRY(launch_axpy<double>(s->rr, n, s->bb, n, n, nr, -1.0, c->stream));
+    }
+    TRY(colnorms2<double, double>(s, s->rr, n, nullptr, 0, nr, rnorm + c0, nullptr));
+  }
+  return KORE_OK;
+}
+
+int32_t kore_solve_gmres(kore_sys* s, const double* b, double* x, double tol, int32_t maxit, int32_t restart,
+                         kore_solve_info* info) {
+  if (!s || !b || !x || maxit < 1) return fail(KORE_ERR_ARG, "bad GMRES arguments");
+  if (!s->f || !s->factored) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int64_t n = s->n;
+  double nA = 0;
+  TRY(kore_sys_norm(s, s->sre, s->sim, &nA));
+  HIP_TRY(hipMemcpyAsync(s->st, b, n * sizeof(zd), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_gather_scale<double, double>(s->st, n, s->bb, n, n, 1, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                              c->stream));
+  kore_solve_info li{};
+  TRY(gmres_dev(s, s->bb, s->xr, tol, maxit, restart, nA, &li));
+  HIP_TRY(launch_scatter(s->xr, n, s->st, n, n, 1, s->d_perm.as<int64_t>(), c->stream));
+  HIP_TRY(hipMemcpyAsync(x, s->st, n * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (info) *info = li;
+  return li.converged ? KORE_OK : KORE_WARN_IR_NOT_CONVERGED;
 }
 
 // ---- workspace ---------------------------------------------------------------

@@ -6,6 +6,11 @@ with a seeded unit-norm complex right-hand side) or an eigensolve with target
 sigma (nev pairs nearest sigma).  Both factor A0 - sigma B0 (sigma = i w for
 forced solves), so they share one kernel set.
 
+Precision policy of the mixed path: forced solves factor in complex64 and
+solve by FP64 GMRES preconditioned with that factor (kore_solve_gmres), with
+an FP64 refactorisation if GMRES does not reach the tolerance; eigensolves,
+which are bound by their many solve sweeps, factor in FP64 from the start.
+
 Pipeline:
 
 * CPU assembly workers build (A0, B0) for queued requests ahead of the GPU;
@@ -16,9 +21,10 @@ Pipeline:
   run concurrently and their kernels overlap on the device; a finished
   request frees its slot and memory at once and the next admissible request
   starts (continuous batching, not a FIFO of whole batches);
-* results are checked (forced: ||A x - b|| / (||A||_F ||x|| + ||b||);
-  eigen: ||(A - lam B) x|| / (||A||_F ||x||)), and latency is measured
-  from arrival to result on the host.
+* results are checked on the device from the resident matrices (forced:
+  ||A x - b|| / (||A||_F ||x|| + ||b||); eigen: ||(A - lam B) x|| /
+  (||A||_F ||x||)), and latency is measured from arrival to result on the
+  host.
 """
 from __future__ import annotations
 
@@ -71,6 +77,7 @@ class Result:
     precision: int = 0
     ir_sweeps: int = 0
     error: str = ""
+    stages: dict = dataclasses.field(default_factory=dict)   # seconds: setup, factor, solve, check
 
 
 def serving_trace(nreq=1024, seed=1234, e_range=(1e-7, 1e-5), n_cap=500, forced_frac=0.7, rate=None, nev=6,
@@ -121,6 +128,9 @@ def assemble_request(req: Request):
 
 
 def residual(A0, B0, req: Request, x=None, lam=None, X=None, b=None):
+    """Host check (reference formula; used by tests):
+    forced ||A x - b|| / (||A||_F ||x|| + ||b||) with A = A0 - sigma B0;
+    eigen max_c ||(A0 - lam_c B0) x_c|| / (||A0||_F ||x_c||)."""
     nA = np.sqrt(np.sum(np.abs((A0 - req.sigma * B0).data) ** 2)) if req.kind == "forced" else \
         np.sqrt(np.sum(np.abs(A0.data) ** 2))
     if req.kind == "forced":
@@ -130,11 +140,26 @@ def residual(A0, B0, req: Request, x=None, lam=None, X=None, b=None):
     return float(eigs.residuals(A0, B0, lam, X).max()) if len(lam) else float("inf")
 
 
+def residual_device(S, req: Request, x=None, lam=None, X=None, b=None):
+    """The same check as ``residual`` from the matrices resident on the device
+    (kore_residual, kore_sys_norm): the host only forms vector norms."""
+    if req.kind == "forced":
+        nA = S.norm(req.sigma)
+        r = S.residual_norms(x, b=b, shift=req.sigma)[0]
+        return float(r / (nA * np.linalg.norm(x) + np.linalg.norm(b)))
+    if len(lam) == 0:
+        return float("inf")
+    nA = S.norm(0.0)
+    r = S.residual_norms(X, lam=lam)
+    return float(np.max(r / (nA * np.linalg.norm(X, axis=0))))
+
+
 class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
     def __init__(self, device=0, slots=4, mem_budget=200 << 30, precision=backend.PREC_C64, block=4,
-                 assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None):
+                 assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
+                 gmres_tol=1e-15, gmres_maxit=60):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -142,6 +167,8 @@ class Server:
         self.block = block
         self.tol = tol
         self.ir_tol = ir_tol
+        self.gmres_tol = gmres_tol
+        self.gmres_maxit = gmres_maxit
         self.keep_results = keep_results
         self.log = log
         self.ctxs = [backend.Context(device) for _ in range(slots)]
@@ -164,24 +191,45 @@ class Server:
     def _execute(self, slot, req: Request, par, A0, B0, t0) -> Result:
         ctx = self.ctxs[slot]
         t_start = time.perf_counter()
+        st = {}
         lay = backend.layout_from_params(par)
         S = backend.System(ctx, lay, A0, B0)
         try:
-            S.factor(req.sigma, self.precision)
+            t1 = time.perf_counter()
+            st["setup"] = t1 - t_start
+            # shift-and-invert eigensolves are bound by the solve sweeps, which a
+            # complex64 factor cannot shorten (each operator application then
+            # needs refinement sweeps); they take the FP64 factor directly
+            prec = self.precision if req.kind == "forced" else backend.PREC_FP64
+            S.factor(req.sigma, prec)
+            t2 = time.perf_counter()
+            st["factor"] = t2 - t1
             if req.kind == "forced":
                 b = rhs_for(req, S.n)
-                x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=20 if self.precision else 0)
+                if self.precision == backend.PREC_FP64:
+                    x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=0)
+                else:
+                    # complex64 factor as the preconditioner of FP64 GMRES
+                    x, info = S.solve_gmres(b, tol=self.gmres_tol, maxit=self.gmres_maxit)
                 if self.precision != backend.PREC_FP64 and not info.converged:
                     # refinement stalled (low Ekman number): FP64 refactorisation
+                    t3 = time.perf_counter()
                     S.factor(req.sigma, backend.PREC_FP64)
+                    st["factor"] += time.perf_counter() - t3
                     x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=0)
-                res = residual(A0, B0, req, x=x, b=b)
+                t4 = time.perf_counter()
+                st["solve"] = t4 - t2 - (st["factor"] - (t2 - t1))
+                res = residual_device(S, req, x=x, b=b)
+                st["check"] = time.perf_counter() - t4
                 out = Result(req.rid, req.kind, 0.0, 0.0, res, res <= 1e-10, x=x if self.keep_results else None,
                              precision=S.precision, ir_sweeps=info.iterations)
             else:
                 r = eigs.eigs(S, req.sigma, nev=req.nev, tol=self.tol, block=self.block, ir_tol=self.ir_tol)
+                t4 = time.perf_counter()
+                st["solve"] = t4 - t2
                 k = min(req.nev, r.nconv)
-                res = residual(A0, B0, req, lam=r.lam[:k], X=r.X[:, :k])
+                res = residual_device(S, req, lam=r.lam[:k], X=r.X[:, :k])
+                st["check"] = time.perf_counter() - t4
                 out = Result(req.rid, req.kind, 0.0, 0.0, res, (res <= 1e-10) and r.nconv >= req.nev,
                              lam=r.lam[:k], x=r.X[:, :k] if self.keep_results else None,
                              precision=r.precision, ir_sweeps=r.ir_max_iterations)
@@ -190,6 +238,7 @@ class Server:
         now = time.perf_counter()
         out.service = now - t_start
         out.latency = now - (t0 + req.arrival)
+        out.stages = st
         return out
 
     def run(self, trace: List[Request]) -> List[Result]:
@@ -226,7 +275,9 @@ class Server:
             if dt > 0:
                 time.sleep(dt)
             par, A0, B0 = futs[req.rid].result()
-            need = footprint(par, self.precision, req.nev, self.block)
+            # FP64 bytes even on the mixed path: eigensolves and refinement
+            # fallbacks hold an FP64 factor
+            need = footprint(par, backend.PREC_FP64, req.nev, self.block)
             if need > self.mem_budget:
                 results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
                                           error="request needs %d bytes, budget %d" % (need, self.mem_budget))
@@ -251,7 +302,14 @@ class Server:
 def summarize(results: List[Result], wall: float):
     lat = np.array([r.latency for r in results])
     ok = np.array([r.ok for r in results])
+    stages = {}
+    for r in results:
+        for k, v in r.stages.items():
+            stages[k] = stages.get(k, 0.0) + v / len(results)
     return dict(requests=len(results), ok=int(ok.sum()), solves_per_s=len(results) / wall,
                 p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_service_s=float(np.mean([r.service for r in results])),
+                mean_stage_s=stages,
+                fp64_results=int(sum(r.precision == backend.PREC_FP64 for r in results)),
+                mean_ir_sweeps=float(np.mean([r.ir_sweeps for r in results])),
                 worst_residual=float(max(r.residual for r in results)))

@@ -25,7 +25,8 @@ Contract kept from the reference (bin/solve.py):
   (solve.py:36-37, 308-311); the exit code is always 0 (solve.py:314).
 
 Extra options of this backend: -kore_precision fp64|c64 (c64: complex64
-factor + FP64 refinement), -kore_block <bs> (Krylov block size),
+factor + FP64 refinement; forced problems by FP64 GMRES preconditioned with
+the complex64 factor), -kore_block <bs> (Krylov block size),
 -kore_device <id>, -kore_params <path to parameters.py>.
 Only rank 0 of an MPI launch works; other ranks exit at once.
 """
@@ -185,7 +186,14 @@ def main(argv=None):
         b = np.asarray(b0[:, 0].toarray()).ravel().astype(complex)
         S = backend.System(ctx, lay, A, None)
         S.factor(0.0, prec)
-        x, info = S.solve(b, ir_tol=1e-15, ir_maxit=20 if prec != backend.PREC_FP64 else 1)
+        if prec == backend.PREC_FP64:
+            x, info = S.solve(b, ir_tol=1e-15, ir_maxit=1)
+        else:
+            # complex64 factor as the preconditioner of FP64 GMRES; FP64 refactor if it stalls
+            x, info = S.solve_gmres(b, tol=1e-15, maxit=60)
+            if not info.converged:
+                S.factor(0.0, backend.PREC_FP64)
+                x, info = S.solve(b, ir_tol=1e-15, ir_maxit=1)
         S.close()
         fields = split_fields(x.reshape(-1, 1), par)
         flow = fields.get("flow", x.reshape(-1, 1))

@@ -201,3 +201,33 @@ def test_expand_breakdown_recovers(ctx, config1):
     for lo in lam_o[:4]:
         assert np.min(np.abs(r.lam - lo)) / abs(lo) <= 1e-10
     S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_gmres(ctx, precision):
+    """GMRES preconditioned by the factor reaches FP64 accuracy with a
+    complex64 factor on a low-Ekman case (E=1e-6): normwise backward error
+    <= 1e-15 (the refinement tolerance).  The forward error against SuperLU
+    is then bounded by cond(A - sigma B) * 1e-15 (cond ~ 2e5 here): <= 1e-9
+    for the complex64 factor, <= 1e-10 for the FP64 factor."""
+    from kore_amd import assembly
+    from kore_amd.params import Params
+    par = Params(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=48, lmax=61)
+    A0, B0 = assembly.assemble(par, which="unscaled")
+    sigma = 0.7j
+    S = _system(ctx, par, A0, B0)
+    S.factor(sigma, precision)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(S.n) + 1j * rng.standard_normal(S.n)
+    say("case: gmres precision", precision, "n", S.n)
+    x, info = S.solve_gmres(b, tol=1e-15, maxit=60)
+    say("  iterations", info.iterations, "converged", info.converged, "backward error", info.rel_correction)
+    assert info.converged and info.rel_correction <= 1e-15
+    M = (A0 - sigma * B0).tocsc()
+    xr = sla.splu(M).solve(b)
+    fwd = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    say("  forward error vs SuperLU", fwd)
+    assert fwd <= (1e-10 if precision == 0 else 1e-9)
+    if precision == 0:
+        assert info.iterations <= 3
+    S.close()

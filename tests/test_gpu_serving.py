@@ -58,3 +58,30 @@ def test_oversized_request_is_refused_not_crashed():
     out = srv.run(trace)
     assert all((not r.ok) and "budget" in r.error for r in out)
     srv.close()
+
+
+def test_device_residual_matches_host():
+    """kore_residual + kore_sys_norm give the host formula's residuals (forced
+    and eigen) for perturbed solutions (residuals well above rounding)."""
+    trace = small_trace()
+    rng = np.random.default_rng(3)
+    ctx = backend.Context(0)
+    for kind in ("forced", "eigen"):
+        r = next(q for q in trace if q.kind == kind)
+        print("case: device residual", kind, r.N, r.m, flush=True)
+        par, A0, B0 = serving.assemble_request(r)
+        S = backend.System(ctx, backend.layout_from_params(par), A0, B0)
+        n = S.n
+        if kind == "forced":
+            b = serving.rhs_for(r, n)
+            x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            host = serving.residual(A0, B0, r, x=x, b=b)
+            dev = serving.residual_device(S, r, x=x, b=b)
+        else:
+            X = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+            lam = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+            host = serving.residual(A0, B0, r, lam=lam, X=X)
+            dev = serving.residual_device(S, r, lam=lam, X=X)
+        assert abs(dev - host) <= 1e-10 * host, (kind, dev, host)
+        S.close()
+    ctx.close()
