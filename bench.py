@@ -105,7 +105,13 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
                 host_ms_per_step={k: 1e3 * v / steps for k, v in host.items()})
 
 
-def roofline(run, precision):
+# HBM bytes per solve-sweep dispatch measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE (scaled by a plain 1 GiB copy) on the default workload (FP64,
+# block 4); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
+MEASURED_SWEEP_TRAFFIC = {("config2", 0, 4): 58.0e6}
+
+
+def roofline(run, precision, workload=None, block=None):
     t = run["timing"]
     steps = run["timing_steps"]
     f = run["finfo"]
@@ -132,7 +138,7 @@ def roofline(run, precision):
     dom = max(cand, key=lambda k: cand[k][0])
     r = dict(cand[dom][1])
     r["frac"] = r["achieved"] / r["peak"]
-    r["traffic"] = None
+    r["traffic"] = MEASURED_SWEEP_TRAFFIC.get((workload, precision, block)) if dom == "skinny" else None
     classes = {k: dict(launches_per_step=v[0] / steps, device_ms_per_step=v[1] / steps) for k, v in t.items()}
     return r, classes
 
@@ -245,6 +251,19 @@ def run_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings (KiB per dispatch) against nbytes give the scale that turns the
+    counters of the solver kernels into HBM bytes."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}))
+
+
 def dataclasses_replace(obj, **kw):
     import dataclasses
     return dataclasses.replace(obj, **kw)
@@ -265,7 +284,12 @@ def main():
     ap.add_argument("--ncv", type=int, default=None)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy, three times (scale for HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -289,7 +313,7 @@ def main():
         r = run_path(S, ctx, wl, prec, args.steps, args.warmup, args.block, args.ncv)
         res = r["res"]
         resid = eigs.residuals(A, B, res.lam[:wl["nev"]], res.X[:, :wl["nev"]])
-        rl, classes = roofline(r, prec)
+        rl, classes = roofline(r, prec, args.workload, args.block)
         paths[name] = dict(
             solves_per_s=args.steps / r["total"], ms_per_step=1e3 * r["total"] / args.steps,
             p50_ms=1e3 * pct(r["lat"], 50), p99_ms=1e3 * pct(r["lat"], 99),

@@ -8,6 +8,8 @@
 // degree-major order, the reduction plan (descriptor lists uploaded once
 // per factorisation) and the row equilibration.
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <atomic>
 #include <cmath>
 #include <complex>
@@ -60,6 +62,20 @@ static int32_t fail(int32_t code, const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// Host-side stage timer: prints "kore_amd: <stage> <ms>" to stderr when the
+// environment variable KORE_AMD_TRACE is set (read once).
+struct StageTrace {
+  bool on;
+  std::chrono::steady_clock::time_point t0;
+  explicit StageTrace() : on(std::getenv("KORE_AMD_TRACE") != nullptr), t0(std::chrono::steady_clock::now()) {}
+  void mark(const char* what) {
+    if (!on) return;
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "kore_amd: %s %.2f ms\n", what, std::chrono::duration<double, std::milli>(t - t0).count());
+    t0 = t;
+  }
+};
+
 struct TimerRec {
   int cls;
   hipEvent_t a, b;
@@ -75,7 +91,25 @@ struct kore_ctx {
   std::vector<hipEvent_t> pool;
   double tot_ms[KORE_TIMER_COUNT] = {0};
   int64_t cnt[KORE_TIMER_COUNT] = {0};
+  // pinned host arena for building and uploading system matrices: kept for
+  // the life of the context (pre-faulted, DMA-able), grown on demand
+  char* pin = nullptr;
+  size_t pin_cap = 0;
 };
+
+static int32_t ctx_pinned(kore_ctx* c, size_t bytes, char** out) {
+  if (bytes > c->pin_cap) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->pin) HIP_TRY(hipHostFree(c->pin));
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = bytes + bytes / 4;
+    HIP_TRY(hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
+    c->pin_cap = cap;
+  }
+  *out = c->pin;
+  return KORE_OK;
+}
 
 static hipEvent_t get_event(kore_ctx* c) {
   if (!c->pool.empty()) {
@@ -1029,6 +1063,7 @@ void kore_ctx_destroy(kore_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   resolve_timers(ctx);
   for (auto e : ctx->pool) hipEventDestroy(e);
+  if (ctx->pin) hipHostFree(ctx->pin);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1099,6 +1134,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   s->ctx = ctx;
   s->lay = *lay;
   int64_t n = 0;
+  StageTrace trace;
   TRY(layout_build(lay, &n, &s->bsz, &s->perm));
   if (n != sizmat) return fail(KORE_ERR_ARG, "matrix size %lld does not match the layout (%lld)", (long long)sizmat, (long long)n);
   s->n = n;
@@ -1123,6 +1159,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
       if (ix[k] < 0 || ix[k] >= n) return fail(KORE_ERR_ARG, "%s: column index out of range", nm);
     return KORE_OK;
   };
+  trace.mark("sys_create layout");
   TRY(check_csr(a_indptr, a_indices, "A"));
   s->has_b = b_indptr != nullptr;
   if (s->has_b) {
@@ -1137,18 +1174,36 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   };
   const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
                                                                std::max<int64_t>(1, n / 4096)}));
-  std::vector<std::vector<int32_t>> tix(nth);
-  std::vector<std::vector<zd>> tva(nth), tvb(nth);
+  // per-range upper bounds (entries of A plus B on the range's rows) place
+  // each range's output in the pinned arena before anything is built
+  std::vector<size_t> ub(nth + 1, 0);
+  for (int t = 0; t < nth; ++t) {
+    const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
+    size_t u = 0;
+    for (int64_t i = r0; i < r1; ++i) {
+      const int64_t r = s->perm[i];
+      u += (size_t)(a_indptr[r + 1] - a_indptr[r]);
+      if (s->has_b) u += (size_t)(b_indptr[r + 1] - b_indptr[r]);
+    }
+    ub[t + 1] = ub[t] + u;
+  }
+  const size_t cap = std::max<size_t>(ub[nth], 1);
+  char* arena = nullptr;
+  TRY(ctx_pinned(ctx, cap * (sizeof(int32_t) + 2 * sizeof(zd)) + 256, &arena));
+  zd* pva = reinterpret_cast<zd*>(arena);
+  zd* pvb = pva + cap;
+  int32_t* pix = reinterpret_cast<int32_t*>(pvb + cap);
+  std::vector<size_t> tcnt(nth, 0);
   std::vector<int32_t> rowcnt(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
   auto build_rows = [&](int t) {
     const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
     std::vector<Ent> row;
-    const int64_t est = (int64_t)a_indptr[n] / nth + 1024;
-    tix[t].reserve(est);
-    tva[t].reserve(est);
-    tvb[t].reserve(est);
+    int32_t* oix = pix + ub[t];
+    zd* ova = pva + ub[t];
+    zd* ovb = pvb + ub[t];
+    size_t cntt = 0;
     for (int64_t i = r0; i < r1; ++i) {
       const int64_t r = s->perm[i];
       row.clear();
@@ -1159,8 +1214,18 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
           const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
           row.push_back({(int32_t)inv[b_indices[k]], zd(0, 0), bv});
         }
-      auto lt = [](const Ent& x, const Ent& y) { return x.col < y.col; };
-      if (!std::is_sorted(row.begin(), row.end(), lt)) std::stable_sort(row.begin(), row.end(), lt);
+      // rows hold a few dozen entries in a few sorted runs: a stable insertion
+      // sort, no allocation per row
+      for (size_t q = 1; q < row.size(); ++q) {
+        if (row[q].col >= row[q - 1].col) continue;
+        const Ent e = row[q];
+        size_t p = q;
+        while (p > 0 && row[p - 1].col > e.col) {
+          row[p] = row[p - 1];
+          --p;
+        }
+        row[p] = e;
+      }
       const int kb = s->h_blk[i];
       int32_t cnt = 0;
       for (size_t q = 0; q < row.size();) {
@@ -1178,20 +1243,24 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
           bad_kb[t] = kb;
           bad_kc[t] = kc;
         }
-        tix[t].push_back(col);
-        tva[t].push_back(a);
-        tvb[t].push_back(b);
+        oix[cntt] = col;
+        ova[cntt] = a;
+        ovb[cntt] = b;
+        ++cntt;
         ++cnt;
       }
       rowcnt[i] = cnt;
     }
+    tcnt[t] = cntt;
   };
   {
     std::vector<std::thread> pool;
     for (int t = 1; t < nth; ++t) pool.emplace_back(build_rows, t);
+    trace.mark("sys_create checks");
     build_rows(0);
     for (auto& th : pool) th.join();
   }
+  trace.mark("sys_create union pattern");
   for (int t = 0; t < nth; ++t)
     if (bad_row[t] >= 0)
       return fail(KORE_ERR_UNSUPPORTED,
@@ -1212,17 +1281,18 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->d_scr.alloc(ctx, 4 * (size_t)n * MAXR * sizeof(cx<double>)));
   TRY(s->d_part.alloc(ctx, 2 * (size_t)s->npart * MAXR * sizeof(double)));
   hipStream_t st = ctx->stream;
+  trace.mark("sys_create device alloc");
   HIP_TRY(hipMemcpyAsync(s->d_ip.p, s->h_ip.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  // each row range goes straight from its builder's arrays to its place on the device
+  // each row range goes from its place in the pinned arena to its place on the device
   {
     size_t off = 0;
     for (int t = 0; t < nth; ++t) {
-      const size_t cnt = tix[t].size();
+      const size_t cnt = tcnt[t];
       if (cnt) {
-        HIP_TRY(hipMemcpyAsync(s->d_ix.as<int32_t>() + off, tix[t].data(), cnt * sizeof(int32_t),
+        HIP_TRY(hipMemcpyAsync(s->d_ix.as<int32_t>() + off, pix + ub[t], cnt * sizeof(int32_t),
                                hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(s->d_va.as<zd>() + off, tva[t].data(), cnt * sizeof(zd), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(s->d_vb.as<zd>() + off, tvb[t].data(), cnt * sizeof(zd), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->d_va.as<zd>() + off, pva + ub[t], cnt * sizeof(zd), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->d_vb.as<zd>() + off, pvb + ub[t], cnt * sizeof(zd), hipMemcpyHostToDevice, st));
       }
       off += cnt;
     }
@@ -1239,6 +1309,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   s->st = scr + 3 * (size_t)n * MAXR;
   s->part = s->d_part.as<double>();
   TRY(s->up.init(ctx, 1 << 20));
+  trace.mark("sys_create upload");
   *out = s.release();
   return KORE_OK;
 }
