@@ -1,7 +1,8 @@
 """Assembly of Kore's sparse A and B (the input side of the solve path).
 
-Restates, for the Boussinesq hydrodynamic case with an inner core, what the
-reference computes in three stages:
+Restates, for the hydrodynamic case with an inner core and optionally the
+Boussinesq heat equation with buoyancy (differential or internal heating),
+what the reference computes in three stages:
 
 1. radial operators ``r^p d^d/dr^d`` in the Gegenbauer basis of each equation
    (bin/submatrices.py:138-188 and 533-590);
@@ -296,10 +297,69 @@ def v_off_coefs(l, par: Params, side: int):
     return np.stack([-2 * G * C * (l + 2), -2 * G * C], axis=-1).astype(complex)
 
 
+# Heat equation (section h, C^(2) basis, 2 boundary rows) and buoyancy, the
+# Boussinesq case (bin/operators.py:386-406, 699-775).
+H_OPS = [(0, 0), (1, 0), (2, 0), (3, 0), (1, 1), (2, 1), (2, 2), (3, 2)]   # diagonal, section h
+H_ADV = [(0, 0), (2, 0)]                                                     # h <- upol(l)
+U_BUOY = [(4, 0)]                                                            # u <- theta(l)
+
+
+def _heating(par: Params):
+    if par.heating not in ("differential", "internal"):
+        raise NotImplementedError("heating %r is not implemented" % par.heating)
+    return par.heating == "differential"
+
+
+def h_term_theta(l, par: Params):
+    """theta(l, 'h'): r3D0 (differential heating, equation times r^3) or r2D0
+    (internal heating)   (bin/operators.py:699-718)."""
+    l, L = _L(l)
+    z = 0 * L
+    one = 1.0 + z
+    if _heating(par):
+        return np.stack([z, z, z, one, z, z, z, z], axis=-1).astype(complex)
+    return np.stack([z, z, one, z, z, z, z, z], axis=-1).astype(complex)
+
+
+def h_term_diffusion(l, par: Params):
+    """thermal_diffusion(l, 'h'): ThermaD (-L r1D0 + 2 r2D1 + r3D2) for
+    differential heating, ThermaD (-L r0D0 + 2 r1D1 + r2D2) otherwise
+    (bin/operators.py:746-775)."""
+    l, L = _L(l)
+    z = 0 * L
+    K = par.ThermaD
+    if _heating(par):
+        return np.stack([z, -K * L, z, z, z, 2 * K + z, z, K + z], axis=-1).astype(complex)
+    return np.stack([-K * L, z, z, z, 2 * K + z, z, K + z, z], axis=-1).astype(complex)
+
+
+def h_diag_coefs(l, par: Params, wf: float):
+    """A block of the heat equation: diffusion - i w theta (bin/assemble.py:1043-1046)."""
+    return h_term_diffusion(l, par) - 1j * wf * h_term_theta(l, par)
+
+
+def h_adv_coefs(l, par: Params):
+    """thermal_advection(l, 'h', 'upol'): L r0D0 ricb/(rcmb - ricb) (differential)
+    or L r2D0 (internal), over H_ADV   (bin/operators.py:721-743)."""
+    l, L = _L(l)
+    z = 0 * L
+    if _heating(par):
+        return np.stack([L * par.ricb / (1.0 - par.ricb), z], axis=-1).astype(complex)
+    return np.stack([z, L], axis=-1).astype(complex)
+
+
+def u_buoy_coefs(l, par: Params):
+    """buoyancy(l, 'u'): Beyonce L r4D0   (bin/operators.py:386-406)."""
+    l, L = _L(l)
+    return (par.Beyonce * L)[..., None].astype(complex)
+
+
 def degree_block(par: Params, R: "RadialOps", term: str, l: int):
     """One per-degree operator block, by term name:
     'u:u', 'cori:u', 'visc:u', 'cori:u:-1', 'cori:u:+1', 'u:v', 'cori:v', 'visc:v',
-    'cori:v:-1', 'cori:v:+1'.  Mirrors the functions of bin/operators.py:22-194."""
+    'cori:v:-1', 'cori:v:+1', and for the heat equation 'buoy:u', 'theta:h',
+    'tadv:h', 'tdiff:h'.  Mirrors the functions of bin/operators.py:22-194,
+    386-406 and 699-775."""
     tabs = {
         "u:u": (U_OPS, 4, lambda: u_term_u([l])),
         "cori:u": (U_OPS, 4, lambda: u_term_coriolis([l], par)),
@@ -311,6 +371,10 @@ def degree_block(par: Params, R: "RadialOps", term: str, l: int):
         "visc:v": (V_OPS, 2, lambda: v_term_viscous([l], par)),
         "cori:v:-1": (V_OFF, 2, lambda: v_off_coefs([l], par, -1)),
         "cori:v:+1": (V_OFF, 2, lambda: v_off_coefs([l], par, +1)),
+        "buoy:u": (U_BUOY, 4, lambda: u_buoy_coefs([l], par)),
+        "theta:h": (H_OPS, 2, lambda: h_term_theta([l], par)),
+        "tadv:h": (H_ADV, 2, lambda: h_adv_coefs([l], par)),
+        "tdiff:h": (H_OPS, 2, lambda: h_term_diffusion([l], par)),
     }
     ops, g, coef = tabs[term]
     c = coef()[0]
@@ -367,9 +431,12 @@ def _bc_rows(par: Params, Tb, Ta, section: str, l):
             rows += [Ta[:, 0], Ta[:, 1]]
         else:
             rows += [Ta[:, 0], par.ricb * Ta[:, 2]]
-    else:
+    elif section == "v":
         rows += [Tb[:, 0] if par.bco == 1 else (-Tb[:, 1] + Tb[:, 0])]
         rows += [Ta[:, 0] if par.bci == 1 else (-par.ricb * Ta[:, 1] + Ta[:, 0])]
+    else:                                                # heat equation (bin/assemble.py:1300-1336)
+        rows += [Tb[:, 0] if par.bco_thermal == 0 else Tb[:, 1]]     # isothermal / fixed flux, CMB
+        rows += [Ta[:, 0] if par.bci_thermal == 0 else Ta[:, 1]]     # ICB
     return np.array(rows)
 
 
@@ -385,8 +452,10 @@ def assemble(par: Params, which: str = "eigen"):
                       A(omega) = A0 - i omega B0 (the identity of
                       bin/assemble.py:445/623 and 461/756).
     """
-    if not par.hydro or par.magnetic or par.thermal or par.compositional:
-        raise NotImplementedError("only the hydrodynamic assembly is implemented")
+    if not par.hydro or par.magnetic or par.compositional:
+        raise NotImplementedError("only the hydrodynamic and thermal (Boussinesq) assembly is implemented")
+    if par.thermal and par.ThermaD <= 0:
+        raise NotImplementedError("thermal assembly without thermal diffusion is not implemented")
     if par.anelastic:
         raise NotImplementedError("anelastic assembly is not implemented")
     if par.Ek == 0:
@@ -429,10 +498,21 @@ def assemble(par: Params, which: str = "eigen"):
             ls = bot[sel]
             cols = np.array([pos_top[int(l) + side] for l in ls]) * N1
             add(_triplets(pvo.blocks(v_off_coefs(ls, par, side)), pvo, rows_v[sel], cols))
+    secs = [("u", rows_u, top), ("v", rows_v, bot)]
+    if par.thermal:
+        # heat equation, degrees of the poloidal set (bin/assemble.py:1017-1048)
+        rows_h = 2 * n + kt * N1
+        ph = _Pattern([R.op(p, d, 2) for (p, d) in H_OPS])
+        pha = _Pattern([R.op(p, d, 2) for (p, d) in H_ADV])
+        pub = _Pattern([R.op(p, d, 4) for (p, d) in U_BUOY])
+        add(_triplets(ph.blocks(h_diag_coefs(top, par, wf)), ph, rows_h, rows_h))
+        add(_triplets(pha.blocks(h_adv_coefs(top, par)), pha, rows_h, rows_u))
+        add(_triplets(pub.blocks(u_buoy_coefs(top, par)), pub, rows_u, rows_h))
+        secs.append(("h", rows_h, top))
     # boundary-condition rows (top rows of every diagonal block)
     Tb = boundary_values(1.0, N, 5, par.ricb)
     Ta = boundary_values(-1.0, N, 4, par.ricb)
-    for sec, r0, ls in (("u", rows_u, top), ("v", rows_v, bot)):
+    for sec, r0, ls in secs:
         bc = _bc_rows(par, Tb, Ta, sec, None)            # (nrows, N1), same for every l
         nr = bc.shape[0]
         rr = (r0[:, None, None] + np.arange(nr)[None, :, None] + 0 * np.arange(N1)[None, None, :])
@@ -454,6 +534,9 @@ def assemble(par: Params, which: str = "eigen"):
     bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
     t = _triplets(pv.blocks(v_mass_coefs(bot)).real, pv, rows_v, rows_v)
     bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
+    if par.thermal:
+        t = _triplets(ph.blocks(h_term_theta(top, par)).real, ph, rows_h, rows_h)    # +theta (bin/assemble.py:505-515)
+        bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
     B = ss.csr_matrix((np.concatenate(bdat), (np.concatenate(bri), np.concatenate(bci))),
                       shape=(S, S), dtype=float)
     B.eliminate_zeros()

@@ -62,6 +62,8 @@ class Params:
     Ra_gap: float = 6e5
     bci_thermal: int = 0
     bco_thermal: int = 0
+    Beyonce: Optional[float] = None      # buoyancy factor; None -> -Ra Ek^2 / Prandtl
+    ThermaD: Optional[float] = None      # thermal diffusion factor; None -> Ek / Prandtl
     compositional: int = 0
     # time-unit switches (bin/parameters.py:273-278); None -> reference default
     Gaspard: float = 1.0
@@ -90,6 +92,12 @@ class Params:
             self.Em = self.Ek / self.Pm
         if self.Le2 is None:
             self.Le2 = self.Lambda * self.Em
+        if self.Beyonce is None:
+            # Ra = Ra_gap / (rcmb - ricb)^3, BV2 = -Ra Ek^2 / Prandtl (bin/parameters.py:177-181, 274)
+            ra = self.Ra_gap / (1.0 - self.ricb) ** 3
+            self.Beyonce = -ra * self.Ek ** 2 / self.Prandtl
+        if self.ThermaD is None:
+            self.ThermaD = self.Ek / self.Prandtl       # Etherm (bin/parameters.py:167, 277)
         self.tau = complex(self.tau)
         self.validate()
 
@@ -188,7 +196,7 @@ def load_parameters(path: str) -> Params:
         if hasattr(mod, k):
             kw[k] = getattr(mod, k)
     # the reference derives these at module level; keep its values verbatim
-    for k in ("Le2", "Em", "ViscosD"):
+    for k in ("Le2", "Em", "ViscosD", "Beyonce", "ThermaD"):
         if hasattr(mod, k):
             kw[k] = getattr(mod, k)
     if hasattr(mod, "MagnetD"):

@@ -84,3 +84,36 @@ def pencil_with_null_block(A, B, k=2):
     E = np.zeros((A.shape[0], k), dtype=complex)
     E[cols, np.arange(k)] = 1.0
     return ss.csr_matrix(A), Bn, E
+
+
+# Reference golden of the Dormy et al. (2004) onset test, tests/dormy2004/reference.dormy04:1
+# (Ek, ricb, Ra_c, m, omega_c as printed with '%.3e %.2f %.5e %d %.5e').
+DORMY_GOLDEN = (2.000e-05, 0.35, 1.65404e+06, 9, -1.10162e-02)
+
+
+def params_dormy(Ra_gap):
+    """tests/dormy2004/params.dormy04 of the reference: thermal convection,
+    differential heating, isothermal no-slip walls, m=9, Ek=2e-5, Pr=1,
+    N = Ncheb(Ek)/2 = 74, lmax = 156."""
+    from kore_amd.params import Params
+    return Params(m=9, symm=1, ricb=0.35, bci=1, bco=1, Ek=2e-5, N=74, lmax=156, thermal=1, Prandtl=1.0,
+                  heating="differential", Ra_gap=Ra_gap, bci_thermal=0, bco_thermal=0,
+                  which_eigenpairs="TR", tau=0.0, nev=4, tol=1e-15)
+
+
+def critical_rayleigh(growth, lo=1.6e6, hi=1.7e6):
+    """Brent's method on log10(Ra_gap) with the reference driver's tolerances
+    (tests/jones2000/find_Rac.py:68-88: xtol = rtol = 1e-6); growth(Ra_gap)
+    returns (max Re lambda, the corresponding lambda)."""
+    from scipy.optimize import brentq
+    cache = {}
+
+    def f(x):
+        ra = 10 ** x
+        if ra not in cache:
+            cache[ra] = growth(ra)
+        return cache[ra][0]
+    x0 = brentq(f, np.log10(lo), np.log10(hi), xtol=1e-6, rtol=1e-6, maxiter=200)
+    ra = 10 ** x0
+    lam = growth(ra)[1]
+    return ra, lam, len(cache)

@@ -48,6 +48,13 @@ CASES = {
     # The reference's spin-over test (tests/spinover/params.spinover is
     # identical to bin/parameters.py): m=1, symm=-1, Ek=1e-3, N=68, lmax=64.
     "spinover": dict(),
+    # Thermal convection (Boussinesq, differential heating, isothermal walls),
+    # the setting of the reference's Dormy et al. 2004 onset test
+    # (tests/dormy2004/params.dormy04) at a small resolution.
+    "thermal_m9_N24_L32": dict(
+        m="9", symm="1", ricb="0.35", bci="1", bco="1", Ek="2e-5",
+        N="24", lmax="32", ncpus="2", thermal="1", Prandtl="1", heating="'differential'",
+        Ra_gap="1654042.168683", bci_thermal="0", bco_thermal="0"),
     # Small magneto-thermal (MAC) case: 5 fields, m=2, symmetric flow, axial B0.
     "mac_m2_N16_L17": dict(
         m="2", symm="1", ricb="0.35", bci="1", bco="1", Ek="1e-3",
@@ -166,6 +173,8 @@ if par.magnetic:
     meta["cnorm"] = float(ut.B0_norm())
 
 if par.thermal:
+    meta["Beyonce"] = float(par.Beyonce); meta["ThermaD"] = float(par.ThermaD); meta["heating"] = str(par.heating)
+    meta["Ra_gap"] = float(par.Ra_gap); meta["Prandtl"] = float(par.Prandtl)
     for l in sample(top):
         l = int(l)
         put("buoy:u:%d" % l, op.buoyancy(l, 'u', '', 0))

@@ -231,3 +231,30 @@ def test_solve_gmres(ctx, precision):
     if precision == 0:
         assert info.iterations <= 3
     S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dormy2004_onset_golden_device(ctx, precision):
+    """The reference's Dormy et al. (2004) onset test through the device path:
+    thermal convection pencil (degree blocks alternating 2N1 = 148 and N1 =
+    74), cyclic-reduction factor and block Krylov-Schur ('TR' about 0, nev 4)
+    inside the reference driver's Brent search; Ra_c and omega_c as printed by
+    the driver must equal tests/dormy2004/reference.dormy04:1 at rtol 1e-8."""
+    from conftest import DORMY_GOLDEN, critical_rayleigh, params_dormy
+    from kore_amd import assembly, eigs
+
+    def growth(ra):
+        par = params_dormy(ra)
+        A, B, _ = assembly.assemble(par)
+        S = _system(ctx, par, A, B)
+        S.factor(0.0, precision)
+        r = eigs.eigs(S, 0.0, nev=4, tol=1e-14, which="TR", maxit=200)
+        S.close()
+        lam = r.lam[:4]
+        i = int(np.argmax(lam.real))
+        return float(lam[i].real), lam[i]
+    say("case: dormy2004 onset, precision", precision)
+    ra, lam, nsolves = critical_rayleigh(growth)
+    say("  Ra_c %.5e omega_c %.5e after %d eigensolves" % (ra, lam.imag, nsolves))
+    got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
+    np.testing.assert_allclose(got, [DORMY_GOLDEN[2], DORMY_GOLDEN[4]], rtol=1e-8, atol=1e-20)

@@ -18,13 +18,21 @@ import scipy.linalg as la
 import scipy.sparse as ss
 import scipy.sparse.linalg as sla
 
-from conftest import SPINOVER_LAMBDA, load_ops_fixture, params_config1, params_spinover
+from conftest import (DORMY_GOLDEN, SPINOVER_LAMBDA, critical_rayleigh, load_ops_fixture, params_config1,
+                      params_dormy, params_spinover)
 
 from kore_amd import assembly
 from kore_amd.params import Params, ell as kell
 from oracle import bcr, blocktri, krylov_schur, layout
 
-FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_config1), ("spinover", params_spinover)]
+def params_thermal():
+    # tests/golden/make_fixtures.py case "thermal_m9_N24_L32" (Dormy et al. 2004 setting)
+    return Params(m=9, symm=1, ricb=0.35, bci=1, bco=1, Ek=2e-5, N=24, lmax=32, thermal=1, Prandtl=1.0,
+                  heating="differential", Ra_gap=1654042.168683, bci_thermal=0, bco_thermal=0)
+
+
+FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_config1), ("spinover", params_spinover),
+            ("thermal_m9_N24_L32", params_thermal)]
 
 
 @pytest.mark.parametrize("name,mkpar", FIXTURES)
@@ -62,6 +70,8 @@ def test_degree_blocks_match_reference(name, mkpar):
         elif kind == "cori":
             off = int(parts[3])
             term = "cori:" + sec + ("" if off == 0 else (":-1" if off < 0 else ":+1"))
+        elif kind in ("buoy", "theta", "tadv", "tdiff"):
+            term = kind + ":" + sec
         else:
             continue
         ref = get(k).toarray()
@@ -69,6 +79,16 @@ def test_degree_blocks_match_reference(name, mkpar):
         assert np.abs(ref - mine).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300), k
         n += 1
     assert n >= 100
+
+
+def test_thermal_factors_match_reference():
+    """Buoyancy and thermal-diffusion factors derived from Ra_gap, Ek, Prandtl
+    as bin/parameters.py does (BV2 = -Ra Ek^2/Pr, Ra = Ra_gap/(1-ricb)^3)."""
+    d, meta, get, keys = load_ops_fixture("thermal_m9_N24_L32")
+    par = params_thermal()
+    assert abs(par.Beyonce - meta["Beyonce"]) <= 1e-14 * abs(meta["Beyonce"])
+    assert abs(par.ThermaD - meta["ThermaD"]) <= 1e-14 * abs(meta["ThermaD"])
+    assert par.sizmat == meta["sizmat"]
 
 
 @pytest.mark.parametrize("name,mkpar", FIXTURES)
@@ -212,3 +232,23 @@ def test_sort_keys_agree():
     for which in ("TM", "TR", "TI", "LM", "SM", "LR", "SR", "LI", "SI"):
         np.testing.assert_array_equal(np.argsort(k_product(which, lam, tau)),
                                       np.argsort(krylov_schur.sort_key(which, lam, tau)))
+
+
+def test_dormy2004_onset_golden():
+    """End-to-end pin of the thermal assembly and the oracle eigensolver: the
+    critical Rayleigh number and drift frequency of the reference's Dormy et
+    al. (2004) test (tests/dormy2004/reference.dormy04:1), found as the
+    reference driver does (Brent on log10 Ra, max growth rate of the nev=4
+    'TR' eigenvalues nearest 0), printed as the driver prints them and
+    compared at the reference test's rtol 1e-8 (tests/test_convection_bouss.py:27-42)."""
+    def growth(ra):
+        par = params_dormy(ra)
+        A, B, _ = assembly.assemble(par)
+        lu = sla.splu(A.tocsc())
+        lam, _, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], 0.0, 4, tol=1e-14,
+                                                 which="TR", maxit=200)
+        i = int(np.argmax(lam.real))
+        return float(lam[i].real), lam[i]
+    ra, lam, nsolves = critical_rayleigh(growth)
+    got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
+    np.testing.assert_allclose(got, [DORMY_GOLDEN[2], DORMY_GOLDEN[4]], rtol=1e-8, atol=1e-20)
