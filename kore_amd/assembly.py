@@ -125,15 +125,24 @@ def multiplication(a: np.ndarray, lam: int) -> ss.csr_matrix:
 class RadialOps:
     """Radial operators ``r^p D^d`` expressed in the C^(g) basis of a section.
 
-    ``op(p, d, g)`` equals Kore's ``rP_Dd_<section>`` (bin/submatrices.py:545,
-    582-588): conversion C^(d) -> C^(g), times multiplication by r^p in
-    C^(d), times the d-th derivative, with the top ``g`` rows emptied to make
-    room for the boundary conditions (inner-core case).
+    ``op(p, d, g, vpar)`` equals Kore's ``rP_Dd_<section>`` (bin/submatrices.py:545,
+    550-588): conversion C^(d) -> C^(g), times multiplication by r^p in
+    C^(d), times the d-th derivative, then
+
+    * with an inner core: the top ``g`` rows emptied for the boundary
+      conditions;
+    * without one (full sphere, r = x on [-1, 1]): only the coefficients of
+      one parity are kept -- columns of the vector parity ``vpar`` of the
+      field acted on, rows of the parity ``vpar (-1)^(p+d)`` of the result
+      (``remroco``, bin/utils.py:187-196) -- and the top ``g/2`` rows are
+      emptied (one boundary, at the CMB).
     """
 
     def __init__(self, N: int, ricb: float):
-        if ricb <= 0:
-            raise NotImplementedError("assembly without an inner core (ricb = 0) is not supported")
+        if ricb < 0:
+            raise ValueError("ricb must be >= 0")
+        if ricb == 0 and N % 2:
+            raise ValueError("N must be even without an inner core")
         self.N, self.ricb = N, ricb
         self._S = [conversion(l, N) for l in range(4)]
 
@@ -144,7 +153,7 @@ class RadialOps:
         return out
 
     @lru_cache(maxsize=None)
-    def op(self, p: int, d: int, g: int) -> ss.csr_matrix:
+    def op(self, p: int, d: int, g: int, vpar: int = 0) -> ss.csr_matrix:
         N = self.N
         a = self.convert(0, d) @ cheb_rpow(p, N, self.ricb)
         a[np.abs(a) < 1e-15 * max(1.0, np.abs(a).max())] = 0.0
@@ -153,9 +162,18 @@ class RadialOps:
         if d > 0:
             mat = mat @ derivative(d, N, self.ricb)
         mat = ss.csr_matrix(mat)
-        chop = g
+        if self.ricb > 0:
+            chop = g
+        else:
+            if vpar not in (1, -1):
+                raise ValueError("full-sphere operators need the vector parity (+1 or -1) of the field")
+            overall = vpar * (1 if (p + d) % 2 == 0 else -1)
+            mat = mat[(1 - overall) // 2::2, (1 - vpar) // 2::2]
+            chop = g // 2
+        nr, nc = mat.shape
         if chop > 0:
-            mat = ss.vstack([ss.csr_matrix((chop, N)), mat[:-chop, :]], format="csr")
+            mat = ss.vstack([ss.csr_matrix((chop, nc)), mat[:-chop, :]], format="csr")
+        mat = ss.csr_matrix(mat)
         mat.eliminate_zeros()
         return mat
 
@@ -377,12 +395,14 @@ def degree_block(par: Params, R: "RadialOps", term: str, l: int):
         "tdiff:h": (H_OPS, 2, lambda: h_term_diffusion([l], par)),
     }
     ops, g, coef = tabs[term]
+    # parity of the field the block acts on (used without an inner core)
+    vpar = -par.vP if term in ("cori:u:-1", "cori:u:+1", "u:v", "cori:v", "visc:v") else par.vP
     c = coef()[0]
     out = None
     for cj, (p, d) in zip(c, ops):
         if cj == 0:
             continue
-        t = cj * R.op(p, d, g)
+        t = cj * R.op(p, d, g, vpar)
         out = t if out is None else out + t
     return ss.csr_matrix(out) if out is not None else ss.csr_matrix((par.N1, par.N1))
 
@@ -420,23 +440,33 @@ def _triplets(values, pat, row_off, col_off):
 
 
 def _bc_rows(par: Params, Tb, Ta, section: str, l):
-    """Dense boundary-condition rows for one section (bin/assemble.py:1188-1296)."""
+    """Dense boundary-condition rows for one section (bin/assemble.py:1188-1296,
+    1300-1336).  Without an inner core only the CMB rows exist, over the
+    coefficients of the section's parity (bin/assemble.py:1214-1219, 1311-1315)."""
     rows = []
+    core = par.ricb > 0
+    if not core:
+        s = par.s
+        ix = {"u": (par.m + 1 - s) % 2, "v": (par.m + s) % 2, "h": (par.m + 1 - s) % 2}[section]
+        Tb = Tb[ix::2]
     if section == "u":
         if par.bco == 1:
             rows += [Tb[:, 0], Tb[:, 1]]                  # P = 0, P' = 0 at the CMB
         else:
             rows += [Tb[:, 0], Tb[:, 2]]                  # P = 0, r P'' = 0 (rcmb = 1)
-        if par.bci == 1:
-            rows += [Ta[:, 0], Ta[:, 1]]
-        else:
-            rows += [Ta[:, 0], par.ricb * Ta[:, 2]]
+        if core:
+            if par.bci == 1:
+                rows += [Ta[:, 0], Ta[:, 1]]
+            else:
+                rows += [Ta[:, 0], par.ricb * Ta[:, 2]]
     elif section == "v":
         rows += [Tb[:, 0] if par.bco == 1 else (-Tb[:, 1] + Tb[:, 0])]
-        rows += [Ta[:, 0] if par.bci == 1 else (-par.ricb * Ta[:, 1] + Ta[:, 0])]
-    else:                                                # heat equation (bin/assemble.py:1300-1336)
+        if core:
+            rows += [Ta[:, 0] if par.bci == 1 else (-par.ricb * Ta[:, 1] + Ta[:, 0])]
+    else:                                                # heat equation
         rows += [Tb[:, 0] if par.bco_thermal == 0 else Tb[:, 1]]     # isothermal / fixed flux, CMB
-        rows += [Ta[:, 0] if par.bci_thermal == 0 else Ta[:, 1]]     # ICB
+        if core:
+            rows += [Ta[:, 0] if par.bci_thermal == 0 else Ta[:, 1]]     # ICB
     return np.array(rows)
 
 
@@ -467,10 +497,11 @@ def assemble(par: Params, which: str = "eigen"):
     pos_bot = {int(l): k for k, l in enumerate(bot)}
     wf = par.wf if which == "forced" else 0.0
 
-    pu = _Pattern([R.op(p, d, 4) for (p, d) in U_OPS])
-    puo = _Pattern([R.op(p, d, 4) for (p, d) in U_OFF])
-    pv = _Pattern([R.op(p, d, 2) for (p, d) in V_OPS])
-    pvo = _Pattern([R.op(p, d, 2) for (p, d) in V_OFF])
+    vP, vT = par.vP, -par.vP       # parities of upol (and theta) / utor, used when ricb = 0
+    pu = _Pattern([R.op(p, d, 4, vP) for (p, d) in U_OPS])
+    puo = _Pattern([R.op(p, d, 4, vT) for (p, d) in U_OFF])
+    pv = _Pattern([R.op(p, d, 2, vT) for (p, d) in V_OPS])
+    pvo = _Pattern([R.op(p, d, 2, vP) for (p, d) in V_OFF])
 
     kt = np.arange(nb)
     rows_u = kt * N1
@@ -502,9 +533,9 @@ def assemble(par: Params, which: str = "eigen"):
     if par.thermal:
         # heat equation, degrees of the poloidal set (bin/assemble.py:1017-1048)
         rows_h = 2 * n + kt * N1
-        ph = _Pattern([R.op(p, d, 2) for (p, d) in H_OPS])
-        pha = _Pattern([R.op(p, d, 2) for (p, d) in H_ADV])
-        pub = _Pattern([R.op(p, d, 4) for (p, d) in U_BUOY])
+        ph = _Pattern([R.op(p, d, 2, vP) for (p, d) in H_OPS])
+        pha = _Pattern([R.op(p, d, 2, vP) for (p, d) in H_ADV])
+        pub = _Pattern([R.op(p, d, 4, vP) for (p, d) in U_BUOY])
         add(_triplets(ph.blocks(h_diag_coefs(top, par, wf)), ph, rows_h, rows_h))
         add(_triplets(pha.blocks(h_adv_coefs(top, par)), pha, rows_h, rows_u))
         add(_triplets(pub.blocks(u_buoy_coefs(top, par)), pub, rows_u, rows_h))

@@ -148,6 +148,12 @@ class Params:
         return (2 * int(np.sign(self.m)) - 1) * self.symm
 
     @property
+    def vP(self) -> int:
+        """Radial parity of the poloidal flow coefficients without an inner
+        core (bin/submatrices.py:45-47); toroidal vT = -vP, temperature vP."""
+        return int((1 - 2 * (self.m % 2)) * self.symm)
+
+    @property
     def symmB0(self) -> int:
         if self.B0 in ("axial", "dipole", "G21 dipole", "Luo_S1"):
             return -1

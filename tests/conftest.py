@@ -117,3 +117,17 @@ def critical_rayleigh(growth, lo=1.6e6, hi=1.7e6):
     ra = 10 ** x0
     lam = growth(ra)[1]
     return ra, lam, len(cache)
+
+
+# Reference golden of the Jones et al. (2000) onset test, tests/jones2000/reference.jones:1.
+JONES_GOLDEN = (6.325e-05, 0.00, 4.66986e+06, 9, -1.93444e-02)
+
+
+def params_jones(Ra_gap):
+    """tests/jones2000/params.jones of the reference: full sphere (ricb = 0),
+    internal heating, fixed-flux CMB, no-slip, m=9, Ek = sqrt(4/Ta) with
+    Ta = 1e9, Pr = 1, N = Ncheb(Ek) = 118, lmax = 124."""
+    from kore_amd.params import Params
+    return Params(m=9, symm=1, ricb=0.0, bci=1, bco=1, Ek=float(np.sqrt(4 / 1e9)), N=118, lmax=124, thermal=1,
+                  Prandtl=1.0, heating="internal", Ra_gap=Ra_gap, bci_thermal=1, bco_thermal=1,
+                  which_eigenpairs="TR", tau=0.0, nev=5, tol=1e-15)

@@ -55,6 +55,13 @@ CASES = {
         m="9", symm="1", ricb="0.35", bci="1", bco="1", Ek="2e-5",
         N="24", lmax="32", ncpus="2", thermal="1", Prandtl="1", heating="'differential'",
         Ra_gap="1654042.168683", bci_thermal="0", bco_thermal="0"),
+    # Full sphere (no inner core): thermal convection with internal heating
+    # and fixed-flux boundary, the setting of the reference's Jones et al.
+    # 2000 onset test (tests/jones2000/params.jones) at a small resolution.
+    "jones_m9_N24_L32": dict(
+        m="9", symm="1", ricb="0", bci="1", bco="1", Ek="np.sqrt(4/1e9)",
+        N="24", lmax="32", ncpus="2", thermal="1", Prandtl="1", heating="'internal'",
+        Ra_gap="4.6e6", bci_thermal="1", bco_thermal="1"),
     # Small magneto-thermal (MAC) case: 5 fields, m=2, symmetric flow, axial B0.
     "mac_m2_N16_L17": dict(
         m="2", symm="1", ricb="0.35", bci="1", bco="1", Ek="1e-3",

@@ -233,28 +233,37 @@ def test_solve_gmres(ctx, precision):
     S.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_dormy2004_onset_golden_device(ctx, precision):
-    """The reference's Dormy et al. (2004) onset test through the device path:
-    thermal convection pencil (degree blocks alternating 2N1 = 148 and N1 =
-    74), cyclic-reduction factor and block Krylov-Schur ('TR' about 0, nev 4)
-    inside the reference driver's Brent search; Ra_c and omega_c as printed by
-    the driver must equal tests/dormy2004/reference.dormy04:1 at rtol 1e-8."""
-    from conftest import DORMY_GOLDEN, critical_rayleigh, params_dormy
+def _device_growth(ctx, mkpar, nev, precision):
     from kore_amd import assembly, eigs
 
     def growth(ra):
-        par = params_dormy(ra)
+        par = mkpar(ra)
         A, B, _ = assembly.assemble(par)
         S = _system(ctx, par, A, B)
         S.factor(0.0, precision)
-        r = eigs.eigs(S, 0.0, nev=4, tol=1e-14, which="TR", maxit=200)
+        r = eigs.eigs(S, 0.0, nev=nev, tol=1e-14, which="TR", maxit=200)
         S.close()
-        lam = r.lam[:4]
+        lam = r.lam[:nev]
         i = int(np.argmax(lam.real))
         return float(lam[i].real), lam[i]
-    say("case: dormy2004 onset, precision", precision)
-    ra, lam, nsolves = critical_rayleigh(growth)
+    return growth
+
+
+@pytest.mark.parametrize("case", ["dormy2004", "jones2000"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_onset_golden_device(ctx, case, precision):
+    """The reference's onset tests through the device path: thermal
+    convection pencils (degree blocks alternating 2N1 and N1; Dormy: inner
+    core, N1 = 74; Jones: full sphere, N1 = 59), cyclic-reduction factor and
+    block Krylov-Schur ('TR' about 0) inside the reference driver's Brent
+    search; Ra_c and omega_c as printed by the driver must equal
+    tests/dormy2004/reference.dormy04:1 and tests/jones2000/reference.jones:1
+    at rtol 1e-8 (tests/test_convection_bouss.py)."""
+    from conftest import DORMY_GOLDEN, JONES_GOLDEN, critical_rayleigh, params_dormy, params_jones
+    mk, nev, golden, lo, hi = {"dormy2004": (params_dormy, 4, DORMY_GOLDEN, 1.6e6, 1.7e6),
+                               "jones2000": (params_jones, 5, JONES_GOLDEN, 4.6e6, 4.8e6)}[case]
+    say("case: onset", case, "precision", precision)
+    ra, lam, nsolves = critical_rayleigh(_device_growth(ctx, mk, nev, precision), lo=lo, hi=hi)
     say("  Ra_c %.5e omega_c %.5e after %d eigensolves" % (ra, lam.imag, nsolves))
     got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
-    np.testing.assert_allclose(got, [DORMY_GOLDEN[2], DORMY_GOLDEN[4]], rtol=1e-8, atol=1e-20)
+    np.testing.assert_allclose(got, [golden[2], golden[4]], rtol=1e-8, atol=1e-20)
