@@ -44,6 +44,10 @@ WORKLOADS = {
     "config1": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=40),
                     sigma=1.0j, nev=5, tol=1e-12,
                     name="eigen m=1 E=1e-4 N=24 lmax=40 nev=5 sigma=1i"),
+    "config4": dict(par=dict(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=150, lmax=201, magnetic=1,
+                             thermal=1, Le2=1e-8, Pm=1e-3, Prandtl=1.0, Ra_gap=6e5),
+                    sigma=0.1j, nev=8, tol=1e-12,
+                    name="MAC eigen m=2 E=1e-6 Le=1e-4 Pm=1e-3 Pr=1 N=150 lmax=201 nev=8 sigma=0.1i"),
     "config3": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-7, N=400, lmax=500),
                     sigma=-1e-3 + 0.5j, nev=10, tol=1e-12,
                     name="eigen m=1 E=1e-7 N=400 lmax=500 nev=10 sigma=-1e-3+0.5i"),

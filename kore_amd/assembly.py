@@ -372,6 +372,182 @@ def u_buoy_coefs(l, par: Params):
     return (par.Beyonce * L)[..., None].astype(complex)
 
 
+# ---------------------------------------------------------------------------
+# magnetic field: Lorentz force, induction, magnetic diffusion
+# (bin/operators.py:198-380, 433-696) for the uniform axial background field
+# ---------------------------------------------------------------------------
+# Background field B0 = curl curl (h(r) Y_1^0 r-hat) with h(r) = r/2 for the
+# uniform axial field (bin/utils.py:555-735), so every operator label
+# rX_hY_DZ (r^X h^(Y) D^Z) is a plain rP_DZ times cnorm/2: h0 r^X = r^(X+1)/2,
+# h1 r^X = r^X/2, h2 = h3 = 0; qX means r^-X.  The magnetic diffusivity is
+# uniform (bin/radial_profiles.py:252-262), so rX_eta0_DZ = rX_DZ and the
+# eta1 (derivative) terms vanish.
+
+def _label_op(label: str):
+    """(p, d, scale, uses_h) of a reference operator label, or None if it
+    vanishes for the axial field."""
+    parts = label.split("_")
+    d = int(parts[-1][1:])
+    if parts[0][0] == "q":
+        x = -int(parts[0][1:])
+    else:
+        x = int(parts[0][1:])
+    mids = parts[1:-1]
+    if not mids:
+        return (x, d, 1.0, False)
+    prof = mids[0]
+    if prof.startswith("eta"):
+        return None if prof != "eta0" else (x, d, 1.0, False)
+    if prof == "h0":
+        return (x + 1, d, 0.5, True)
+    if prof == "h1":
+        return (x, d, 0.5, True)
+    return None                                           # h2, h3
+
+
+def b0_norm(par: Params) -> float:
+    """cnorm of bin/utils.py:837-880 for the axial field: 'rms_cmb' or the
+    total magnetic energy ('mag_energy', 'Schmitt2012'), by the same 240-point
+    Chebyshev-Gauss quadrature."""
+    l, L = 1, 2
+    if par.cnorm == "rms_cmb":
+        return float(np.sqrt(2 * l + 1) / (L * 0.5))
+    if par.cnorm in ("mag_energy", "Schmitt2012"):
+        Nq = 240
+        i = np.arange(Nq)
+        xk = np.cos((i + 0.5) * np.pi / Nq)
+        sqx = np.sqrt(1 - xk ** 2)
+        rk = 0.5 * (1 - par.ricb) * (xk + 1) + par.ricb
+        y0, y1 = 0.5 * rk, 0.5 + 0 * rk
+        f0 = 4 * np.pi * L / (2 * l + 1)
+        integ = (np.pi / Nq) * ((1 - par.ricb) / 2) * np.sum(sqx * f0 * ((L + 1) * y0 ** 2 + 2 * rk * y0 * y1 +
+                                                                        rk ** 2 * y1 ** 2))
+        return float((1.0 if par.cnorm == "mag_energy" else 2.0) / np.sqrt(integ))
+    return float(par.cnorm)
+
+
+def _lor_u_bpol(l, par, side):
+    m = par.m
+    l, L = _L(l)
+    if side < 0:
+        C = np.sqrt(l ** 2 - m ** 2) * (l ** 2 - 1) / (2 * l - 1)
+        t = [("r1_h0_D1", -2 * (l ** 2 + 2)), ("r2_h1_D1", -2 * (l - 2)), ("r2_h0_D2", -(l - 4)),
+             ("r3_h1_D2", -(l - 2)), ("r0_h0_D0", L * (l + 2)), ("r1_h1_D0", L * (l - 4)), ("r3_h0_D3", 2 + 0 * l)]
+    else:
+        C = np.sqrt((1 + l + m) * (1 + l - m)) * l * (l + 2) / (2 * l + 3)
+        t = [("r1_h0_D1", -2 * (l ** 2 + 2 * l + 3)), ("r2_h1_D1", 2 * (l + 3)), ("r2_h0_D2", l + 5),
+             ("r3_h1_D2", l + 3), ("r0_h0_D0", -L * (l - 1)), ("r1_h1_D0", -L * (l + 5)), ("r3_h0_D3", 2 + 0 * l)]
+    return [(lab, par.Le2 * C * c) for lab, c in t]
+
+
+def _lor_u_btor(l, par, side):
+    l, L = _L(l)
+    c = 2j * par.m * par.Le2
+    return [("r1_h0_D0", -c + 0 * l), ("r2_h1_D0", -c * (l ** 2 + l - 1)), ("r2_h0_D1", c + 0 * l),
+            ("r3_h1_D1", c + 0 * l), ("r3_h0_D2", c + 0 * l)]
+
+
+def _lor_v_bpol(l, par, side):
+    l, L = _L(l)
+    c = 1j * par.m * par.Le2
+    return [("r0_h0_D1", 4 * c + 0 * l), ("r0_h1_D0", -2 * c * L), ("r1_h0_D2", 2 * c + 0 * l)]
+
+
+def _lor_v_btor(l, par, side):
+    m = par.m
+    l, L = _L(l)
+    if side < 0:
+        C = np.sqrt((l - m) * (l + m)) * (l ** 2 - 1) / (2 * l - 1)
+        t = [("r0_h0_D0", l - 2), ("r1_h1_D0", l), ("r1_h0_D1", -2 + 0 * l)]
+    else:
+        C = -np.sqrt((l + m + 1) * (l + 1 - m)) * l * (l + 2) / (2 * l + 3)
+        t = [("r0_h0_D0", l + 3), ("r1_h1_D0", l + 1), ("r1_h0_D1", 2 + 0 * l)]
+    return [(lab, par.Le2 * C * c) for lab, c in t]
+
+
+def _ind_f_upol(l, par, side):
+    m = par.m
+    l, L = _L(l)
+    if side < 0:
+        C = np.sqrt(l ** 2 - m ** 2) * (l ** 2 - 1) / (2 * l - 1)
+        t = [("r0_h0_D0", l - 2), ("r1_h1_D0", l), ("r1_h0_D1", -2 + 0 * l)]
+    else:
+        C = np.sqrt((l + 1) ** 2 - m ** 2) * l * (l + 2) / (2 * l + 3)
+        t = [("r0_h0_D0", -(l + 3)), ("r1_h1_D0", -(l + 1)), ("r1_h0_D1", -2 + 0 * l)]
+    return [(lab, C * c) for lab, c in t]
+
+
+def _ind_f_utor(l, par, side):
+    l, L = _L(l)
+    return [("r1_h0_D0", -2j * par.m + 0 * l)]
+
+
+def _ind_g_upol(l, par, side):
+    l, L = _L(l)
+    c = 2j * par.m
+    return [("r0_h0_D1", c + 0 * l), ("r1_h1_D1", c + 0 * l), ("q1_h0_D0", -c * (L + 1)), ("r0_h1_D0", c + 0 * l),
+            ("r1_h0_D2", c + 0 * l)]
+
+
+def _ind_g_utor(l, par, side):
+    m = par.m
+    l, L = _L(l)
+    if side < 0:
+        C = (l ** 2 - 1) * np.sqrt(l ** 2 - m ** 2) / (2 * l - 1)
+        t = [("r0_h0_D0", l + 0 * l), ("r1_h0_D1", -2 + 0 * l), ("r1_h1_D0", l - 2)]
+    else:
+        C = l * (l + 2) * np.sqrt((l + 1) ** 2 - m ** 2) / (3 + 2 * l)
+        t = [("r1_h0_D1", -2 + 0 * l), ("r0_h0_D0", -(l + 1)), ("r1_h1_D0", -(l + 3))]
+    return [(lab, C * c) for lab, c in t]
+
+
+def _mag_b(l, par, side):
+    l, L = _L(l)
+    return [("r2_D0", L)]
+
+
+def _mag_diff(l, par, side):
+    l, L = _L(l)
+    K = par.Em
+    return [("r0_eta0_D0", -K * L * L), ("r1_eta0_D1", 2 * K * L), ("r2_eta0_D2", K * L)]
+
+
+# term name -> (function, basis order g of the row section, side)
+MAG_TERMS = {
+    "lorentz:u:bpol:-1": (_lor_u_bpol, 4, -1), "lorentz:u:bpol:1": (_lor_u_bpol, 4, 1),
+    "lorentz:u:btor:0": (_lor_u_btor, 4, 0), "lorentz:v:bpol:0": (_lor_v_bpol, 2, 0),
+    "lorentz:v:btor:-1": (_lor_v_btor, 2, -1), "lorentz:v:btor:1": (_lor_v_btor, 2, 1),
+    "induction:f:upol:-1": (_ind_f_upol, 2, -1), "induction:f:upol:1": (_ind_f_upol, 2, 1),
+    "induction:f:utor:0": (_ind_f_utor, 2, 0), "induction:g:upol:0": (_ind_g_upol, 2, 0),
+    "induction:g:utor:-1": (_ind_g_utor, 2, -1), "induction:g:utor:1": (_ind_g_utor, 2, 1),
+    "b:f": (_mag_b, 2, 0), "b:g": (_mag_b, 2, 0), "mdiff:f": (_mag_diff, 2, 0), "mdiff:g": (_mag_diff, 2, 0),
+}
+
+
+def _mag_coefs(term, l, par, cnorm):
+    """Resolve a magnetic term into {(p, d): coefficient per degree}."""
+    fn, g, side = MAG_TERMS[term]
+    out = {}
+    for lab, c in fn(np.asarray(l, dtype=float), par, side):
+        r = _label_op(lab)
+        if r is None:
+            continue
+        p, d, scale, uses_h = r
+        c = np.asarray(c, dtype=complex) * scale * (cnorm if uses_h else 1.0)
+        out[(p, d)] = out.get((p, d), 0) + c
+    return out
+
+
+def _mag_blocks(R, term, l, par, cnorm):
+    """(pattern, values (nblk, nnz)) of a magnetic term for the degrees l."""
+    fn, g, side = MAG_TERMS[term]
+    co = _mag_coefs(term, l, par, cnorm)
+    ops = sorted(co)
+    pat = _Pattern([R.op(p, d, g) for (p, d) in ops])
+    coefs = np.stack([np.broadcast_to(co[k], (len(np.atleast_1d(l)),)) for k in ops], axis=-1)
+    return pat, pat.blocks(coefs)
+
+
 def degree_block(par: Params, R: "RadialOps", term: str, l: int):
     """One per-degree operator block, by term name:
     'u:u', 'cori:u', 'visc:u', 'cori:u:-1', 'cori:u:+1', 'u:v', 'cori:v', 'visc:v',
@@ -394,6 +570,10 @@ def degree_block(par: Params, R: "RadialOps", term: str, l: int):
         "tadv:h": (H_ADV, 2, lambda: h_adv_coefs([l], par)),
         "tdiff:h": (H_OPS, 2, lambda: h_term_diffusion([l], par)),
     }
+    if term in MAG_TERMS:
+        pat, vals = _mag_blocks(R, term, [l], par, b0_norm(par))
+        mat = ss.csr_matrix((vals[0], (pat.row, pat.col)), shape=(par.N1, par.N1))
+        return mat
     ops, g, coef = tabs[term]
     # parity of the field the block acts on (used without an inner core)
     vpar = -par.vP if term in ("cori:u:-1", "cori:u:+1", "u:v", "cori:v", "visc:v") else par.vP
@@ -482,8 +662,12 @@ def assemble(par: Params, which: str = "eigen"):
                       A(omega) = A0 - i omega B0 (the identity of
                       bin/assemble.py:445/623 and 461/756).
     """
-    if not par.hydro or par.magnetic or par.compositional:
-        raise NotImplementedError("only the hydrodynamic and thermal (Boussinesq) assembly is implemented")
+    if not par.hydro or par.compositional:
+        raise NotImplementedError("assembly without the flow or with composition is not implemented")
+    if par.magnetic and (par.B0 != "axial" or par.ricb <= 0 or par.innercore != "insulator"
+                         or par.mantle != "insulator"):
+        raise NotImplementedError("magnetic assembly is implemented for the axial background field with an "
+                                  "insulating inner core and mantle")
     if par.thermal and par.ThermaD <= 0:
         raise NotImplementedError("thermal assembly without thermal diffusion is not implemented")
     if par.anelastic:
@@ -530,9 +714,51 @@ def assemble(par: Params, which: str = "eigen"):
             cols = np.array([pos_top[int(l) + side] for l in ls]) * N1
             add(_triplets(pvo.blocks(v_off_coefs(ls, par, side)), pvo, rows_v[sel], cols))
     secs = [("u", rows_u, top), ("v", rows_v, bot)]
+    if par.magnetic:
+        # induced field: bpol on the flow's toroidal degrees (section f, rows
+        # 2n..3n), btor on the poloidal ones (section g, rows 3n..4n), as for
+        # an equatorially antisymmetric B0 (bin/assemble.py:69-74, 817-1010)
+        cn = b0_norm(par)
+        rows_f = 2 * n + kt * N1
+        rows_g = 3 * n + kt * N1
+
+        def place(term, ls, rows, cols):
+            if len(ls) == 0:
+                return
+            pat, vals = _mag_blocks(R, term, ls, par, cn)
+            add(_triplets(vals, pat, rows, cols))
+
+        for side in (-1, 1):
+            # Lorentz force on the poloidal flow <- bpol(l +- 1)   (bin/assemble.py:653-669)
+            sel = np.array([int(l) + side in pos_bot for l in top])
+            place("lorentz:u:bpol:%d" % side, top[sel], rows_u[sel],
+                  2 * n + np.array([pos_bot[int(l) + side] for l in top[sel]], dtype=np.int64) * N1)
+            # Lorentz force on the toroidal flow <- btor(l +- 1)   (bin/assemble.py:783-796)
+            sel = np.array([int(l) + side in pos_top for l in bot])
+            place("lorentz:v:btor:%d" % side, bot[sel], rows_v[sel],
+                  3 * n + np.array([pos_top[int(l) + side] for l in bot[sel]], dtype=np.int64) * N1)
+            # induction, bpol equation <- upol(l +- 1)   (bin/assemble.py:837-849)
+            sel = np.array([int(l) + side in pos_top for l in bot])
+            place("induction:f:upol:%d" % side, bot[sel], rows_f[sel],
+                  np.array([pos_top[int(l) + side] for l in bot[sel]], dtype=np.int64) * N1)
+            # induction, btor equation <- utor(l +- 1)   (bin/assemble.py:948-960)
+            sel = np.array([int(l) + side in pos_bot for l in top])
+            place("induction:g:utor:%d" % side, top[sel], rows_g[sel],
+                  n + np.array([pos_bot[int(l) + side] for l in top[sel]], dtype=np.int64) * N1)
+        place("lorentz:u:btor:0", top, rows_u, rows_g)          # u <- btor(l)   (bin/assemble.py:671-684)
+        place("lorentz:v:bpol:0", bot, rows_v, rows_f)          # v <- bpol(l)   (bin/assemble.py:767-781)
+        place("induction:f:utor:0", bot, rows_f, rows_v)        # f <- utor(l)   (bin/assemble.py:851-861)
+        place("induction:g:upol:0", top, rows_g, rows_u)        # g <- upol(l)   (bin/assemble.py:936-946)
+        # i w b - Em lap b on the diagonal blocks   (bin/assemble.py:866-874, 972-980)
+        for term_b, term_d, ls, rows in (("b:f", "mdiff:f", bot, rows_f), ("b:g", "mdiff:g", top, rows_g)):
+            pat_b, vb = _mag_blocks(R, term_b, ls, par, cn)
+            pat_d, vd = _mag_blocks(R, term_d, ls, par, cn)
+            if wf != 0.0:
+                add(_triplets(1j * wf * vb, pat_b, rows, rows))
+            add(_triplets(-vd, pat_d, rows, rows))
     if par.thermal:
         # heat equation, degrees of the poloidal set (bin/assemble.py:1017-1048)
-        rows_h = 2 * n + kt * N1
+        rows_h = (2 + 2 * par.magnetic) * n + kt * N1
         ph = _Pattern([R.op(p, d, 2, vP) for (p, d) in H_OPS])
         pha = _Pattern([R.op(p, d, 2, vP) for (p, d) in H_ADV])
         pub = _Pattern([R.op(p, d, 4, vP) for (p, d) in U_BUOY])
@@ -550,6 +776,18 @@ def assemble(par: Params, which: str = "eigen"):
         cc = (r0[:, None, None] + 0 * np.arange(nr)[None, :, None] + np.arange(N1)[None, None, :])
         vv = np.broadcast_to(bc[None].astype(complex), rr.shape)
         add((vv.ravel(), rr.ravel(), cc.ravel()))
+    if par.magnetic:
+        # insulating inner core (first row) and mantle (second row): potential-field matching
+        # bpol: l b - ricb b' = 0 at the ICB, (l+1) b + rcmb b' = 0 at the CMB;  btor: b = 0
+        # (bin/assemble.py:1472-1571)
+        lb = bot.astype(float)[:, None]
+        bcf = np.stack([lb * Ta[None, :, 0] - par.ricb * Ta[None, :, 1],
+                        (lb + 1) * Tb[None, :, 0] + Tb[None, :, 1]], axis=1)          # (nb, 2, N1)
+        bcg = np.broadcast_to(np.stack([Ta[:, 0], Tb[:, 0]])[None], (nb, 2, N1))
+        for bc, r0 in ((bcf, rows_f), (bcg, rows_g)):
+            rr = r0[:, None, None] + np.arange(2)[None, :, None] + 0 * np.arange(N1)[None, None, :]
+            cc = r0[:, None, None] + 0 * np.arange(2)[None, :, None] + np.arange(N1)[None, None, :]
+            add((np.asarray(bc, dtype=complex).ravel(), rr.ravel(), cc.ravel()))
 
     S = par.sizmat
     A = ss.csr_matrix((np.concatenate(dat), (np.concatenate(ri), np.concatenate(ci))),
@@ -565,6 +803,11 @@ def assemble(par: Params, which: str = "eigen"):
     bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
     t = _triplets(pv.blocks(v_mass_coefs(bot)).real, pv, rows_v, rows_v)
     bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
+    if par.magnetic:
+        for term_b, ls, rows in (("b:f", bot, rows_f), ("b:g", top, rows_g)):           # -b (bin/assemble.py:467-501)
+            pat_b, vb = _mag_blocks(R, term_b, ls, par, cn)
+            t = _triplets(-vb.real, pat_b, rows, rows)
+            bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])
     if par.thermal:
         t = _triplets(ph.blocks(h_term_theta(top, par)).real, ph, rows_h, rows_h)    # +theta (bin/assemble.py:505-515)
         bdat.append(t[0]); bri.append(t[1]); bci.append(t[2])

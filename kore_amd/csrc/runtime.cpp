@@ -252,9 +252,8 @@ static int32_t layout_build(const kore_layout* L, int64_t* sizmat, std::vector<i
   if ((L->symm != 1 && L->symm != -1) || (L->symmB0 != 1 && L->symmB0 != -1))
     return fail(KORE_ERR_ARG, "symm and symmB0 must be +1 or -1");
   if (!(L->hydro || L->magnetic || L->thermal || L->compositional)) return fail(KORE_ERR_ARG, "no fields");
-  std::vector<int> top, bot, all, mtop, mbot, mall;
+  std::vector<int> top, bot, all;
   ell_lists(L->m, L->lmax, L->symm, top, bot, all);
-  ell_lists(L->m, L->lmax, L->symm * L->symmB0, mtop, mbot, mall);
   const int64_t nb = (L->lmax - L->m + 1) / 2;
   const int64_t n = (int64_t)L->N1 * nb;
   struct Sec {
@@ -269,8 +268,11 @@ static int32_t layout_build(const kore_layout* L, int64_t* sizmat, std::vector<i
     off += 2 * n;
   }
   if (L->magnetic) {
-    const std::vector<int>* f_l = L->symmB0 == 1 ? &mtop : &mbot;
-    const std::vector<int>* g_l = L->symmB0 == 1 ? &mbot : &mtop;
+    // the induced field's poloidal (f) and toroidal (g) sections use the flow's
+    // degree lists: f on the toroidal ones and g on the poloidal ones when B0 is
+    // equatorially antisymmetric, the other way round otherwise (bin/assemble.py:69-74)
+    const std::vector<int>* f_l = L->symmB0 == 1 ? &top : &bot;
+    const std::vector<int>* g_l = L->symmB0 == 1 ? &bot : &top;
     secs.push_back({off, f_l});
     secs.push_back({off + n, g_l});
     off += 2 * n;

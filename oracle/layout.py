@@ -30,7 +30,6 @@ def ell(m, lmax, vsymm):
 def sections(m, lmax, symm, N1, hydro=1, magnetic=0, thermal=0, compositional=0, symmB0=-1):
     """[(name, row_offset, degree_list)] in Kore's section-major order."""
     top, bot, _ = ell(m, lmax, symm)
-    mtop, mbot, _ = ell(m, lmax, symm * symmB0)
     nb = (lmax - m + 1) // 2
     n = N1 * nb
     out, off = [], 0
@@ -38,8 +37,10 @@ def sections(m, lmax, symm, N1, hydro=1, magnetic=0, thermal=0, compositional=0,
         out += [("upol", off, top), ("utor", off + n, bot)]
         off += 2 * n
     if magnetic:
-        f_l = mtop if symmB0 == 1 else mbot       # bin/assemble.py:69-74
-        g_l = mbot if symmB0 == 1 else mtop
+        # bpol (f) and btor (g) take the FLOW's degree lists: loc_mag_f = loc_bot,
+        # loc_mag_g = loc_top for an antisymmetric B0 (bin/assemble.py:69-74)
+        f_l = top if symmB0 == 1 else bot
+        g_l = bot if symmB0 == 1 else top
         out += [("bpol", off, f_l), ("btor", off + n, g_l)]
         off += 2 * n
     if thermal:

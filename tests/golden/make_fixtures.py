@@ -10,7 +10,9 @@ in a fresh subprocess imports the reference modules that depend only on
 numpy/scipy and records their outputs:
 
 * ``submatrices.main`` (bin/submatrices.py:28-597): every radial operator
-  ``*.mtx`` it writes, stored as CSR arrays;
+  ``*.mtx`` it writes, stored as CSR arrays (for magnetic cases after
+  ``compute_profiles`` has written the radial profiles, bin/compute_profiles.py,
+  as the reference's workflow does);
 * ``operators.u / coriolis / viscous_diffusion`` (bin/operators.py:22-194) and,
   for the magneto-thermal case, ``lorentz / b / induction /
   magnetic_diffusion / buoyancy / theta / thermal_advection /
@@ -92,6 +94,8 @@ sys.path.insert(0, os.getcwd())
 sys.path.insert(1, REF_BIN)
 import parameters as par
 import utils as ut
+if par.magnetic:
+    import compute_profiles      # writes radProfs.mat (the diffusivity profile submatrices reads)
 import submatrices
 submatrices.main(2)
 import bc_variables as bv
@@ -150,21 +154,32 @@ for l in sample(bot):
         put("cori:v:upol:%d:%d" % (i, l), c[0]); out["offd:cori:v:upol:%d:%d" % (i, l)] = np.asarray(c[1])
 
 if par.magnetic:
-    for sec, comp, offs, src in (('u', 'bpol', (-2, -1, 0, 1, 2), top), ('u', 'btor', (-1, 0, 1), top),
-                                 ('v', 'bpol', (-1, 0, 1), bot), ('v', 'btor', (-2, -1, 0, 1, 2), bot)):
+    # only the (degree, offset) pairs bin/assemble.py:653-684, 765-796, 837-861,
+    # 936-960 actually request (l+i inside the target degree set)
+    topset, botset = set(int(x) for x in ll[0]), set(int(x) for x in ll[1])
+    bpolset, btorset = set(int(x) for x in llm[0]), set(int(x) for x in llm[1])
+    for sec, comp, offs, src, tgt in (('u', 'bpol', (-2, -1, 0, 1, 2), top, bpolset),
+                                      ('u', 'btor', (-1, 0, 1), top, btorset),
+                                      ('v', 'bpol', (-1, 0, 1), bot, bpolset),
+                                      ('v', 'btor', (-2, -1, 0, 1, 2), bot, btorset)):
         for l in sample(src):
             for i in offs:
+                if int(l) + i not in tgt:
+                    continue
                 c = op.lorentz(int(l), sec, comp, i)
                 put("lorentz:%s:%s:%d:%d" % (sec, comp, i, l), c[0])
                 out["offd:lorentz:%s:%s:%d:%d" % (sec, comp, i, l)] = np.asarray(c[1])
     f_l = llm[0] if ut.symmB0 == 1 else llm[1]
     g_l = llm[1] if ut.symmB0 == 1 else llm[0]
+    f_l, g_l = ll[0] if ut.symmB0 == 1 else ll[1], ll[1] if ut.symmB0 == 1 else ll[0]
     for l in sample(f_l):
         l = int(l)
         put("b:f:bpol:%d" % l, op.b(l, 'f', 'bpol', 0))
         put("mdiff:f:bpol:%d" % l, op.magnetic_diffusion(l, 'f', 'bpol', 0))
-        for comp, offs in (('upol', (-1, 0, 1)), ('utor', (-1, 0, 1)), ('bpol', (0,))):
+        for comp, offs, tgt in (('upol', (-2, -1, 0, 1, 2), topset), ('utor', (-1, 0, 1), botset)):
             for i in offs:
+                if l + i not in tgt:
+                    continue
                 c = op.induction(l, 'f', comp, i)
                 put("induction:f:%s:%d:%d" % (comp, i, l), c[0])
                 out["offd:induction:f:%s:%d:%d" % (comp, i, l)] = np.asarray(c[1])
@@ -172,12 +187,16 @@ if par.magnetic:
         l = int(l)
         put("b:g:btor:%d" % l, op.b(l, 'g', 'btor', 0))
         put("mdiff:g:btor:%d" % l, op.magnetic_diffusion(l, 'g', 'btor', 0))
-        for comp, offs in (('upol', (-2, -1, 0, 1, 2)), ('utor', (-1, 0, 1))):
+        for comp, offs, tgt in (('upol', (-1, 0, 1), topset), ('utor', (-2, -1, 0, 1, 2), botset)):
             for i in offs:
+                if l + i not in tgt:
+                    continue
                 c = op.induction(l, 'g', comp, i)
                 put("induction:g:%s:%d:%d" % (comp, i, l), c[0])
                 out["offd:induction:g:%s:%d:%d" % (comp, i, l)] = np.asarray(c[1])
     meta["cnorm"] = float(ut.B0_norm())
+    meta["Le2"] = float(par.Le2); meta["Em"] = float(par.Em); meta["Hendrik"] = float(par.Hendrik)
+    meta["MagnetD"] = float(par.MagnetD)
 
 if par.thermal:
     meta["Beyonce"] = float(par.Beyonce); meta["ThermaD"] = float(par.ThermaD); meta["heating"] = str(par.heating)

@@ -267,3 +267,28 @@ def test_onset_golden_device(ctx, case, precision):
     say("  Ra_c %.5e omega_c %.5e after %d eigensolves" % (ra, lam.imag, nsolves))
     got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
     np.testing.assert_allclose(got, [golden[2], golden[4]], rtol=1e-8, atol=1e-20)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mac_eigs_device(ctx, precision):
+    """Magneto-thermal pencil (degree blocks alternating 3N1 = 72 and 2N1 =
+    48: upol+btor+theta / utor+bpol) through the device path: eigenvalues
+    nearest 0.1i match the oracle (SuperLU + Krylov-Schur) to 1e-10."""
+    from kore_amd import assembly, eigs
+    from kore_amd.params import Params
+    from oracle import krylov_schur
+    par = Params(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=29, magnetic=1, thermal=1,
+                 Lambda=0.1, Pm=1e-2, Prandtl=1.0, Ra_gap=6e5)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.1j
+    say("case: MAC eigs, sizmat", A.shape[0], "precision", precision)
+    S = _system(ctx, par, A, B)
+    S.factor(sigma, precision)
+    r = eigs.eigs(S, sigma, nev=6, tol=1e-12, maxit=200)
+    lu = sla.splu((A - sigma * B).tocsc())
+    lo, _, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], sigma, 6, tol=1e-13, maxit=200)
+    assert r.nconv >= 6
+    for l in lo[:6]:
+        assert np.min(np.abs(r.lam - l)) / abs(l) <= 1e-10
+    assert eigs.residuals(A, B, r.lam[:6], r.X[:, :6]).max() <= 1e-10
+    S.close()

@@ -37,8 +37,15 @@ def params_jones_small():
                   Prandtl=1.0, heating="internal", Ra_gap=4.6e6, bci_thermal=1, bco_thermal=1)
 
 
+def params_mac_small():
+    # tests/golden/make_fixtures.py case "mac_m2_N16_L17" (5 fields, axial B0, insulating walls)
+    return Params(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-3, N=16, lmax=17, magnetic=1, thermal=1,
+                  Lambda=0.1, Pm=1e-3, Prandtl=1.0, Ra_gap=6e5)
+
+
 FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_config1), ("spinover", params_spinover),
-            ("thermal_m9_N24_L32", params_thermal), ("jones_m9_N24_L32", params_jones_small)]
+            ("thermal_m9_N24_L32", params_thermal), ("jones_m9_N24_L32", params_jones_small),
+            ("mac_m2_N16_L17", params_mac_small)]
 
 
 def _label_parity(par, lab):
@@ -65,9 +72,18 @@ def test_radial_operators_match_reference(name, mkpar):
         if not k.startswith("mtx:"):
             continue
         lab = k[4:]
-        p, dd, g = int(lab[1]), int(lab[4]), 4 if lab.endswith("_u") else 2
+        g = 4 if lab.endswith("_u") else 2
         ref = get(k).toarray()
-        mine = R.op(p, dd, g, _label_parity(par, lab)).toarray()
+        if len(lab) == 7:                                   # rP_DD_s
+            mine = R.op(int(lab[1]), int(lab[4]), g, _label_parity(par, lab)).toarray()
+        else:                                               # background-field / diffusivity profiles
+            r = assembly._label_op(lab[:-2])
+            if r is None:
+                assert np.abs(ref).max() == 0, lab
+                n += 1
+                continue
+            pp, dd, scale, uses_h = r
+            mine = scale * (meta["cnorm"] if uses_h else 1.0) * R.op(pp, dd, g).toarray()
         assert np.abs(ref - mine).max() <= 1e-13 * np.abs(ref).max(), lab
         n += 1
     assert n >= 15
@@ -91,6 +107,10 @@ def test_degree_blocks_match_reference(name, mkpar):
             term = "cori:" + sec + ("" if off == 0 else (":-1" if off < 0 else ":+1"))
         elif kind in ("buoy", "theta", "tadv", "tdiff"):
             term = kind + ":" + sec
+        elif kind in ("lorentz", "induction"):
+            term = ":".join(parts[:4])
+        elif kind in ("b", "mdiff"):
+            term = kind + ":" + sec
         else:
             continue
         ref = get(k).toarray()
@@ -98,6 +118,16 @@ def test_degree_blocks_match_reference(name, mkpar):
         assert np.abs(ref - mine).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300), k
         n += 1
     assert n >= 100
+
+
+def test_magnetic_factors_match_reference():
+    """B0 normalisation (bin/utils.py:837-880) and the Lorentz / magnetic
+    diffusion factors (Le2 = Lambda Em, Em = Ek/Pm)."""
+    d, meta, get, keys = load_ops_fixture("mac_m2_N16_L17")
+    par = params_mac_small()
+    assert abs(assembly.b0_norm(par) - meta["cnorm"]) <= 1e-14 * meta["cnorm"]
+    assert abs(par.Le2 - meta["Hendrik"]) <= 1e-15 and abs(par.Em - meta["MagnetD"]) <= 1e-15
+    assert par.sizmat == meta["sizmat"]
 
 
 def test_thermal_factors_match_reference():
@@ -293,3 +323,35 @@ def test_dormy2004_onset_golden():
     ra, lam, nsolves = critical_rayleigh(growth)
     got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
     np.testing.assert_allclose(got, [DORMY_GOLDEN[2], DORMY_GOLDEN[4]], rtol=1e-8, atol=1e-20)
+
+
+def test_mac_structure_and_decoupling():
+    """Magneto-thermal (MAC) pencil: block-tridiagonal in degree order with
+    5-field blocks alternating 3N1 / 2N1, B zero on every boundary row, and
+    with the Lorentz force switched off (Le2 = 0) the flow-thermal eigenvalues
+    are exactly those of the non-magnetic pencil (the induction equation then
+    couples one way only), which pins the placement of the flow and heat
+    sections around the inserted magnetic ones."""
+    kw = dict(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-3, N=16, lmax=17, thermal=1, Prandtl=1.0, Ra_gap=6e5)
+    par = Params(magnetic=1, Lambda=0.1, Pm=1e-3, **kw)
+    A, B, _ = assembly.assemble(par)
+    perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1, magnetic=1, thermal=1)
+    assert sorted(set(sizes)) == [2 * par.N1, 3 * par.N1]
+    layout.block_tridiag(layout.permute(A, perm), sizes)
+    layout.block_tridiag(layout.permute(B, perm), sizes)
+    n, N1 = par.n, par.N1
+    bc_rows = []
+    for base, nr in ((0, 4), (n, 2), (2 * n, 2), (3 * n, 2), (4 * n, 2)):
+        for k in range(par.nb):
+            bc_rows += [base + k * N1 + q for q in range(nr)]
+    assert abs(B[bc_rows]).max() == 0
+    assert all(A[r].nnz > 0 for r in bc_rows[::7])
+    par0 = Params(magnetic=1, Le2=0.0, Pm=1e-3, **kw)
+    A0, B0, _ = assembly.assemble(par0)
+    AT, BT, _ = assembly.assemble(Params(magnetic=0, **kw))
+    w0 = la.eigvals(A0.toarray(), B0.toarray())
+    wT = la.eigvals(AT.toarray(), BT.toarray())
+    w0, wT = w0[np.isfinite(w0)], wT[np.isfinite(wT)]
+    wT = wT[np.abs(wT) < 50]
+    assert len(wT) > 100
+    assert max(np.min(np.abs(w0 - w)) / abs(w) for w in wT) <= 1e-6
