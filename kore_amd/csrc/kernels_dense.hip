@@ -411,6 +411,300 @@ __global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __res
 }
 
 // ---------------------------------------------------------------------------
+// batched Gauss-Jordan inverse, multi-workgroup form (large blocks)
+// ---------------------------------------------------------------------------
+// The same elimination as gj_inverse_kernel, split so that the rank-w update
+// of a block spreads over many workgroups (one 64 x 64 tile each) instead of
+// one compute unit: per panel J = [p0, p0+w)
+//   gj_panel_kernel   one workgroup per block: Gauss-Jordan on the panel
+//                     columns in registers (pivot rows P), C -> M[:, J],
+//                     R = M[P, :] (rows before the update, zero on J) -> scratch;
+//   gj_update_kernel  M[i, j] <- [i not in P] M[i, j] + sum_k C[i, k] R[k, j]
+//                     for j outside J, on matrix cores;
+// then gj_perm_kernel twice: A^{-1}[k, l] = M[r_k, r^{-1}(l)] through the
+// scratch block and back.  Scratch layout (per block, n x n complex): R
+// (32 x n, R[k, j] at R[j 32 + k]), then n ints of row roles (-1 unused, -2
+// pivot of an earlier panel, c >= 0: pivot c of the current panel).  The
+// multi-panel composition: with T the product of the panel's elementary
+// transforms, T differs from the identity only in the pivot columns P, so for
+// a column j outside J,  T M[:, j] = [i not in P] M[i, j] + T[:, P] M[P, j],
+// and T[:, P] is exactly what in-place elimination leaves in M[:, J].
+// v[idx] by selects, so a register array never becomes indexable memory
+template <int N, class V>
+__device__ __forceinline__ V pick(const V (&v)[N], int idx) {
+  V x = v[0];
+#pragma unroll
+  for (int q = 1; q < N; ++q)
+    if (q == idx) x = v[q];
+  return x;
+}
+
+constexpr int GJM_WP = 32;
+constexpr int GJ_MULTI_MIN = 96;          // blocks at least this large take the multi-workgroup form
+
+template <class T, int NMAX>
+__global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
+  constexpr int NT = 512, NW = NT / 64, WP = GJM_WP;
+  constexpr int TPR = NT / NMAX, CPT = WP / TPR;
+  static_assert(TPR >= 1 && CPT * TPR == WP, "panel split");
+  __shared__ cx<T> sCand[2][NW][WP];
+  __shared__ unsigned long long sKey[2][NW];
+  __shared__ int sPiv[WP];
+  __shared__ int sSing;
+  const InvDesc<T> d = descs[blockIdx.x];
+  const int n = d.n;
+  if (p0 >= n) return;                    // uniform: this block is already done
+  const int64_t ld = d.ld;
+  cx<T>* __restrict__ M = d.M;
+  int* __restrict__ rowof = d.ipiv;
+  cx<T>* __restrict__ R = d.scratch;
+  int* __restrict__ roles = reinterpret_cast<int*>(d.scratch + (size_t)WP * n);
+  const int w = min(WP, n - p0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = tid / TPR, h = tid % TPR;
+  const bool live = r < n;
+  if (tid == 0) sSing = 0;
+  for (int q = tid; q < WP; q += NT) sPiv[q] = 0;
+  __syncthreads();
+  // every row is unused at the first panel (the scratch holds an earlier call's
+  // roles); afterwards any pivot of an earlier panel is "used"
+  int role = -2;
+  if (live) role = (p0 == 0 || roles[r] == -1) ? -1 : -2;
+  cx<T> v[CPT];
+#pragma unroll
+  for (int jj = 0; jj < CPT; ++jj) {
+    const int j = h * CPT + jj;
+    v[jj] = (live && j < w) ? M[(int64_t)(p0 + j) * ld + r] : czero<T>();
+  }
+  // one elimination step per panel column; the two half loops keep each
+  // unrolled body small enough that the compiler unrolls it fully, so that v
+  // is indexed by constants only and stays in registers
+  auto step = [&](const int c) {
+    if (c < w) {                          // uniform
+      const int par = c & 1;
+      unsigned long long key = 0;
+      if (live && h == c / CPT && role == -1) key = pivot_key((double)cabs1(pick<CPT>(v, c % CPT)), r);
+      key = wave_max_u64(key);
+      if (key != 0 && r == 511 - (int)(key & 0x1FFULL)) {
+#pragma unroll
+        for (int jj = 0; jj < CPT; ++jj) sCand[par][wave][h * CPT + jj] = v[jj];
+      }
+      if (lane == 0) sKey[par][wave] = key;
+      __syncthreads();
+      unsigned long long best = sKey[par][0];
+      int q = 0;
+#pragma unroll
+      for (int qq = 1; qq < NW; ++qq) {
+        const unsigned long long kq = sKey[par][qq];
+        if (kq > best) {
+          best = kq;
+          q = qq;
+        }
+      }
+      const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
+      if (tid == 0) sPiv[c] = p;
+      if (r == p) role = c;
+      const cx<T>* __restrict__ prow = sCand[par][q];
+      cx<T> pv = prow[c];
+      T den = pv.re * pv.re + pv.im * pv.im;
+      if (!(den > T(0))) {
+        if (tid == 0 && sSing == 0) sSing = p0 + c + 1;
+        pv = cx<T>{tiny_pivot<T>(), T(0)};
+        den = pv.re * pv.re;
+      }
+      const T iden = T(1) / den;
+      const cx<T> ipv{pv.re * iden, -pv.im * iden};
+      cx<T> vc = pick<CPT>(v, c % CPT);
+      if constexpr (TPR == 2) {
+        if (c / CPT == 0) {
+          vc.re = dpp_t<0xA0>(vc.re);
+          vc.im = dpp_t<0xA0>(vc.im);
+        } else {
+          vc.re = dpp_t<0xF5>(vc.re);
+          vc.im = dpp_t<0xF5>(vc.im);
+        }
+      }
+      // Rows past n and columns past w hold zeros (and the pivot row's copy
+      // too), so the update needs no row or column guard: zeros stay zeros.
+      // Column c is held by slice c / CPT at register c % CPT.
+      if (r == p) {
+#pragma unroll
+        for (int jj = 0; jj < CPT; ++jj) {
+          const bool isc = jj == c % CPT && h == c / CPT;
+          v[jj] = isc ? ipv : cmul(v[jj], ipv);
+        }
+      } else {
+        const cx<T> l = cmul(vc, ipv), nl{-l.re, -l.im};
+#pragma unroll
+        for (int jj = 0; jj < CPT; ++jj) {
+          const cx<T> u = prow[h * CPT + jj];
+          v[jj] = (jj == c % CPT && h == c / CPT) ? nl : cfma(v[jj], nl, u);   // four FMAs
+        }
+      }
+    }
+  };
+#pragma unroll
+  for (int c = 0; c < WP / 2; ++c) step(c);
+#pragma unroll
+  for (int c = WP / 2; c < WP; ++c) step(c);
+  __syncthreads();
+  // C -> M[:, J]; R = M[P, :] outside J (disjoint columns, so the order of the
+  // two does not matter; the update kernel has not run yet)
+  if (live) {
+#pragma unroll
+    for (int jj = 0; jj < CPT; ++jj) {
+      const int j = h * CPT + jj;
+      if (j < w) M[(int64_t)(p0 + j) * ld + r] = v[jj];
+    }
+    if (h == 0) roles[r] = role;
+  }
+  if (tid < w) rowof[p0 + tid] = sPiv[tid];
+  constexpr int RB = 8;                   // loads in flight per thread
+  for (int base = 0; base < WP * n; base += RB * NT) {
+    cx<T> x[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int idx = base + t * NT + tid;
+      const int k = idx % WP, j = idx / WP;
+      x[t] = czero<T>();
+      if (idx < WP * n && k < w && (j < p0 || j >= p0 + w)) x[t] = M[(int64_t)j * ld + sPiv[k]];
+    }
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int idx = base + t * NT + tid;
+      if (idx < WP * n) R[idx] = x[t];
+    }
+  }
+  // info: reset by the first panel, then the first singular column wins
+  if (tid == 0 && d.info && (p0 == 0 || (sSing && *d.info == 0))) *d.info = sSing;
+}
+
+// One 64 x 64 tile of M per workgroup, 4 waves of 2 x 2 MFMA 16 x 16 tiles.
+template <class T>
+__global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
+  using A4 = typename acc4<T>::type;
+  constexpr int WP = GJM_WP;
+  __shared__ cx<T> sA[WP][64];            // C tile, k-major
+  __shared__ cx<T> sB[64][WP + 1];        // R tile, j-major
+  __shared__ int sRole[64];
+  const InvDesc<T> d = descs[blockIdx.y];
+  const int n = d.n;
+  if (p0 >= n) return;
+  const int nt = (n + 63) / 64;
+  if ((int)blockIdx.x >= nt * nt) return;
+  const int i0 = (blockIdx.x % nt) * 64, j0 = (blockIdx.x / nt) * 64;
+  const int w = min(WP, n - p0);
+  const int64_t ld = d.ld;
+  cx<T>* __restrict__ M = d.M;
+  const cx<T>* __restrict__ R = d.scratch;
+  const int* __restrict__ roles = reinterpret_cast<const int*>(d.scratch + (size_t)WP * n);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int l16 = lane & 15, lq = lane >> 4;
+  for (int idx = tid; idx < WP * 64; idx += 256) {
+    const int k = idx / 64, i = idx % 64;           // consecutive threads: consecutive rows
+    cx<T> x = czero<T>();
+    if (k < w && i0 + i < n) x = M[(int64_t)(p0 + k) * ld + i0 + i];
+    sA[k][i] = x;
+  }
+  for (int idx = tid; idx < WP * 64; idx += 256) {
+    const int k = idx % WP, j = idx / WP;           // consecutive threads: consecutive k of R
+    cx<T> x = czero<T>();
+    if (j0 + j < n) x = R[(int64_t)(j0 + j) * WP + k];   // zero beyond w (panel kernel)
+    sB[j][k] = x;
+  }
+  if (tid < 64) sRole[tid] = (i0 + tid < n) ? roles[i0 + tid] : -1;
+  __syncthreads();
+  A4 cre[2][2], cim[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      cre[a][b] = A4{0, 0, 0, 0};
+      cim[a][b] = A4{0, 0, 0, 0};
+    }
+#pragma unroll
+  for (int kk = 0; kk < WP; kk += 4) {
+    cx<T> a[2], b[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      a[q] = sA[kk + lq][wr + 16 * q + l16];
+      b[q] = sB[wc + 16 * q + l16][kk + lq];
+    }
+#pragma unroll
+    for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+      for (int cs = 0; cs < 2; ++cs) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
+  }
+#pragma unroll
+  for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+    for (int cs = 0; cs < 2; ++cs)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int li = wr + 16 * rs + l16;
+        const int gi = i0 + li;
+        const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, rr);
+        if (gi < n && gj < n && (gj < p0 || gj >= p0 + w)) {
+          cx<T>* cp = M + (int64_t)gj * ld + gi;
+          cx<T> x{cre[rs][cs][rr], cim[rs][cs][rr]};
+          if (!(sRole[li] >= 0)) {
+            const cx<T> o = *cp;
+            x.re += o.re;
+            x.im += o.im;
+          }
+          *cp = x;
+        }
+      }
+}
+
+// phase 0: S[l n + k] = M[r_k, r^{-1}(l)];  phase 1: M <- S.   Grid: (column chunks, blocks).
+template <class T>
+__global__ __launch_bounds__(256) void gj_perm_kernel(const InvDesc<T>* __restrict__ descs, int phase) {
+  __shared__ int sRinv[512];
+  __shared__ int sRow[512];
+  const InvDesc<T> d = descs[blockIdx.y];
+  const int n = d.n;
+  constexpr int CC = 16;                                   // columns per workgroup
+  const int l0 = blockIdx.x * CC;
+  if (l0 >= n) return;
+  const int64_t ld = d.ld;
+  cx<T>* __restrict__ M = d.M;
+  cx<T>* __restrict__ S = d.scratch;
+  if (phase == 0) {
+    for (int j = threadIdx.x; j < n; j += 256) {
+      const int rj = d.ipiv[j];
+      sRow[j] = rj;
+      sRinv[rj] = j;
+    }
+    __syncthreads();
+    for (int l = l0; l < min(n, l0 + CC); ++l) {
+      const cx<T>* src = M + (int64_t)sRinv[l] * ld;
+      for (int k = threadIdx.x; k < n; k += 256) S[(int64_t)l * n + k] = src[sRow[k]];
+    }
+  } else {
+    for (int l = l0; l < min(n, l0 + CC); ++l)
+      for (int k = threadIdx.x; k < n; k += 256) M[(int64_t)l * ld + k] = S[(int64_t)l * n + k];
+  }
+}
+
+template <class T>
+static hipError_t launch_gj_inverse_multi(const InvDesc<T>* d_descs, int ndesc, int nmax, hipStream_t s) {
+  const int nt = (nmax + 63) / 64;
+  for (int p0 = 0; p0 < nmax; p0 += GJM_WP) {
+    if (nmax <= 256)
+      hipLaunchKernelGGL((gj_panel_kernel<T, 256>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
+    else
+      hipLaunchKernelGGL((gj_panel_kernel<T, 512>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
+    hipLaunchKernelGGL((gj_update_kernel<T>), dim3(nt * nt, ndesc), dim3(256), 0, s, d_descs, p0);
+  }
+  const int nc = (nmax + 15) / 16;
+  hipLaunchKernelGGL((gj_perm_kernel<T>), dim3(nc, ndesc), dim3(256), 0, s, d_descs, 0);
+  hipLaunchKernelGGL((gj_perm_kernel<T>), dim3(nc, ndesc), dim3(256), 0, s, d_descs, 1);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class T>
@@ -422,8 +716,12 @@ hipError_t launch_gemm_batched(const GemmDesc<T>* d_descs, int ndesc, int max_ti
 }
 
 template <class T>
-hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hipStream_t s) {
+hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmin, int nmax, hipStream_t s) {
   if (ndesc <= 0) return hipSuccess;
+  // the multi-workgroup form keeps R (32 x n) and the row roles in each block's
+  // n x n scratch, which needs n > 32
+  if (nmax >= GJ_MULTI_MIN && nmax <= 512 && nmin > GJM_WP)
+    return launch_gj_inverse_multi<T>(d_descs, ndesc, nmax, s);
   if constexpr (sizeof(T) == 8) {
     if (nmax <= 256)
       hipLaunchKernelGGL((gj_inverse_kernel<T, 32, 256>), dim3(ndesc), dim3(512), 0, s, d_descs);
@@ -444,7 +742,7 @@ hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hip
 
 template hipError_t launch_gemm_batched<double>(const GemmDesc<double>*, int, int, hipStream_t);
 template hipError_t launch_gemm_batched<float>(const GemmDesc<float>*, int, int, hipStream_t);
-template hipError_t launch_gj_inverse<double>(const InvDesc<double>*, int, int, hipStream_t);
-template hipError_t launch_gj_inverse<float>(const InvDesc<float>*, int, int, hipStream_t);
+template hipError_t launch_gj_inverse<double>(const InvDesc<double>*, int, int, int, hipStream_t);
+template hipError_t launch_gj_inverse<float>(const InvDesc<float>*, int, int, int, hipStream_t);
 
 }  // namespace kore

@@ -45,7 +45,9 @@ template <class T> struct SkinnyDesc {
 template <class T>
 hipError_t launch_gemm_batched(const GemmDesc<T>* d_descs, int ndesc, int max_tiles, hipStream_t s);
 template <class T>
-hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmax, hipStream_t s);
+// nmin / nmax: smallest / largest block size in the list; blocks of 96..512
+// rows take the multi-workgroup form (panel kernel + tiled MFMA update).
+hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmin, int nmax, hipStream_t s);
 template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s);
 // single product with the descriptor passed by value

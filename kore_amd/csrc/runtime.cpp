@@ -321,7 +321,7 @@ struct FactorBase {
 
 template <class T> struct Level {
   InvDesc<T>* inv = nullptr;
-  int ninv = 0, maxn_inv = 0;
+  int ninv = 0, maxn_inv = 0, minn_inv = 1 << 30;
   GemmDesc<T>* g1 = nullptr;
   int ng1 = 0, t1 = 0;
   GemmDesc<T>* g2 = nullptr;
@@ -546,6 +546,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       flops += 8.0 * nk(k) * nk(k) * nk(k);
       F->flops_inv += 8.0 * nk(k) * nk(k) * nk(k);
       L.maxn_inv = std::max(L.maxn_inv, (int)nk(k));
+      L.minn_inv = std::min(L.minn_inv, (int)nk(k));
     }
     rinv[l].n = vinv.size() - rinv[l].off;
     // g1: E_t = L_t Dinv_{t-1}, F_t = U_t Dinv_{t+1}
@@ -731,7 +732,7 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
     Level<T>& L = F->lev[l];
     {
       Timed tm(c, KORE_TIMER_INV);
-      HIP_TRY(launch_gj_inverse<T>(L.inv, L.ninv, L.maxn_inv, st));
+      HIP_TRY(launch_gj_inverse<T>(L.inv, L.ninv, L.minn_inv, L.maxn_inv, st));
     }
     {
       Timed tm(c, KORE_TIMER_GEMM);
@@ -744,7 +745,7 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
   }
   {
     Timed tm(c, KORE_TIMER_INV);
-    HIP_TRY(launch_gj_inverse<T>(F->root_inv, 1, F->root_n, st));
+    HIP_TRY(launch_gj_inverse<T>(F->root_inv, 1, F->root_n, F->root_n, st));
   }
   return KORE_OK;
 }
