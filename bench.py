@@ -135,7 +135,7 @@ def roofline(run, precision, workload=None, block=None):
         if cnt and ms > 0:
             tf = steps * flops / (ms * 1e-3) / 1e12
             cand[key] = (ms, dict(bound="mfma", achieved=tf, peak=peak, unit="TFLOP/s",
-                                  kernel="gj_inverse_kernel" if key == "inv" else "gemm_batched_kernel",
+                                  kernel="gj_inverse (panel + MFMA update kernels)" if key == "inv" else "gemm_batched_kernel",
                                   avg_launch_ms=ms / cnt))
     if not cand:
         return None, {}
@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--e-range", default="1e-5,1e-4")
     ap.add_argument("--n-cap", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--block", type=int, default=4)
+    ap.add_argument("--block", type=int, default=1)
     ap.add_argument("--ncv", type=int, default=None)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")

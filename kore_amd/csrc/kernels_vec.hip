@@ -172,7 +172,12 @@ template <int NR> struct SkLds {
   static constexpr int value = x > p ? x : p;
 };
 
-template <class T, int NR>
+// PRE (launches of few workgroups, where each product's load latency is the
+// cost): a product with K <= min(SK_PRE_K, chunk) has all of this thread's A entries and
+// its share of the X chunk requested before the first wait, so the product
+// costs one memory round trip instead of one per unrolled group.
+constexpr int SK_PRE_K = 256;
+template <class T, int NR, bool PRE>
 __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
   const int r0 = blockIdx.x * SK_ROWS;
@@ -193,6 +198,46 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
     if (A == nullptr || K <= 0) continue;
     const int64_t lda = d.lda[p], ldx = d.ldx[p];
     const T al = d.alpha[p];
+    if constexpr (PRE) {
+      constexpr int KP = SK_PRE_K < KC ? SK_PRE_K : KC;  // k entries staged at once
+      if (K <= KP) {                      // uniform
+        constexpr int NA = KP / SK_KS;                   // A entries per thread
+        constexpr int NXL = (KC * NR + 255) / 256;       // X entries per thread
+        cx<T> xr[NXL];
+#pragma unroll
+        for (int t = 0; t < NXL; ++t) {
+          const int idx = threadIdx.x + 256 * t;
+          const int kk = idx / NR, c = idx - kk * NR;
+          xr[t] = czero<T>();
+          if (idx < KC * NR && kk < K && c < nrhs) xr[t] = X[(int64_t)c * ldx + kk];
+        }
+        cx<T> a[NA];
+        const cx<T>* __restrict__ Ac = A + row;
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+          const int kk = ks + SK_KS * t;
+          a[t] = czero<T>();
+          if (row < d.m && kk < K) a[t] = Ac[(int64_t)kk * lda];
+        }
+        __syncthreads();                  // earlier readers of the X chunk are done
+#pragma unroll
+        for (int t = 0; t < NXL; ++t) {
+          const int idx = threadIdx.x + 256 * t;
+          if (idx < KC * NR) {
+            const int kk = idx / NR, c = idx - kk * NR;
+            sX[kk][c] = cx<T>{xr[t].re * al, xr[t].im * al};
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+          const int kk = ks + SK_KS * t;  // < KP <= KC; entries past K are zero on both sides
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a[t], sX[kk][c]);
+        }
+        continue;
+      }
+    }
     for (int kc0 = 0; kc0 < K; kc0 += KC) {
       __syncthreads();
       for (int idx = threadIdx.x; idx < KC * NR; idx += 256) {
@@ -257,15 +302,15 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   }
 }
 
-template <class T, int NR>
+template <class T, int NR, bool PRE>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
-  skinny_tile<T, NR>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
+  skinny_tile<T, NR, PRE>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
 }
 
 // one product, descriptor passed by value (no descriptor upload)
 template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny1_kernel(const SkinnyDesc<T> d, int nrhs) {
-  skinny_tile<T, NR>(d, nrhs);
+  skinny_tile<T, NR, false>(d, nrhs);
 }
 
 template <class T>
@@ -292,16 +337,30 @@ hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int
   if (ndesc <= 0 || max_m <= 0) return hipSuccess;
   if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((max_m + SK_ROWS - 1) / SK_ROWS), (unsigned)ndesc);
+  // fewer workgroups than compute units: latency-bound, take the preloading form
+  if ((int64_t)grid.x * grid.y < 256) {
+    if (nrhs == 1)
+      hipLaunchKernelGGL((skinny_kernel<T, 1, true>), grid, dim3(256), 0, s, d_descs, nrhs);
+    else if (nrhs <= 2)
+      hipLaunchKernelGGL((skinny_kernel<T, 2, true>), grid, dim3(256), 0, s, d_descs, nrhs);
+    else if (nrhs <= 4)
+      hipLaunchKernelGGL((skinny_kernel<T, 4, true>), grid, dim3(256), 0, s, d_descs, nrhs);
+    else if (nrhs <= 8)
+      hipLaunchKernelGGL((skinny_kernel<T, 8, true>), grid, dim3(256), 0, s, d_descs, nrhs);
+    else
+      hipLaunchKernelGGL((skinny_kernel<T, 16, true>), grid, dim3(256), 0, s, d_descs, nrhs);
+    return hipGetLastError();
+  }
   if (nrhs == 1)
-    hipLaunchKernelGGL((skinny_kernel<T, 1>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 1, false>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 2)
-    hipLaunchKernelGGL((skinny_kernel<T, 2>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 2, false>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 4)
-    hipLaunchKernelGGL((skinny_kernel<T, 4>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 4, false>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 8)
-    hipLaunchKernelGGL((skinny_kernel<T, 8>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 8, false>), grid, dim3(256), 0, s, d_descs, nrhs);
   else
-    hipLaunchKernelGGL((skinny_kernel<T, 16>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 16, false>), grid, dim3(256), 0, s, d_descs, nrhs);
   return hipGetLastError();
 }
 template hipError_t launch_skinny<double>(const SkinnyDesc<double>*, int, int, int, hipStream_t);

@@ -26,6 +26,7 @@ wanted order, consecutively, as SLEPc does.
 from __future__ import annotations
 
 import dataclasses
+import threading
 import time
 
 import numpy as np
@@ -165,6 +166,42 @@ def _triu_eigvecs(S):
     return Z / np.linalg.norm(Z, axis=0)
 
 
+@dataclasses.dataclass
+class _Ritz:
+    """Rayleigh-Ritz data of the decomposition of size j (wanted order)."""
+    j: int
+    S: np.ndarray
+    Q: np.ndarray
+    Bres: np.ndarray
+    order: np.ndarray
+    theta: np.ndarray
+    Y: np.ndarray
+    res: np.ndarray
+    nconv: int
+
+
+def _rayleigh_ritz(H, j, bs, sigma, which, tol, host):
+    """Ordered Schur form of T = H[:j, :j] (host, FP64, LAPACK), Ritz vectors
+    and residual estimates ||Bres y|| with Bres = H[j:j+bs, :j]."""
+    T = H[:j, :j]
+    Bres = H[j:j + bs, :j]
+    tq = time.perf_counter()
+    S, Q = sla.schur(T, output="complex")
+    host["schur"] = host.get("schur", 0.0) + time.perf_counter() - tq
+    theta_s = np.diag(S).copy()
+    Ys = Q @ _triu_eigvecs(S)
+    res_s = np.linalg.norm(Bres @ Ys, axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):     # theta = 0 (null space of B) sorts last
+        order = np.argsort(sort_key(which, sigma + 1.0 / theta_s, sigma), kind="stable")
+    theta = theta_s[order]
+    res = res_s[order]
+    conv = res <= tol * np.abs(theta)
+    nconv = 0
+    while nconv < j and conv[nconv]:
+        nconv += 1
+    return _Ritz(j, S, Q, Bres, order, theta, Ys[:, order], res, nconv)
+
+
 def eigs(*args, **kwargs) -> EigResult:
     """See ``_eigs``.  The projected problem is tiny (at most a few hundred
     columns): multithreaded BLAS only adds start-up cost to each call, so the
@@ -177,27 +214,34 @@ def eigs(*args, **kwargs) -> EigResult:
         return _eigs(*args, **kwargs)
 
 
-def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=4, seed=0,
-          ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None, fallback_sweeps=4) -> EigResult:
+def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=1, seed=0,
+          ir_tol=1e-13, ir_maxit=12, v0=None, log=None, workspace=None, fallback_sweeps=4,
+          check_every=None) -> EigResult:
     """fallback_sweeps: with a complex64 factor, if one OP application needs
     more than this many refinement sweeps (or does not converge), the system
     is refactored in FP64 at the same shift and the iteration restarts: every
-    OP application costs ~one FP64 sweep instead of many complex64 ones."""
+    OP application costs ~one FP64 sweep instead of many complex64 ones.
+
+    check_every: columns between the early convergence checks made while the
+    basis grows (0 disables them; default max(8, 2 bs)).  Each check is a
+    Rayleigh-Ritz step on the decomposition reached so far, computed on the
+    host while the device expands the next columns; when the wanted pairs
+    have converged the iteration ends there, without expanding to ncv."""
     res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
-                     fallback_sweeps if sysm.precision != PREC_FP64 else None)
+                     fallback_sweeps if sysm.precision != PREC_FP64 else None, check_every)
     if res is None:
         if log:
             log("complex64 refinement too slow: refactoring in FP64")
         sysm.factor(sigma, PREC_FP64)
         res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log,
-                         workspace, None)
+                         workspace, None, check_every)
         res.fallback = True
     res.precision = sysm.precision
     return res
 
 
 def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
-               fallback_sweeps):
+               fallback_sweeps, check_every):
     """Eigenpairs nearest ``sigma`` of the pencil held by ``sysm``.
 
     ``sysm`` must have been factored at ``sigma`` (System.factor).
@@ -210,10 +254,14 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     n = sysm.n
     bs = max(1, min(int(block), 16))
     if ncv is None:
-        # a block space needs more columns to reach the same polynomial degree;
-        # 16 blocks of 4 gave the fewest device sweeps plus host Schur time
-        # on the 60 000-unknown m=1, E=1e-6 inertial-mode problem (nev=10)
-        ncv = max(2 * nev, nev + 15, 16 * bs)
+        # with the early convergence checks a large space costs nothing until
+        # it is needed and saves restarts (each restart is a serial host Schur
+        # step).  Measured on the 60 000-unknown m=1, E=1e-6 inertial-mode
+        # problem (nev=10): bs=1 converges after ~80 OP columns, bs=2 after
+        # ~120 in ~60 sweeps, bs=4 after ~230 in ~58 sweeps plus 3 restarts;
+        # with the sweep cost nearly flat in bs, bs=1 with 96 columns was the
+        # fastest (18 solves/s against 16 for bs=2 and 11 for bs=4)
+        ncv = max(2 * nev, nev + 15, 96)
     ncv = max(ncv, nev + 2 * bs)
     ncv = int(np.ceil(ncv / bs) * bs)
     ncv = min(ncv, max(bs, (n // bs) * bs - bs))
@@ -252,11 +300,64 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     if expand is None:
         def expand(j0, j1, bs_, cw, Hm, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
             return expand_host(ws, j0, j1, bs_, cw, Hm, ir_tol, ir_maxit, ir_stop)
+    if check_every is None:
+        check_every = max(8, 2 * bs)
+    check_every = int(np.ceil(check_every / bs) * bs) if check_every > 0 else 0
+    j_first = int(np.ceil((nev + bs) / bs) * bs)    # fewest columns worth a check
+    j_base = j                                      # columns of the last restart (no check there)
+    rr = None
+    t_col = None                                    # device wall time per OP column (last chunk)
+    t_rr, j_rr = None, None                         # last check's host time and size
+    hidden = {}                                     # host time spent beside the device
     while True:
-        # ---- expand the decomposition to m columns
+        # ---- expand the decomposition to m columns; from j_first on, each
+        # chunk of check_every columns is expanded on the device (worker
+        # thread: the library call releases the GIL) while the host runs the
+        # Rayleigh-Ritz step of the decomposition reached before the chunk
+        rr = None
         while j < m:
-            st, jd, ws_info = expand(j, m, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
-                                     ir_stop=fallback_sweeps or 0)
+            j_stop, early = m, None
+            can_check = bool(check_every) and j >= j_first and j > j_base
+            threaded = can_check and j + check_every < m
+            if threaded:
+                # a chunk long enough to cover the check (Schur cost ~ j^3)
+                chunk = check_every
+                if t_col and t_rr:
+                    chunk = max(chunk, int(np.ceil(t_rr * (j / j_rr) ** 3 / t_col / bs)) * bs)
+                j_stop = min(m, j + chunk)
+            elif check_every and not can_check:
+                j_stop = min(m, max(j_first, j + check_every))   # grow to where a check is worth it
+            if threaded:
+                Hs = np.asfortranarray(H[:j + bs, :j])
+                out = {}
+
+                def work(j0=j, j1=j_stop):
+                    try:
+                        t = time.perf_counter()
+                        out["r"] = expand(j0, j1, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
+                                          ir_stop=fallback_sweeps or 0)
+                        out["t"] = time.perf_counter() - t
+                    except BaseException as e:          # re-raised on the calling thread
+                        out["e"] = e
+                th = threading.Thread(target=work)
+                th.start()
+                try:
+                    t = time.perf_counter()
+                    early = _rayleigh_ritz(Hs, j, bs, sigma, which, tol, hidden)
+                    t_rr, j_rr = time.perf_counter() - t, j
+                finally:
+                    th.join()
+                if "e" in out:
+                    raise out["e"]
+                st, jd, ws_info = out["r"]
+                if jd > j:
+                    t_col = out["t"] / (jd - j)
+            else:
+                t = time.perf_counter()
+                st, jd, ws_info = expand(j, j_stop, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
+                                         ir_stop=fallback_sweeps or 0)
+                if jd > j:
+                    t_col = (time.perf_counter() - t) / (jd - j)
             ir_iters = max(ir_iters, ws_info.iterations)
             done_blocks = (jd - j) // bs
             if st == KORE_WARN_IR_NOT_CONVERGED and fallback_sweeps is not None:
@@ -267,6 +368,9 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
                 done_blocks += 1
             op_calls += done_blocks
             op_cols += bs * done_blocks
+            if early is not None and early.nconv >= nev:
+                rr = early                              # converged before the chunk
+                break
             if st == KORE_WARN_BREAKDOWN:
                 # invariant subspace: continue with a fresh random block
                 breakdowns += 1
@@ -281,26 +385,12 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
                 jd = nb_all
             j = jd
         # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
-        T = H[:m, :m]
-        Bres = H[m:m + bs, :m]
-        tq = time.perf_counter()
-        S, Q = sla.schur(T, output="complex")
-        host["schur"] = host.get("schur", 0.0) + time.perf_counter() - tq
-        theta_s = np.diag(S).copy()
-        Zs = _triu_eigvecs(S)
-        Ys = Q @ Zs
-        res_s = np.linalg.norm(Bres @ Ys, axis=0)
-        order = np.argsort(sort_key(which, sigma + 1.0 / theta_s, sigma), kind="stable")
-        theta = theta_s[order]
-        Y = Ys[:, order]
-        res = res_s[order]
-        conv = res <= tol * np.abs(theta)
-        nconv = 0
-        while nconv < m and conv[nconv]:
-            nconv += 1
+        if rr is None:
+            rr = _rayleigh_ritz(H, m, bs, sigma, which, tol, host)
+        nconv = rr.nconv
         if log:
-            log("restart %d: nconv %d, worst wanted residual %.3e" % (restarts, nconv,
-                float(np.max(res[:nev] / np.abs(theta[:nev])))))
+            log("restart %d: size %d, nconv %d, worst wanted residual %.3e" % (
+                restarts, rr.j, nconv, float(np.max(rr.res[:nev] / np.abs(rr.theta[:nev])))))
         if nconv >= nev or restarts >= maxit:
             break
         # ---- thick restart: keep the 'keep' best Ritz values' invariant subspace
@@ -309,9 +399,9 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
         keep = int(np.ceil(keep / bs) * bs)       # the expansion then ends exactly at m
         keep = max(bs, min(keep, m - bs))
         select = np.zeros(m, dtype=np.int32)
-        select[order[:keep]] = 1
+        select[rr.order[:keep]] = 1
         tq = time.perf_counter()
-        St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, S, Q, job="N")
+        St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, rr.S, rr.Q, job="N")
         host["reorder"] = host.get("reorder", 0.0) + time.perf_counter() - tq
         if info != 0 or msel != keep:
             raise RuntimeError("Schur reordering failed (info %d, %d of %d)" % (info, msel, keep))
@@ -320,22 +410,25 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
         ws.copy(m, keep, bs)
         Hn = np.zeros_like(H)
         Hn[:keep, :keep] = np.triu(St[:keep, :keep])
-        Hn[keep:keep + bs, :keep] = Bres @ Qt[:, :keep]
+        Hn[keep:keep + bs, :keep] = rr.Bres @ Qt[:, :keep]
         H = Hn
         j = keep
+        j_base = keep
         restarts += 1
     k = nconv
     X = np.zeros((n, 0), dtype=complex)
     if k > 0:
-        ws.gemm(cT, k, 0, m, Y[:, :k], alpha=1.0, beta=0.0)
+        ws.gemm(cT, k, 0, rr.j, rr.Y[:, :k], alpha=1.0, beta=0.0)
         X = ws.get(cT, k)
         X = X / np.linalg.norm(X, axis=0)
-    lam = sigma + 1.0 / theta[:k]
+    lam = sigma + 1.0 / rr.theta[:k]
     if own_ws:
         ws.close()
     total = time.perf_counter() - t0
     host["projected"] = total - sum(host.values())
-    return EigResult(lam=lam, X=X, errest=res[:k] / np.abs(theta[:k]), nconv=k, restarts=restarts,
+    if hidden:
+        host["checks_beside_device"] = sum(hidden.values())
+    return EigResult(lam=lam, X=X, errest=rr.res[:k] / np.abs(rr.theta[:k]), nconv=k, restarts=restarts,
                      op_columns=op_cols, op_calls=op_calls, seconds=total,
                      ir_max_iterations=ir_iters, host_seconds=host, breakdowns=breakdowns)
 

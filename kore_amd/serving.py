@@ -109,14 +109,14 @@ def rhs_for(req: Request, n: int) -> np.ndarray:
     return b / np.linalg.norm(b)
 
 
-def footprint(par: Params, precision: int, nev: int = 6, block: int = 4) -> int:
+def footprint(par: Params, precision: int, nev: int = 6, block: int = 1) -> int:
     """Predicted device bytes of one request (factor + system + workspace)."""
     lay = backend.layout_from_params(par)
     fb = backend.factor_bytes(lay, precision)
     n = par.sizmat
     nnz_est = 48 * n
     sysb = nnz_est * (4 + 32) + n * (8 + 4 + 8) + 4 * n * 16 * 16
-    ncv = max(2 * nev, nev + 15, 16 * block)
+    ncv = max(2 * nev, nev + 15, 96)
     wsb = n * (2 * ncv + 4 * block) * 16
     return int(fb + sysb + wsb)
 
@@ -157,7 +157,7 @@ def residual_device(S, req: Request, x=None, lam=None, X=None, b=None):
 class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
-    def __init__(self, device=0, slots=4, mem_budget=200 << 30, precision=backend.PREC_C64, block=4,
+    def __init__(self, device=0, slots=4, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60):
         self.device = device

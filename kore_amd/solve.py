@@ -147,7 +147,7 @@ def main(argv=None):
     if st != "sinvert":
         raise SystemExit("kore_amd: -st_type %s is not implemented (only sinvert)" % st)
     prec = {"fp64": backend.PREC_FP64, "c64": backend.PREC_C64}[(opts.get("kore_precision") or "fp64").lower()]
-    block = int(opts.get("kore_block") or 4)
+    block = int(opts.get("kore_block") or 1)
     device = int(opts.get("kore_device") or 0)
 
     A = load_csr("A.npz")

@@ -149,6 +149,29 @@ def test_restart_path_is_exercised():
         assert abs(r1.lam[i] - r2.lam[i]) / abs(r2.lam[i]) <= 1e-10
 
 
+def test_early_convergence_checks():
+    """The Rayleigh-Ritz checks made while the basis grows (host step beside
+    the device expansion) stop the iteration as soon as the wanted pairs have
+    converged: fewer OP applications than a full expansion to ncv, the same
+    eigenvalues as dense QZ, and residuals at the tolerance."""
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.9j
+    w = la.eig(A.toarray(), B.toarray(), right=False)
+    w = w[np.isfinite(w)]
+    w = w[np.argsort(np.abs(w - sigma))]
+    S = NumpySystem(A, B, sigma)
+    full = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=2, check_every=0, workspace=NumpyWorkspace)
+    early = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=2, workspace=NumpyWorkspace)
+    assert full.op_columns == 120 and full.restarts == 0
+    assert early.op_columns < full.op_columns and early.restarts == 0
+    for r in (full, early):
+        assert r.nconv >= 4
+        for i in range(4):
+            assert abs(r.lam[i] - w[i]) / abs(w[i]) <= 1e-10
+        assert eigs.residuals(A, B, r.lam[:4], r.X[:, :4]).max() <= 1e-12
+
+
 def test_triangular_eigenvectors():
     """S z_j = d_j z_j for every column, unit norm, and the same directions as
     LAPACK's eigenvectors of the triangular matrix (equal up to a phase)."""
