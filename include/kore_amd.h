@@ -181,7 +181,7 @@ int32_t kore_ws_copy(kore_ws* ws, int32_t src, int32_t dst, int32_t nc);
  * blocks of bs: for j = j0, j0+bs, ...: W = OP V[:, j:j+bs] (in the scratch
  * block V[:, cw:cw+bs], cw >= j1+bs), two passes of classical Gram-Schmidt of
  * W against V[:, 0:j+bs], two passes of Cholesky QR giving V[:, j+bs:j+2bs] R
- * = W.  H (host, column-major, ldh >= j1+bs) receives the projected columns:
+ * = W (one pass when bs = 1: the normalised column).  H (host, column-major, ldh >= j1+bs) receives the projected columns:
  * H[0:j+bs, j:j+bs] = Gram-Schmidt coefficients, H[j+bs:j+2bs, j:j+bs] = R.
  * ir_tol / ir_maxit as in kore_ws_apply_op; ir_stop > 0 ends the expansion
  * when one block needs more than ir_stop refinement sweeps or does not

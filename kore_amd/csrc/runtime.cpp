@@ -1735,11 +1735,12 @@ int32_t kore_ws_copy(kore_ws* w, int32_t src, int32_t dst, int32_t nc) {
 
 // Block Arnoldi expansion with everything but the bookkeeping on the device:
 // per block, OP, two classical Gram-Schmidt passes and two Cholesky-QR
-// passes.  The coefficients of block j travel to a pinned slot behind an
+// passes (one for a single column).  The coefficients of block j travel to a pinned slot behind an
 // event; block j+1 is queued before the host waits for block j, so the device
 // never drains between blocks.  A breakdown found at block j discards block
 // j+1 (everything it wrote lies in columns the recovery overwrites).
 static int32_t ws_expand_finish(kore_ws* w, int j, int bs, int slot, double* H, int64_t ldh, bool* breakdown) {
+  const bool one_pass = bs == 1;   // a single column needs one Cholesky-QR pass (R2 = 1)
   const int ncol = w->ncol;
   HIP_TRY(hipEventSynchronize(w->hev[slot]));
   const zd* h1 = reinterpret_cast<const zd*>(w->hh) + (size_t)slot * ws_hb_elems(ncol);
@@ -1751,8 +1752,12 @@ static int32_t ws_expand_finish(kore_ws* w, int j, int bs, int slot, double* H, 
   zd* Hz = reinterpret_cast<zd*>(H);
   for (int q = 0; q < bs; ++q)
     for (int i = 0; i < nb; ++i) Hz[(size_t)(j + q) * ldh + i] = h1[(size_t)q * nb + i] + h2[(size_t)q * nb + i];
-  *breakdown = flag[0] || flag[1];
+  *breakdown = flag[0] || (!one_pass && flag[1]);
   if (*breakdown) return KORE_OK;
+  if (one_pass) {
+    Hz[(size_t)j * ldh + nb] = R1[0];
+    return KORE_OK;
+  }
   // R = R2 R1 (both upper triangular, ld bs)
   for (int q = 0; q < bs; ++q)
     for (int i = 0; i < bs; ++i) {
@@ -1820,14 +1825,18 @@ int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t c
     TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h1, nb, -1.0, 1.0));
     TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h2));
     TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h2, nb, -1.0, 1.0));
-    // two passes of Cholesky QR: V[:, nb:nb+bs] (R2 R1) = W
+    // Cholesky QR: V[:, nb:nb+bs] (R2 R1) = W, two passes for a block; a
+    // single column is normalised once (its norm is already exact to
+    // rounding, so a second pass would only return R2 = 1)
     TRY(ws_dot_dev(w, cw, bs, cw, bs, d_G));
     HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R1, d_Ri1, d_flag, c->stream));
     TRY(ws_gemm_dev(w, nb, bs, cw, bs, d_Ri1, bs, 1.0, 0.0));
-    TRY(ws_dot_dev(w, nb, bs, nb, bs, d_G));
-    HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R2, d_Ri2, d_flag + 1, c->stream));
-    TRY(ws_gemm_dev(w, cw, bs, nb, bs, d_Ri2, bs, 1.0, 0.0));
-    HIP_TRY(launch_copy_cols(wcol(w, cw), s->n, wcol(w, nb), s->n, s->n, bs, c->stream));
+    if (bs > 1) {
+      TRY(ws_dot_dev(w, nb, bs, nb, bs, d_G));
+      HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R2, d_Ri2, d_flag + 1, c->stream));
+      TRY(ws_gemm_dev(w, cw, bs, nb, bs, d_Ri2, bs, 1.0, 0.0));
+      HIP_TRY(launch_copy_cols(wcol(w, cw), s->n, wcol(w, nb), s->n, s->n, bs, c->stream));
+    }
     HIP_TRY(hipMemcpyAsync(w->hh + (size_t)slot * ws_hb_elems(ncol), d_h1, ws_hb_elems(ncol) * sizeof(zd),
                            hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipEventRecord(w->hev[slot], c->stream));

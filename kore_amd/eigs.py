@@ -96,11 +96,14 @@ class _Timed:
 
 def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
     """Orthonormalise V[:, src:src+nc] into V[:, dst:dst+nc]; returns R with
-    V_src = V_dst R.  Two Cholesky-QR passes (the second in place via src)."""
+    V_src = V_dst R.  Two Cholesky-QR passes (the second in place via src);
+    a single column is normalised once, as kore_ws_expand does."""
     G = ws.dot(src, nc, src, nc)
     G = 0.5 * (G + G.conj().T)
     R1 = sla.cholesky(G, lower=False)
     ws.gemm(dst, nc, src, nc, sla.solve_triangular(R1, np.eye(nc), lower=False), alpha=1.0, beta=0.0)
+    if nc == 1:
+        return R1
     G2 = ws.dot(dst, nc, dst, nc)
     G2 = 0.5 * (G2 + G2.conj().T)
     R2 = sla.cholesky(G2, lower=False)
@@ -112,7 +115,8 @@ def _cholqr(ws: Workspace, src: int, dst: int, nc: int):
 def expand_host(ws, j0, j1, bs, cw, H, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
     """Block Arnoldi expansion driven from the host through the workspace
     primitives; the same contract and arithmetic as kore_ws_expand (OP, two
-    classical Gram-Schmidt passes, two Cholesky-QR passes per block).
+    classical Gram-Schmidt passes, two Cholesky-QR passes per block, one for
+    a single column).
     Returns (status, j_done, info)."""
     iters, conv = 0, True
     for j in range(j0, j1, bs):
