@@ -40,6 +40,10 @@ __global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* __
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int i0 = (tile % d.tiles_m) * GT_M;
     const int j0 = (tile / d.tiles_m) * GT_N;
+    // 16 x 16 sub-tiles wholly outside C (edge tiles of sizes that are not
+    // multiples of 64, e.g. 200 = 3*64 + 8) are skipped, wave-uniformly
+    const bool rv0 = i0 + wr < d.m, rv1 = i0 + wr + 16 < d.m;
+    const bool cv0 = j0 + wc < d.n, cv1 = j0 + wc + 16 < d.n;
     A4 cre[2][2], cim[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -49,67 +53,87 @@ __global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* __
         cim[a][b] = A4{0, 0, 0, 0};
       }
 
-    for (int p = 0; p < 2; ++p) {
-      const cx<T>* __restrict__ A = d.A[p];
-      const cx<T>* __restrict__ B = d.B[p];
-      const int K = d.k[p];
-      if (A == nullptr || K <= 0) continue;
-      const int64_t lda = d.lda[p], ldb = d.ldb[p];
-      for (int k0 = 0; k0 < K; k0 += GT_K) {
-        __syncthreads();
-        // A slab -> sA[k][i]
-        if (d.opA == 0) {
-          const int kk = tid >> 4, ii = (tid & 15) * 4;
-          const int gk = k0 + kk;
+    // Slabs of 16 k-columns over the (up to two) products, register
+    // double-buffered: slab s+1 is requested from memory before slab s is
+    // multiplied, so the load latency hides behind the matrix-core work.
+    cx<T> ra[4], rb[4];
+    auto load = [&](int p, int k0) {      // selects, not indexing: d stays in registers
+      const cx<T>* __restrict__ A = p ? d.A[1] : d.A[0];
+      const cx<T>* __restrict__ B = p ? d.B[1] : d.B[0];
+      const int K = p ? d.k[1] : d.k[0];
+      const int64_t lda = p ? d.lda[1] : d.lda[0], ldb = p ? d.ldb[1] : d.ldb[0];
+      if (d.opA == 0) {        // A slab element (k = tid/16, i = 4 (tid%16) + q)
+        const int gk = k0 + (tid >> 4), ii = (tid & 15) * 4;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int gi = i0 + ii + q;
-            cx<T> v = czero<T>();
-            if (gk < K && gi < d.m) v = A[(int64_t)gk * lda + gi];
-            sA[kk][ii + q] = v;
-          }
-        } else {  // A_eff(i,k) = conj(A_store(k,i)), A_store column-major K x m
-          const int ii = tid >> 2, kk = (tid & 3) * 4;
-          const int gi = i0 + ii;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int gk = k0 + kk + q;
-            cx<T> v = czero<T>();
-            if (gk < K && gi < d.m) {
-              v = A[(int64_t)gi * lda + gk];
-              v.im = -v.im;
-            }
-            sA[kk + q][ii] = v;
-          }
+        for (int q = 0; q < 4; ++q) {
+          const int gi = i0 + ii + q;
+          ra[q] = czero<T>();
+          if (gk < K && gi < d.m) ra[q] = A[(int64_t)gk * lda + gi];
         }
-        // B slab -> sB[j][k]
-        {
-          const int jj = tid >> 2, kk = (tid & 3) * 4;
-          const int gj = j0 + jj;
+      } else {                 // A_eff(i,k) = conj(A_store(k,i)), A_store column-major K x m
+        const int gi = i0 + (tid >> 2), kk = (tid & 3) * 4;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int gk = k0 + kk + q;
-            cx<T> v = czero<T>();
-            if (gk < K && gj < d.n) v = B[(int64_t)gj * ldb + gk];
-            sB[jj][kk + q] = v;
+        for (int q = 0; q < 4; ++q) {
+          const int gk = k0 + kk + q;
+          ra[q] = czero<T>();
+          if (gk < K && gi < d.m) {
+            ra[q] = A[(int64_t)gi * lda + gk];
+            ra[q].im = -ra[q].im;
           }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GT_K; kk += 4) {
-          cx<T> a[2], b[2];
-#pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            a[s] = sA[kk + lq][wr + 16 * s + l16];
-            b[s] = sB[wc + 16 * s + l16][kk + lq];
-          }
-#pragma unroll
-          for (int rs = 0; rs < 2; ++rs)
-#pragma unroll
-            for (int cs = 0; cs < 2; ++cs) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
         }
       }
+      const int gj = j0 + (tid >> 2), kk = (tid & 3) * 4;   // B slab element (j, k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gk = k0 + kk + q;
+        rb[q] = czero<T>();
+        if (gk < K && gj < d.n) rb[q] = B[(int64_t)gj * ldb + gk];
+      }
+    };
+    auto valid = [&](int p) { return d.A[p] != nullptr && d.k[p] > 0; };
+    int p = valid(0) ? 0 : (valid(1) ? 1 : 2), k0 = 0;
+    if (p < 2) load(p, 0);
+    while (p < 2) {
+      __syncthreads();                      // the previous slab has been consumed
+      if (d.opA == 0) {
+        const int kk = tid >> 4, ii = (tid & 15) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sA[kk][ii + q] = ra[q];
+      } else {
+        const int ii = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sA[kk + q][ii] = ra[q];
+      }
+      {
+        const int jj = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sB[jj][kk + q] = rb[q];
+      }
+      __syncthreads();
+      int pn = p, kn = k0 + GT_K;           // next slab
+      if (kn >= (p ? d.k[1] : d.k[0])) {
+        pn = (p == 0 && valid(1)) ? 1 : 2;
+        kn = 0;
+      }
+      if (pn < 2) load(pn, kn);
+#pragma unroll
+      for (int kk = 0; kk < GT_K; kk += 4) {
+        cx<T> a[2], b[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          a[s] = sA[kk + lq][wr + 16 * s + l16];
+          b[s] = sB[wc + 16 * s + l16][kk + lq];
+        }
+#pragma unroll
+        for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+          for (int cs = 0; cs < 2; ++cs)
+            if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
+      }
+      p = pn;
+      k0 = kn;
     }
+    __syncthreads();                        // LDS is reused by the next tile
     // epilogue: lane holds C[i0+wr+16rs+l16][j0+wc+16cs+mfma_mrow(lq,r)]
 #pragma unroll
     for (int rs = 0; rs < 2; ++rs)
@@ -601,6 +625,8 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, lq = lane >> 4;
+  const bool rv0 = i0 + wr < n, rv1 = i0 + wr + 16 < n;    // sub-tiles inside the block
+  const bool cv0 = j0 + wc < n, cv1 = j0 + wc + 16 < n;
   for (int idx = tid; idx < WP * 64; idx += 256) {
     const int k = idx / 64, i = idx % 64;           // consecutive threads: consecutive rows
     cx<T> x = czero<T>();
@@ -634,7 +660,8 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
 #pragma unroll
     for (int rs = 0; rs < 2; ++rs)
 #pragma unroll
-      for (int cs = 0; cs < 2; ++cs) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
+      for (int cs = 0; cs < 2; ++cs)
+        if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
   }
 #pragma unroll
   for (int rs = 0; rs < 2; ++rs)

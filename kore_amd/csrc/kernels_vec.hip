@@ -26,9 +26,13 @@ __global__ __launch_bounds__(256) void spmv_kernel(CsrDev m, cx<double> ca, cx<d
   cx<double> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = czero<double>();
+  // a zero coefficient skips its value array (B x and A x read one of the two)
+  const bool use_a = ca.re != 0.0 || ca.im != 0.0, use_b = cb.re != 0.0 || cb.im != 0.0;
   for (int k = s + gl; k < e; k += 16) {
     const int j = m.indices[k];
-    const cx<double> v = cadd(cmul(ca, m.va[k]), cmul(cb, m.vb[k]));
+    cx<double> v = czero<double>();
+    if (use_a) v = cmul(ca, m.va[k]);
+    if (use_b) v = cadd(v, cmul(cb, m.vb[k]));
 #pragma unroll
     for (int c = 0; c < NR; ++c)
       if (c < nrhs) acc[c] = cfma(acc[c], v, X[(int64_t)c * ldx + j]);

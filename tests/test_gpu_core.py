@@ -97,6 +97,25 @@ def test_solve_config1(ctx, config1, precision):
     assert err.max() <= 1e-10
 
 
+@pytest.mark.parametrize("case", ["hydro_N100", "hydro_N200", "thermal_N60", "hydro_N300"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_large_blocks(ctx, case, precision):
+    """Degree blocks of 96-512 rows take the multi-workgroup inversion (panel
+    kernel + tiled MFMA update) and GEMM edge tiles of sizes that are not
+    multiples of 64: 100, 200, 300, and thermal 120 / 60 in one level list.
+    Solves match SuperLU to 1e-10."""
+    from kore_amd.params import Params
+    from kore_amd import assembly
+    kw = dict(hydro_N100=dict(N=100, lmax=14), hydro_N200=dict(N=200, lmax=10), hydro_N300=dict(N=300, lmax=8),
+              thermal_N60=dict(N=60, lmax=16, thermal=1, Prandtl=1.0, Ra_gap=1e5))[case]
+    par = Params(m=1, symm=1, Ek=1e-4, **kw)
+    A, B, _ = assembly.assemble(par)
+    say("case: large-block solve", case, "sizmat", A.shape[0], "precision", precision)
+    err, info = _solve_case(ctx, par, A, B, 0.4j, precision, nrhs=2, seed=7)
+    say("  rel err", err, "sweeps", info.iterations)
+    assert err.max() <= 1e-10
+
+
 def test_solve_many_rhs(ctx, config1):
     par, A, B = config1
     say("case: config1 solve 21 rhs fp64")
