@@ -471,7 +471,11 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   constexpr int NT = 512, NW = NT / 64, WP = GJM_WP;
   constexpr int TPR = NT / NMAX, CPT = WP / TPR;
   static_assert(TPR >= 1 && CPT * TPR == WP, "panel split");
-  __shared__ cx<T> sCand[2][NW][WP];
+  // candidate rows; the two column slices of a row are one complex apart
+  // beyond their natural offset, so that the two slices of a wave's pivot-row
+  // reads fall in different LDS banks
+  constexpr int SL = CPT + (TPR > 1 ? 1 : 0);
+  __shared__ cx<T> sCand[2][NW][TPR * SL];
   __shared__ unsigned long long sKey[2][NW];
   __shared__ int sPiv[WP];
   __shared__ int sSing;
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
       key = wave_max_u64(key);
       if (key != 0 && r == 511 - (int)(key & 0x1FFULL)) {
 #pragma unroll
-        for (int jj = 0; jj < CPT; ++jj) sCand[par][wave][h * CPT + jj] = v[jj];
+        for (int jj = 0; jj < CPT; ++jj) sCand[par][wave][h * SL + jj] = v[jj];
       }
       if (lane == 0) sKey[par][wave] = key;
       __syncthreads();
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
       if (tid == 0) sPiv[c] = p;
       if (r == p) role = c;
       const cx<T>* __restrict__ prow = sCand[par][q];
-      cx<T> pv = prow[c];
+      cx<T> pv = prow[(c / CPT) * SL + c % CPT];
       T den = pv.re * pv.re + pv.im * pv.im;
       if (!(den > T(0))) {
         if (tid == 0 && sSing == 0) sSing = p0 + c + 1;
@@ -561,7 +565,7 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
         const cx<T> l = cmul(vc, ipv), nl{-l.re, -l.im};
 #pragma unroll
         for (int jj = 0; jj < CPT; ++jj) {
-          const cx<T> u = prow[h * CPT + jj];
+          const cx<T> u = prow[h * SL + jj];
           v[jj] = (jj == c % CPT && h == c / CPT) ? nl : cfma(v[jj], nl, u);   // four FMAs
         }
       }
