@@ -691,6 +691,47 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(const cx<double>* __restri
   if (tid == 0) *flag = fail ? 1 : 0;
 }
 
+// Normalisation of one column from the split-K partials of its Gram value
+// (tsdot partials, nparts x 2 doubles): every workgroup sums the partials in
+// the same fixed order, then scales its rows, v = w / ||w||; workgroup 0
+// writes R = ||w||, Rinv = 1/||w|| and *flag (1 when ||w||^2 is not positive,
+// with R = Rinv = 1, as chol_inv_kernel does for a 1 x 1 block).
+__global__ __launch_bounds__(256) void normalize1_kernel(const double* __restrict__ part, int nparts,
+                                                         const cx<double>* __restrict__ w, cx<double>* __restrict__ v,
+                                                         int64_t n, cx<double>* __restrict__ R,
+                                                         cx<double>* __restrict__ Rinv, int* __restrict__ flag) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) acc += part[2 * (int64_t)p];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double g = red[0];
+  const bool fail = !(g > 0.0);
+  const double r = fail ? 1.0 : sqrt(g), ri = fail ? 1.0 : 1.0 / r;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const cx<double> x = w[i];
+    v[i] = cx<double>{x.re * ri, x.im * ri};
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    R[0] = cx<double>{r, 0.0};
+    Rinv[0] = cx<double>{ri, 0.0};
+    *flag = fail ? 1 : 0;
+  }
+}
+
+hipError_t launch_normalize1(const double* part, int nparts, const cx<double>* w, cx<double>* v, int64_t n,
+                             cx<double>* R, cx<double>* Rinv, int* flag, hipStream_t s) {
+  if (n <= 0 || nparts <= 0) return hipErrorInvalidValue;
+  const int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(normalize1_kernel, dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(256), 0, s, part, nparts, w, v,
+                     n, R, Rinv, flag);
+  return hipGetLastError();
+}
+
 hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, cx<double>* Rinv, int* flag,
                            hipStream_t s) {
   if (n < 1 || n > 16) return hipErrorInvalidValue;

@@ -104,6 +104,8 @@ hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, 
 // partial[b] (b < nblocks) = slice sums of |va - sigma vb|^2 over nnz entries
 hipError_t launch_csr_fro2(const cx<double>* va, const cx<double>* vb, int64_t nnz, double sre, double sim,
                            double* partial, int nblocks, hipStream_t s);
+hipError_t launch_normalize1(const double* part, int nparts, const cx<double>* w, cx<double>* v, int64_t n,
+                             cx<double>* R, cx<double>* Rinv, int* flag, hipStream_t s);
 hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, cx<double>* Rinv, int* flag,
                            hipStream_t s);
 // fill complex buffer with zeros (graph-capture-safe clear)

@@ -1828,9 +1828,22 @@ int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t c
     // Cholesky QR: V[:, nb:nb+bs] (R2 R1) = W, two passes for a block; a
     // single column is normalised once (its norm is already exact to
     // rounding, so a second pass would only return R2 = 1)
-    TRY(ws_dot_dev(w, cw, bs, cw, bs, d_G));
-    HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R1, d_Ri1, d_flag, c->stream));
-    TRY(ws_gemm_dev(w, nb, bs, cw, bs, d_Ri1, bs, 1.0, 0.0));
+    if (bs == 1) {
+      // Gram partials, then one kernel sums them and scales the column
+      const int64_t n = s->n;
+      int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
+      if (nchunk < 1) nchunk = 1;
+      const int64_t chunk = (n + nchunk - 1) / nchunk;
+      cx<double>* part = w->part.as<cx<double>>();
+      Timed tm(c, KORE_TIMER_ORTHO);
+      HIP_TRY(launch_tsdot(wcol(w, cw), n, 1, wcol(w, cw), n, 1, n, nchunk, chunk, part, c->stream));
+      HIP_TRY(launch_normalize1(reinterpret_cast<const double*>(part), nchunk, wcol(w, cw), wcol(w, nb), n, d_R1,
+                                d_Ri1, d_flag, c->stream));
+    } else {
+      TRY(ws_dot_dev(w, cw, bs, cw, bs, d_G));
+      HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R1, d_Ri1, d_flag, c->stream));
+      TRY(ws_gemm_dev(w, nb, bs, cw, bs, d_Ri1, bs, 1.0, 0.0));
+    }
     if (bs > 1) {
       TRY(ws_dot_dev(w, nb, bs, nb, bs, d_G));
       HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R2, d_Ri2, d_flag + 1, c->stream));

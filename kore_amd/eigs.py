@@ -25,8 +25,8 @@ wanted order, consecutively, as SLEPc does.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import dataclasses
-import threading
 import time
 
 import numpy as np
@@ -206,15 +206,23 @@ def _rayleigh_ritz(H, j, bs, sigma, which, tol, host):
     return _Ritz(j, S, Q, Bres, order, theta, Ys[:, order], res, nconv)
 
 
+_POOLS = []                        # threadpoolctl controller, found once per process
+
+
 def eigs(*args, **kwargs) -> EigResult:
     """See ``_eigs``.  The projected problem is tiny (at most a few hundred
     columns): multithreaded BLAS only adds start-up cost to each call, so the
-    host side runs with one BLAS thread."""
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:            # pragma: no cover
+    host side runs with one BLAS thread.  The BLAS libraries are located once
+    (about 10 ms) and only their thread count is switched per call."""
+    if not _POOLS:
+        try:
+            from threadpoolctl import ThreadpoolController
+            _POOLS.append(ThreadpoolController())
+        except ImportError:        # pragma: no cover
+            _POOLS.append(None)
+    if _POOLS[0] is None:          # pragma: no cover
         return _eigs(*args, **kwargs)
-    with threadpool_limits(limits=1):
+    with _POOLS[0].limit(limits=1):
         return _eigs(*args, **kwargs)
 
 
@@ -227,10 +235,14 @@ def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=1,
     OP application costs ~one FP64 sweep instead of many complex64 ones.
 
     check_every: columns between the early convergence checks made while the
-    basis grows (0 disables them; default max(8, 2 bs)).  Each check is a
-    Rayleigh-Ritz step on the decomposition reached so far, computed on the
-    host while the device expands the next columns; when the wanted pairs
-    have converged the iteration ends there, without expanding to ncv."""
+    basis grows (0 disables them).  Each check is a Rayleigh-Ritz step on the
+    decomposition reached so far, computed on the host while the device
+    expands the next columns; when the wanted pairs have converged the
+    iteration ends there, without expanding to ncv.  By default the base
+    spacing is max(4, 2 bs) columns; a chunk grows (up to 4x) so that its
+    device time covers the predicted Schur time, or so that it ends a base
+    step short of where the trend of the worst wanted residual reaches tol.
+    An explicit value is used as a fixed spacing."""
     res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
                      fallback_sweeps if sysm.precision != PREC_FP64 else None, check_every)
     if res is None:
@@ -304,8 +316,9 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     if expand is None:
         def expand(j0, j1, bs_, cw, Hm, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
             return expand_host(ws, j0, j1, bs_, cw, Hm, ir_tol, ir_maxit, ir_stop)
+    adaptive = check_every is None                 # explicit spacing: fixed chunks (deterministic)
     if check_every is None:
-        check_every = max(8, 2 * bs)
+        check_every = max(4, 2 * bs)
     check_every = int(np.ceil(check_every / bs) * bs) if check_every > 0 else 0
     j_first = int(np.ceil((nev + bs) / bs) * bs)    # fewest columns worth a check
     j_base = j                                      # columns of the last restart (no check there)
@@ -313,118 +326,136 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     t_col = None                                    # device wall time per OP column (last chunk)
     t_rr, j_rr = None, None                         # last check's host time and size
     hidden = {}                                     # host time spent beside the device
-    while True:
-        # ---- expand the decomposition to m columns; from j_first on, each
-        # chunk of check_every columns is expanded on the device (worker
-        # thread: the library call releases the GIL) while the host runs the
-        # Rayleigh-Ritz step of the decomposition reached before the chunk
-        rr = None
-        while j < m:
-            j_stop, early = m, None
-            can_check = bool(check_every) and j >= j_first and j > j_base
-            threaded = can_check and j + check_every < m
-            if threaded:
-                # a chunk long enough to cover the check (Schur cost ~ j^3)
-                chunk = check_every
-                if t_col and t_rr:
-                    chunk = max(chunk, int(np.ceil(t_rr * (j / j_rr) ** 3 / t_col / bs)) * bs)
-                j_stop = min(m, j + chunk)
-            elif check_every and not can_check:
-                j_stop = min(m, max(j_first, j + check_every))   # grow to where a check is worth it
-            if threaded:
-                Hs = np.asfortranarray(H[:j + bs, :j])
-                out = {}
-
-                def work(j0=j, j1=j_stop):
+    worker = None
+    trend = []                                      # (columns, worst wanted relative residual) per check
+    try:
+        while True:
+            # ---- expand the decomposition to m columns; from j_first on, each
+            # chunk of check_every columns is expanded on the device (a persistent
+            # worker thread: the library call releases the GIL) while the host runs
+            # the Rayleigh-Ritz step of the decomposition reached before the chunk
+            rr = None
+            while j < m:
+                j_stop, early = m, None
+                can_check = bool(check_every) and j >= j_first and j > j_base
+                threaded = can_check and j + check_every < m
+                if threaded:
+                    chunk = check_every
+                    if adaptive:
+                        # a chunk long enough to cover the check (Schur cost ~ j^3) ...
+                        if t_col and t_rr:
+                            chunk = max(chunk, int(np.ceil(t_rr * (j / j_rr) ** 3 / t_col / bs)) * bs)
+                        # ... and, once the worst wanted residual has been seen
+                        # falling at two checks, one that ends a base step short
+                        # of the column where the trend reaches tol, then base
+                        # steps (the chunk beside the final check is wasted)
+                        if len(trend) >= 2 and trend[-1][1] < trend[-2][1]:
+                            (ja, ra), (jb, rb) = trend[-2], trend[-1]
+                            slope = (np.log(rb) - np.log(ra)) / (jb - ja)
+                            j_conv = jb + (np.log(tol) - np.log(rb)) / slope
+                            chunk = check_every if j_conv - j <= 2 * check_every else \
+                                max(chunk, int((j_conv - j - check_every) / bs) * bs)
+                        chunk = min(chunk, 4 * check_every)
+                    j_stop = min(m, j + chunk)
+                elif check_every and not can_check:
+                    j_stop = min(m, max(j_first, j + check_every))   # grow to where a check is worth it
+                if threaded:
+                    Hs = np.asfortranarray(H[:j + bs, :j])
+                    def work(j0=j, j1=j_stop):
+                        t = time.perf_counter()
+                        r = expand(j0, j1, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
+                                   ir_stop=fallback_sweeps or 0)
+                        return r, time.perf_counter() - t
+                    if worker is None:                      # one thread per eigensolve, reused by every chunk
+                        worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+                    fut = worker.submit(work)
                     try:
                         t = time.perf_counter()
-                        out["r"] = expand(j0, j1, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
-                                          ir_stop=fallback_sweeps or 0)
-                        out["t"] = time.perf_counter() - t
-                    except BaseException as e:          # re-raised on the calling thread
-                        out["e"] = e
-                th = threading.Thread(target=work)
-                th.start()
-                try:
+                        early = _rayleigh_ritz(Hs, j, bs, sigma, which, tol, hidden)
+                        t_rr, j_rr = time.perf_counter() - t, j
+                    finally:
+                        (st, jd, ws_info), t_exp = fut.result()    # re-raises the worker's exception
+                    if jd > j:
+                        t_col = t_exp / (jd - j)
+                else:
                     t = time.perf_counter()
-                    early = _rayleigh_ritz(Hs, j, bs, sigma, which, tol, hidden)
-                    t_rr, j_rr = time.perf_counter() - t, j
-                finally:
-                    th.join()
-                if "e" in out:
-                    raise out["e"]
-                st, jd, ws_info = out["r"]
-                if jd > j:
-                    t_col = out["t"] / (jd - j)
-            else:
-                t = time.perf_counter()
-                st, jd, ws_info = expand(j, j_stop, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
-                                         ir_stop=fallback_sweeps or 0)
-                if jd > j:
-                    t_col = (time.perf_counter() - t) / (jd - j)
-            ir_iters = max(ir_iters, ws_info.iterations)
-            done_blocks = (jd - j) // bs
-            if st == KORE_WARN_IR_NOT_CONVERGED and fallback_sweeps is not None:
-                if own_ws:
-                    ws.close()
-                return None
-            if st == KORE_WARN_BREAKDOWN:
-                done_blocks += 1
-            op_calls += done_blocks
-            op_cols += bs * done_blocks
-            if early is not None and early.nconv >= nev:
-                rr = early                              # converged before the chunk
+                    st, jd, ws_info = expand(j, j_stop, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
+                                             ir_stop=fallback_sweeps or 0)
+                    if jd > j:
+                        t_col = (time.perf_counter() - t) / (jd - j)
+                ir_iters = max(ir_iters, ws_info.iterations)
+                done_blocks = (jd - j) // bs
+                if st == KORE_WARN_IR_NOT_CONVERGED and fallback_sweeps is not None:
+                    if own_ws:
+                        ws.close()
+                    return None
+                if st == KORE_WARN_BREAKDOWN:
+                    done_blocks += 1
+                op_calls += done_blocks
+                op_cols += bs * done_blocks
+                if early is not None:
+                    if early.nconv >= nev:
+                        rr = early                          # converged before the chunk
+                        break
+                    worst = float(np.max(early.res[:nev] / np.abs(early.theta[:nev])))
+                    if np.isfinite(worst) and worst > 0:
+                        trend.append((early.j, worst))
+                if st == KORE_WARN_BREAKDOWN:
+                    # invariant subspace: continue with a fresh random block
+                    breakdowns += 1
+                    nb_all = jd + bs
+                    Xr = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
+                    ws.set(cW, Xr)
+                    for _ in range(2):
+                        hh = ws.dot(0, nb_all, cW, bs)
+                        ws.gemm(cW, bs, 0, nb_all, hh, alpha=-1.0, beta=1.0)
+                    _cholqr(ws, cW, nb_all, bs)
+                    H[nb_all:nb_all + bs, jd:jd + bs] = 0.0
+                    jd = nb_all
+                j = jd
+            # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
+            if rr is None:
+                rr = _rayleigh_ritz(H, m, bs, sigma, which, tol, host)
+            nconv = rr.nconv
+            if log:
+                log("restart %d: size %d, nconv %d, worst wanted residual %.3e" % (
+                    restarts, rr.j, nconv, float(np.max(rr.res[:nev] / np.abs(rr.theta[:nev])))))
+            if nconv >= nev or restarts >= maxit:
                 break
-            if st == KORE_WARN_BREAKDOWN:
-                # invariant subspace: continue with a fresh random block
-                breakdowns += 1
-                nb_all = jd + bs
-                Xr = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
-                ws.set(cW, Xr)
-                for _ in range(2):
-                    hh = ws.dot(0, nb_all, cW, bs)
-                    ws.gemm(cW, bs, 0, nb_all, hh, alpha=-1.0, beta=1.0)
-                _cholqr(ws, cW, nb_all, bs)
-                H[nb_all:nb_all + bs, jd:jd + bs] = 0.0
-                jd = nb_all
-            j = jd
-        # ---- Rayleigh-Ritz on the projected matrix (host, FP64, LAPACK)
-        if rr is None:
-            rr = _rayleigh_ritz(H, m, bs, sigma, which, tol, host)
-        nconv = rr.nconv
-        if log:
-            log("restart %d: size %d, nconv %d, worst wanted residual %.3e" % (
-                restarts, rr.j, nconv, float(np.max(rr.res[:nev] / np.abs(rr.theta[:nev])))))
-        if nconv >= nev or restarts >= maxit:
-            break
-        # ---- thick restart: keep the 'keep' best Ritz values' invariant subspace
-        keep = nconv + max(1, (m - nconv) // 2)
-        keep = max(keep, nev)
-        keep = int(np.ceil(keep / bs) * bs)       # the expansion then ends exactly at m
-        keep = max(bs, min(keep, m - bs))
-        select = np.zeros(m, dtype=np.int32)
-        select[rr.order[:keep]] = 1
-        tq = time.perf_counter()
-        St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, rr.S, rr.Q, job="N")
-        host["reorder"] = host.get("reorder", 0.0) + time.perf_counter() - tq
-        if info != 0 or msel != keep:
-            raise RuntimeError("Schur reordering failed (info %d, %d of %d)" % (info, msel, keep))
-        ws.gemm(cT, keep, 0, m, Qt[:, :keep], alpha=1.0, beta=0.0)
-        ws.copy(cT, 0, keep)
-        ws.copy(m, keep, bs)
-        Hn = np.zeros_like(H)
-        Hn[:keep, :keep] = np.triu(St[:keep, :keep])
-        Hn[keep:keep + bs, :keep] = rr.Bres @ Qt[:, :keep]
-        H = Hn
-        j = keep
-        j_base = keep
-        restarts += 1
+            # ---- thick restart: keep the 'keep' best Ritz values' invariant subspace
+            keep = nconv + max(1, (m - nconv) // 2)
+            keep = max(keep, nev)
+            keep = int(np.ceil(keep / bs) * bs)       # the expansion then ends exactly at m
+            keep = max(bs, min(keep, m - bs))
+            select = np.zeros(m, dtype=np.int32)
+            select[rr.order[:keep]] = 1
+            tq = time.perf_counter()
+            St, Qt, _, msel, _, _, info = sla.lapack.ztrsen(select, rr.S, rr.Q, job="N")
+            host["reorder"] = host.get("reorder", 0.0) + time.perf_counter() - tq
+            if info != 0 or msel != keep:
+                raise RuntimeError("Schur reordering failed (info %d, %d of %d)" % (info, msel, keep))
+            ws.gemm(cT, keep, 0, m, Qt[:, :keep], alpha=1.0, beta=0.0)
+            ws.copy(cT, 0, keep)
+            ws.copy(m, keep, bs)
+            Hn = np.zeros_like(H)
+            Hn[:keep, :keep] = np.triu(St[:keep, :keep])
+            Hn[keep:keep + bs, :keep] = rr.Bres @ Qt[:, :keep]
+            H = Hn
+            j = keep
+            j_base = keep
+            trend = []
+            restarts += 1
+    finally:
+        if worker is not None:
+            worker.shutdown(wait=True)
     k = nconv
     X = np.zeros((n, 0), dtype=complex)
     if k > 0:
-        ws.gemm(cT, k, 0, rr.j, rr.Y[:, :k], alpha=1.0, beta=0.0)
+        # the basis is orthonormal, so ||V y|| = ||y||: normalising y gives
+        # unit eigenvectors without a host pass over the downloaded vectors
+        Yk = rr.Y[:, :k] / np.linalg.norm(rr.Y[:, :k], axis=0)
+        ws.gemm(cT, k, 0, rr.j, Yk, alpha=1.0, beta=0.0)
         X = ws.get(cT, k)
-        X = X / np.linalg.norm(X, axis=0)
     lam = sigma + 1.0 / rr.theta[:k]
     if own_ws:
         ws.close()

@@ -162,7 +162,8 @@ def test_early_convergence_checks():
     w = w[np.argsort(np.abs(w - sigma))]
     S = NumpySystem(A, B, sigma)
     full = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=2, check_every=0, workspace=NumpyWorkspace)
-    early = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=2, workspace=NumpyWorkspace)
+    early = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=2, check_every=8, workspace=NumpyWorkspace)
+    assert early.op_columns % 8 == 0 or early.op_columns == 120
     assert full.op_columns == 120 and full.restarts == 0
     assert early.op_columns < full.op_columns and early.restarts == 0
     for r in (full, early):
