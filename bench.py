@@ -111,8 +111,8 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
 
 # HBM bytes per solve-sweep dispatch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE (scaled by a plain 1 GiB copy) on the default workload (FP64,
-# block 4); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
-MEASURED_SWEEP_TRAFFIC = {("config2", 0, 4): 58.0e6}
+# Krylov block 1); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
+MEASURED_SWEEP_TRAFFIC = {("config2", 0, 1): 52.22e6}
 
 
 def roofline(run, precision, workload=None, block=None):
@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--e-range", default="1e-5,1e-4")
     ap.add_argument("--n-cap", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1234)
