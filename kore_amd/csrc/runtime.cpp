@@ -362,6 +362,8 @@ struct kore_sys {
   bool has_b = false;
   DevBuf d_ip, d_ix, d_va, d_vb, d_perm, d_blk, d_boff, d_scale, d_scr, d_part;
   CsrDev csr{};
+  DevBuf d_ipb, d_ixb, d_vbb;      // B alone (degree order): the B x of every OP application
+  CsrDev csr_b{};
   // scratch vectors (complex128, sizmat x 16 each): bb (rhs), xr (solution),
   // rr (residual), st (staging / scaled residual)
   cx<double>* bb = nullptr;
@@ -1179,25 +1181,29 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
                                                                std::max<int64_t>(1, n / 4096)}));
   // per-range upper bounds (entries of A plus B on the range's rows) place
   // each range's output in the pinned arena before anything is built
-  std::vector<size_t> ub(nth + 1, 0);
+  std::vector<size_t> ub(nth + 1, 0), ubb(nth + 1, 0);
   for (int t = 0; t < nth; ++t) {
     const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
-    size_t u = 0;
+    size_t u = 0, ubv = 0;
     for (int64_t i = r0; i < r1; ++i) {
       const int64_t r = s->perm[i];
       u += (size_t)(a_indptr[r + 1] - a_indptr[r]);
-      if (s->has_b) u += (size_t)(b_indptr[r + 1] - b_indptr[r]);
+      if (s->has_b) ubv += (size_t)(b_indptr[r + 1] - b_indptr[r]);
     }
-    ub[t + 1] = ub[t] + u;
+    ub[t + 1] = ub[t] + u + ubv;
+    ubb[t + 1] = ubb[t] + ubv;
   }
-  const size_t cap = std::max<size_t>(ub[nth], 1);
+  const size_t cap = std::max<size_t>(ub[nth], 1), capb = std::max<size_t>(ubb[nth], 1);
   char* arena = nullptr;
-  TRY(ctx_pinned(ctx, cap * (sizeof(int32_t) + 2 * sizeof(zd)) + 256, &arena));
+  TRY(ctx_pinned(ctx, cap * (sizeof(int32_t) + 2 * sizeof(zd)) + capb * (sizeof(int32_t) + sizeof(zd)) + 512,
+                 &arena));
   zd* pva = reinterpret_cast<zd*>(arena);
   zd* pvb = pva + cap;
-  int32_t* pix = reinterpret_cast<int32_t*>(pvb + cap);
-  std::vector<size_t> tcnt(nth, 0);
-  std::vector<int32_t> rowcnt(n, 0);
+  zd* pvbb = pvb + cap;                     // B alone
+  int32_t* pix = reinterpret_cast<int32_t*>(pvbb + capb);
+  int32_t* pixb = pix + cap;
+  std::vector<size_t> tcnt(nth, 0), tcntb(nth, 0);
+  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
   auto build_rows = [&](int t) {
@@ -1206,7 +1212,9 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     int32_t* oix = pix + ub[t];
     zd* ova = pva + ub[t];
     zd* ovb = pvb + ub[t];
-    size_t cntt = 0;
+    int32_t* oixb = pixb + ubb[t];
+    zd* ovbb = pvbb + ubb[t];
+    size_t cntt = 0, cnttb = 0;
     for (int64_t i = r0; i < r1; ++i) {
       const int64_t r = s->perm[i];
       row.clear();
@@ -1230,7 +1238,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         row[p] = e;
       }
       const int kb = s->h_blk[i];
-      int32_t cnt = 0;
+      int32_t cnt = 0, cntb = 0;
       for (size_t q = 0; q < row.size();) {
         const int32_t col = row[q].col;
         zd a = 0, b = 0;
@@ -1251,10 +1259,18 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         ovb[cntt] = b;
         ++cntt;
         ++cnt;
+        if (b != zd(0, 0)) {
+          oixb[cnttb] = col;
+          ovbb[cnttb] = b;
+          ++cnttb;
+          ++cntb;
+        }
       }
       rowcnt[i] = cnt;
+      rowcntb[i] = cntb;
     }
     tcnt[t] = cntt;
+    tcntb[t] = cnttb;
   };
   {
     std::vector<std::thread> pool;
@@ -1277,6 +1293,14 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->d_ix.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(int32_t)));
   TRY(s->d_va.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(zd)));
   TRY(s->d_vb.alloc(ctx, std::max<size_t>(nnz, 1) * sizeof(zd)));
+  std::vector<int32_t> ipb(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) ipb[i + 1] = ipb[i] + rowcntb[i];
+  const size_t nnzb = (size_t)ipb[n];
+  if (s->has_b) {
+    TRY(s->d_ipb.alloc(ctx, (n + 1) * sizeof(int32_t)));
+    TRY(s->d_ixb.alloc(ctx, std::max<size_t>(nnzb, 1) * sizeof(int32_t)));
+    TRY(s->d_vbb.alloc(ctx, std::max<size_t>(nnzb, 1) * sizeof(zd)));
+  }
   TRY(s->d_perm.alloc(ctx, n * sizeof(int64_t)));
   TRY(s->d_blk.alloc(ctx, n * sizeof(int32_t)));
   TRY(s->d_boff.alloc(ctx, (s->nblk + 1) * sizeof(int64_t)));
@@ -1300,11 +1324,28 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
       off += cnt;
     }
   }
+  if (s->has_b) {
+    HIP_TRY(hipMemcpyAsync(s->d_ipb.p, ipb.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    size_t off = 0;
+    for (int t = 0; t < nth; ++t) {
+      const size_t cnt = tcntb[t];
+      if (cnt) {
+        HIP_TRY(hipMemcpyAsync(s->d_ixb.as<int32_t>() + off, pixb + ubb[t], cnt * sizeof(int32_t),
+                               hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->d_vbb.as<zd>() + off, pvbb + ubb[t], cnt * sizeof(zd), hipMemcpyHostToDevice, st));
+      }
+      off += cnt;
+    }
+  }
   HIP_TRY(hipMemcpyAsync(s->d_perm.p, s->perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_blk.p, s->h_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_boff.p, s->boff.data(), (s->nblk + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   s->csr = CsrDev{n, s->d_ip.as<int32_t>(), s->d_ix.as<int32_t>(), s->d_va.as<cx<double>>(), s->d_vb.as<cx<double>>()};
+  // B alone: its A values are never read (ca = 0 skips them)
+  if (s->has_b)
+    s->csr_b = CsrDev{n, s->d_ipb.as<int32_t>(), s->d_ixb.as<int32_t>(), s->d_vbb.as<cx<double>>(),
+                      s->d_vbb.as<cx<double>>()};
   cx<double>* scr = s->d_scr.as<cx<double>>();
   s->bb = scr;
   s->xr = scr + (size_t)n * MAXR;
@@ -1593,7 +1634,7 @@ static int32_t ws_apply_op_dev(kore_ws* w, int src, int dst, int nc, double ir_t
     const int nr = std::min(MAXR, nc - a);
     {
       Timed tm(s->ctx, KORE_TIMER_SPMV);
-      HIP_TRY(launch_spmv(s->csr, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
+      HIP_TRY(launch_spmv(s->csr_b, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
     }
     kore_solve_info li{};
     TRY(solve_dev(s, s->bb, n, wcol(w, dst + a), n, nr, ir_tol, ir_maxit, &li));
