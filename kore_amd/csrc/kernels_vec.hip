@@ -18,7 +18,7 @@ template <int NR>
 __global__ __launch_bounds__(256) void spmv_kernel(CsrDev m, cx<double> ca, cx<double> cb,
                                                    const cx<double>* __restrict__ X, int64_t ldx,
                                                    cx<double>* __restrict__ Y, int64_t ldy, int nrhs,
-                                                   double beta) {
+                                                   double beta, const double* __restrict__ rscale) {
   const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
   const int64_t row = (int64_t)blockIdx.x * 16 + g;
   if (row >= m.n) return;
@@ -46,11 +46,12 @@ __global__ __launch_bounds__(256) void spmv_kernel(CsrDev m, cx<double> ca, cx<d
     }
   }
   if (gl == 0) {
+    const double sc = rscale ? rscale[row] : 1.0;   // optional row scale of the product
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       if (c < nrhs) {
         cx<double>* y = Y + (int64_t)c * ldy + row;
-        cx<double> v = acc[c];
+        cx<double> v{sc * acc[c].re, sc * acc[c].im};
         if (beta != 0.0) {
           const cx<double> o = *y;
           v.re += beta * o.re;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(CsrDev m, cx<double> ca, cx<d
 
 hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re, double cb_im,
                        const cx<double>* X, int64_t ldx, cx<double>* Y, int64_t ldy, int nrhs,
-                       double beta, hipStream_t s) {
+                       double beta, hipStream_t s, const double* rscale) {
   if (m.n == 0) return hipSuccess;
   const cx<double> ca{ca_re, ca_im}, cb{cb_re, cb_im};
   const dim3 grid((unsigned)((m.n + 15) / 16));
@@ -73,11 +74,11 @@ hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re
     const cx<double>* x = X + (int64_t)c0 * ldx;
     cx<double>* y = Y + (int64_t)c0 * ldy;
     if (nr == 1)
-      hipLaunchKernelGGL(spmv_kernel<1>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+      hipLaunchKernelGGL(spmv_kernel<1>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta, rscale);
     else if (nr <= 4)
-      hipLaunchKernelGGL(spmv_kernel<4>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+      hipLaunchKernelGGL(spmv_kernel<4>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta, rscale);
     else
-      hipLaunchKernelGGL(spmv_kernel<16>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta);
+      hipLaunchKernelGGL(spmv_kernel<16>, grid, dim3(256), 0, s, m, ca, cb, x, ldx, y, ldy, nr, beta, rscale);
   }
   return hipGetLastError();
 }

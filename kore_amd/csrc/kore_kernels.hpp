@@ -68,7 +68,7 @@ struct CsrDev {
 // Y[:, c] = beta*Y[:, c] + (ca*A + cb*B) X[:, c],  c < nrhs  (complex128)
 hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re, double cb_im,
                        const cx<double>* X, int64_t ldx, cx<double>* Y, int64_t ldy, int nrhs,
-                       double beta, hipStream_t s);
+                       double beta, hipStream_t s, const double* rscale = nullptr);
 // s[i] = 2^-round(log2 max_j |a_ij - sigma b_ij|)
 hipError_t launch_row_scale(const CsrDev& m, double sre, double sim, double* scale, hipStream_t s);
 // Scatter R (A - sigma B) into the dense level-0 blocks.

@@ -1630,8 +1630,28 @@ static int32_t ws_apply_op_dev(kore_ws* w, int src, int dst, int nc, double ir_t
   info->iterations = 0;
   info->converged = 1;
   info->rel_correction = -1;
+  const bool direct = s->f && s->factored && s->f->prec == KORE_PREC_FP64 && ir_maxit <= 0;
   for (int a = 0; a < nc; a += MAXR) {
     const int nr = std::min(MAXR, nc - a);
+    if (direct) {
+      // FP64 factor, no refinement: B x lands row-scaled in the factor's
+      // right-hand side (the equilibration of the first refinement sweep,
+      // fused), then one cyclic-reduction solve
+      Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
+      {
+        Timed tm(s->ctx, KORE_TIMER_SPMV);
+        HIP_TRY(launch_spmv(s->csr_b, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, F->Bv, n, nr, 0.0, s->ctx->stream,
+                            s->d_scale.as<double>()));
+      }
+      TRY(bcr_solve<double>(s, F, nr));
+      {
+        Timed tm(s->ctx, KORE_TIMER_OTHER);
+        HIP_TRY(launch_gather_scale<double, double>(F->Xv, n, wcol(w, dst + a), n, n, nr, nullptr, nullptr, 1.0,
+                                                    s->ctx->stream));
+      }
+      info->iterations = std::max(info->iterations, 1);
+      continue;
+    }
     {
       Timed tm(s->ctx, KORE_TIMER_SPMV);
       HIP_TRY(launch_spmv(s->csr_b, 0.0, 0.0, 1.0, 0.0, wcol(w, src + a), n, s->bb, n, nr, 0.0, s->ctx->stream));
