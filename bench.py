@@ -38,17 +38,17 @@ PEAK_FP64 = 78.6
 PEAK_FP32 = 157.3
 
 WORKLOADS = {
-    "config2": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=200, lmax=300),
+    "hydro_E1e-6_N200": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=200, lmax=300),
                     sigma=-1e-3 + 0.5j, nev=10, tol=1e-12,
                     name="eigen m=1 E=1e-6 N=200 lmax=300 nev=10 sigma=-1e-3+0.5i"),
-    "config1": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=40),
+    "hydro_E1e-4_N24": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=40),
                     sigma=1.0j, nev=5, tol=1e-12,
                     name="eigen m=1 E=1e-4 N=24 lmax=40 nev=5 sigma=1i"),
-    "config4": dict(par=dict(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=150, lmax=201, magnetic=1,
+    "mac_E1e-6_N150": dict(par=dict(m=2, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-6, N=150, lmax=201, magnetic=1,
                              thermal=1, Le2=1e-8, Pm=1e-3, Prandtl=1.0, Ra_gap=6e5),
                     sigma=0.1j, nev=8, tol=1e-12,
                     name="MAC eigen m=2 E=1e-6 Le=1e-4 Pm=1e-3 Pr=1 N=150 lmax=201 nev=8 sigma=0.1i"),
-    "config3": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-7, N=400, lmax=500),
+    "hydro_E1e-7_N400": dict(par=dict(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-7, N=400, lmax=500),
                     sigma=-1e-3 + 0.5j, nev=10, tol=1e-12,
                     name="eigen m=1 E=1e-7 N=400 lmax=500 nev=10 sigma=-1e-3+0.5i"),
 }
@@ -112,7 +112,7 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
 # HBM bytes per solve-sweep dispatch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE (scaled by a plain 1 GiB copy) on the default workload (FP64,
 # Krylov block 1); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
-MEASURED_SWEEP_TRAFFIC = {("config2", 0, 1): 52.22e6}
+MEASURED_SWEEP_TRAFFIC = {("hydro_E1e-6_N200", 0, 1): 52.22e6}
 
 
 def roofline(run, precision, workload=None, block=None):
@@ -278,7 +278,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--workload", default="hydro_E1e-6_N200", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--e-range", default="1e-5,1e-4")

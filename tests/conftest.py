@@ -34,7 +34,7 @@ def load_ops_fixture(name):
 SPINOVER_LAMBDA = complex(-1.019947097016500742e-01, 1.005733690103750355e+00)
 
 
-def params_config1():
+def params_hydro_small():
     from kore_amd.params import Params
     # hydrodynamic inertial modes: m=1, E=1e-4, ricb=0.35, no-slip, symmetric, N=24, lmax=40
     return Params(m=1, symm=1, ricb=0.35, bci=1, bco=1, Ek=1e-4, N=24, lmax=40, tau=1.0j, nev=5, tol=1e-12)
@@ -46,9 +46,9 @@ def params_spinover():
 
 
 @pytest.fixture(scope="session")
-def config1():
+def hydro_small():
     from kore_amd import assembly
-    par = params_config1()
+    par = params_hydro_small()
     A, B, _ = assembly.assemble(par)
     return par, A, B
 

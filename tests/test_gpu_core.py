@@ -36,10 +36,10 @@ def _system(ctx, par, A, B):
     return backend.System(ctx, lay, A, B)
 
 
-def test_ws_gemm_and_dot(ctx, config1):
+def test_ws_gemm_and_dot(ctx, hydro_small):
     """GEMM kernel (both operand modes) through the workspace primitives."""
     from kore_amd import backend
-    par, A, B = config1
+    par, A, B = hydro_small
     say("case: ws gemm/dot, sizmat", A.shape[0])
     S = _system(ctx, par, A, B)
     ws = backend.Workspace(S, 40)
@@ -89,9 +89,9 @@ def test_solve_tiny_fp64(ctx):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_solve_config1(ctx, config1, precision):
-    par, A, B = config1
-    say("case: config1 solve precision", precision)
+def test_solve_hydro_small(ctx, hydro_small, precision):
+    par, A, B = hydro_small
+    say("case: hydro_small solve precision", precision)
     err, info = _solve_case(ctx, par, A, B, 1.0j, precision, nrhs=3)
     say("  rel err", err, "sweeps", info.iterations, "last correction", info.rel_correction)
     assert err.max() <= 1e-10
@@ -116,19 +116,19 @@ def test_solve_large_blocks(ctx, case, precision):
     assert err.max() <= 1e-10
 
 
-def test_solve_many_rhs(ctx, config1):
-    par, A, B = config1
-    say("case: config1 solve 21 rhs fp64")
+def test_solve_many_rhs(ctx, hydro_small):
+    par, A, B = hydro_small
+    say("case: hydro_small solve 21 rhs fp64")
     err, _ = _solve_case(ctx, par, A, B, 0.5 + 0.7j, 0, nrhs=21, seed=3)
     assert err.max() <= 1e-10
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_eigs_config1(ctx, config1, precision):
+def test_eigs_hydro_small(ctx, hydro_small, precision):
     from kore_amd import eigs
     from oracle import krylov_schur
-    par, A, B = config1
-    say("case: config1 eigs precision", precision)
+    par, A, B = hydro_small
+    say("case: hydro_small eigs precision", precision)
     S = _system(ctx, par, A, B)
     S.factor(par.tau, precision)
     r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=4)
@@ -163,12 +163,12 @@ def test_spinover_golden(ctx, spinover, precision):
 
 
 @pytest.mark.parametrize("bs", [1, 5, 8, 16])
-def test_expand_matches_host_driven(ctx, config1, bs):
+def test_expand_matches_host_driven(ctx, hydro_small, bs):
     """kore_ws_expand (coefficients kept on the device, Cholesky on the
     device) against the same expansion driven from the host through the
     workspace primitives (scipy Cholesky): projected matrix and basis agree."""
     from kore_amd import backend, eigs
-    par, A, B = config1
+    par, A, B = hydro_small
     say("case: expand vs host-driven, bs", bs)
     S = _system(ctx, par, A, B)
     S.factor(par.tau, 0)
@@ -199,14 +199,14 @@ def test_expand_matches_host_driven(ctx, config1, bs):
     S.close()
 
 
-def test_expand_breakdown_recovers(ctx, config1):
+def test_expand_breakdown_recovers(ctx, hydro_small):
     """Start block in the null space of B': the device Cholesky flags the
     breakdown, the host continues with a random block, and the eigenvalues
     match the oracle on the same pencil."""
     from conftest import pencil_with_null_block
     from kore_amd import eigs
     from oracle import krylov_schur
-    par, A, B = config1
+    par, A, B = hydro_small
     A2, Bn, E = pencil_with_null_block(A, B)
     sigma = par.tau
     say("case: expand breakdown, null columns", np.nonzero(E)[0])

@@ -193,12 +193,12 @@ def test_triangular_eigenvectors():
     assert np.all(np.isfinite(eigs._triu_eigvecs(S2)))
 
 
-def test_breakdown_path_recovers(config1):
+def test_breakdown_path_recovers(hydro_small):
     """A start block in the null space of B' breaks down at the first
     expansion (W = 0 exactly); the random replacement block must lead to the
     pencil's true eigenvalues nearest sigma (dense QZ)."""
     from conftest import pencil_with_null_block
-    par, A, B = config1
+    par, A, B = hydro_small
     A2, Bn, E = pencil_with_null_block(A, B)
     sigma = 1.0j
     S = NumpySystem(A2, Bn, sigma)

@@ -19,7 +19,7 @@ import scipy.sparse as ss
 import scipy.sparse.linalg as sla
 
 from conftest import (DORMY_GOLDEN, JONES_GOLDEN, SPINOVER_LAMBDA, critical_rayleigh, load_ops_fixture,
-                      params_config1, params_dormy, params_jones, params_spinover)
+                      params_hydro_small, params_dormy, params_jones, params_spinover)
 
 from kore_amd import assembly
 from kore_amd.params import Params, ell as kell
@@ -43,7 +43,7 @@ def params_mac_small():
                   Lambda=0.1, Pm=1e-3, Prandtl=1.0, Ra_gap=6e5)
 
 
-FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_config1), ("spinover", params_spinover),
+FIXTURES = [("hydro_m1_E1e-4_N24_L40", params_hydro_small), ("spinover", params_spinover),
             ("thermal_m9_N24_L32", params_thermal), ("jones_m9_N24_L32", params_jones_small),
             ("mac_m2_N16_L17", params_mac_small)]
 
@@ -178,8 +178,8 @@ def test_spinover_golden_eigenvalue(spinover):
                                atol=1e-20)
 
 
-def test_layout_invariants(config1):
-    par, A, B = config1
+def test_layout_invariants(hydro_small):
+    par, A, B = hydro_small
     perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
     assert sorted(perm.tolist()) == list(range(par.sizmat))
     assert np.all(sizes == par.N1) and len(sizes) == par.lmax - par.m + 1
@@ -215,8 +215,8 @@ def test_forced_operator_identity():
     assert np.abs(D.data).max(initial=0) <= 1e-12 * np.abs(Af.data).max()
 
 
-def test_normalisation(config1):
-    par, A, B = config1
+def test_normalisation(hydro_small):
+    par, A, B = hydro_small
     assert abs(ss.linalg.norm(B) - 1.0) < 1e-14
     assert B.dtype == np.float64 and A.dtype == np.complex128
 
@@ -227,9 +227,9 @@ def _medium():
     return par, A, B
 
 
-@pytest.mark.parametrize("case", ["config1", "medium"])
-def test_oracle_solvers_vs_superlu(config1, case):
-    par, A, B = config1 if case == "config1" else _medium()
+@pytest.mark.parametrize("case", ["hydro_small", "medium"])
+def test_oracle_solvers_vs_superlu(hydro_small, case):
+    par, A, B = hydro_small if case == "hydro_small" else _medium()
     sigma = 0.3 + 0.8j
     perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
     M = layout.permute(A - sigma * B, perm)
@@ -243,9 +243,9 @@ def test_oracle_solvers_vs_superlu(config1, case):
         assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-10
 
 
-def test_oracle_bcr_planted_mistake_is_caught(config1):
+def test_oracle_bcr_planted_mistake_is_caught(hydro_small):
     """A dropped Schur term (a plausible BCR bug) must fail the same check."""
-    par, A, B = config1
+    par, A, B = hydro_small
     perm, sizes = layout.lmajor_permutation(par.m, par.lmax, par.symm, par.N1)
     M = layout.permute(A - 0.5j * B, perm)
     D, L, U, off = layout.block_tridiag(M, sizes)

@@ -55,8 +55,8 @@ def test_writers_match_numpy_savetxt(tmp_path, monkeypatch):
     assert open("real_flow.field").readline().split()[0] == "%.18e" % 1.0
 
 
-def test_npz_round_trip(tmp_path, config1):
-    par, A, B = config1
+def test_npz_round_trip(tmp_path, hydro_small):
+    par, A, B = hydro_small
     p = tmp_path / "A.npz"
     assembly.save_npz_csr(str(p), A)
     A2 = assembly.load_csr(str(p))
