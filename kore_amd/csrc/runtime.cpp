@@ -339,6 +339,15 @@ template <class T> struct Factor : FactorBase {
   InvDesc<T>* root_inv = nullptr;
   SkinnyDesc<T>* root_solve = nullptr;
   int root_n = 0;
+  // Two-block tail: the last level (blocks k0, k1) and the root solved in
+  // one launch, X0 = Dinv0 B0 - P B1 and X1 = Q B1 - R B0, with
+  // P = Dinv0 F0, R = G1 Dinv0, Q = Dinv1 + G1 P formed after the root
+  // inversion (P in G[k0], R in H[k1], Q over D[k1]).
+  bool tail = false;
+  GemmDesc<T>* tail1 = nullptr;     // P and R
+  GemmDesc<T>* tail2 = nullptr;     // Q
+  int tail_tiles1 = 0, tail_tiles2 = 0, tail_m = 0;
+  SkinnyDesc<T>* tail_solve = nullptr;   // two descriptors
   cx<T>** dD = nullptr;
   cx<T>** dL = nullptr;
   cx<T>** dU = nullptr;
@@ -656,8 +665,34 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   vs.push_back(mkskin(vecp(F->Xv, kr), (int)nk(kr), {Prod{F->D[kr], (int)nk(kr), vecp(F->Bv, kr), (int)nk(kr), T(1)}},
                       T(0)));
   F->root_n = (int)nk(kr);
+  // two-block tail (see Factor): replaces the forward and backward launches
+  // of the last level and the root launch by one; the sweep reads the same
+  // bytes (n0 n1 + n1^2 + n1 n0 + n0^2 either way)
+  size_t rtail1 = 0, rtail2 = 0, rtails = 0;
+  if (nlev >= 1) {
+    const auto& id = ids[nlev - 1];
+    const int k0 = id[0], k1 = id[1];
+    const int n0 = (int)nk(k0), n1 = (int)nk(k1);
+    cx<T>* F0 = base + offF[nlev - 1][0];
+    rtail1 = vg.size();
+    vg.push_back(mkgemm(F->G[k0], n0, n1, F->D[k0], F0, n0, n0, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+    vg.push_back(mkgemm(F->H[k1], n1, n0, F->G[k1], F->D[k0], n0, n1, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+    rtail2 = vg.size();
+    vg.push_back(mkgemm(F->D[k1], n1, n1, F->G[k1], F->G[k0], n0, n1, n0, nullptr, nullptr, 0, 0, 0, T(1), T(1)));
+    F->tail_tiles1 = std::max(tiles(n0, n1), tiles(n1, n0));
+    F->tail_tiles2 = tiles(n1, n1);
+    F->tail_m = std::max(n0, n1);
+    const double sb = F->solve_bytes;      // the replaced launches already counted these bytes
+    rtails = vs.size();
+    vs.push_back(mkskin(vecp(F->Xv, k0), n0, {Prod{F->D[k0], n0, vecp(F->Bv, k0), n0, T(1)},
+                                              Prod{F->G[k0], n0, vecp(F->Bv, k1), n1, T(-1)}}, T(0)));
+    vs.push_back(mkskin(vecp(F->Xv, k1), n1, {Prod{F->D[k1], n1, vecp(F->Bv, k1), n1, T(1)},
+                                              Prod{F->H[k1], n1, vecp(F->Bv, k0), n0, T(-1)}}, T(0)));
+    F->solve_bytes = sb;
+    F->tail = true;
+  }
   F->flops = flops;
-  F->solve_launches = 2 * nlev + 1;
+  F->solve_launches = F->tail ? 2 * nlev - 1 : 2 * nlev + 1;
 
   // upload descriptors
   const size_t bi = vinv.size() * sizeof(InvDesc<T>);
@@ -690,6 +725,11 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   }
   F->root_inv = dinv + rinv[nlev].off;
   F->root_solve = dsk + rroot;
+  if (F->tail) {
+    F->tail1 = dg + rtail1;
+    F->tail2 = dg + rtail2;
+    F->tail_solve = dsk + rtails;
+  }
   F->ndlu = cntDLU;
   return KORE_OK;
 }
@@ -749,22 +789,31 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
     Timed tm(c, KORE_TIMER_INV);
     HIP_TRY(launch_gj_inverse<T>(F->root_inv, 1, F->root_n, F->root_n, st));
   }
+  if (F->tail) {
+    Timed tm(c, KORE_TIMER_GEMM);
+    HIP_TRY(launch_gemm_batched<T>(F->tail1, 2, F->tail_tiles1, st));
+    HIP_TRY(launch_gemm_batched<T>(F->tail2, 1, F->tail_tiles2, st));
+  }
   return KORE_OK;
 }
 
 template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs) {
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
-  for (int l = 0; l < F->nlev; ++l) {
+  const int ltop = F->tail ? F->nlev - 1 : F->nlev;   // levels swept one by one
+  for (int l = 0; l < ltop; ++l) {
     Level<T>& L = F->lev[l];
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
   }
   {
     Timed tm(c, KORE_TIMER_SKINNY);
-    HIP_TRY(launch_skinny<T>(F->root_solve, 1, F->root_n, nrhs, st));
+    if (F->tail)
+      HIP_TRY(launch_skinny<T>(F->tail_solve, 2, F->tail_m, nrhs, st));
+    else
+      HIP_TRY(launch_skinny<T>(F->root_solve, 1, F->root_n, nrhs, st));
   }
-  for (int l = F->nlev - 1; l >= 0; --l) {
+  for (int l = ltop - 1; l >= 0; --l) {
     Level<T>& L = F->lev[l];
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.bw, L.nbw, L.mbw, nrhs, st));
