@@ -22,7 +22,7 @@ constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
 constexpr int GT_PADB = 1;
 
 template <class T>
-__global__ __launch_bounds__(256) void gemm_batched_kernel(const GemmDesc<T>* __restrict__ descs) {
+__global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>* __restrict__ descs) {
   using A4 = typename acc4<T>::type;
   const GemmDesc<T> d = descs[blockIdx.y];
   const int ntiles = d.tiles_m * d.tiles_n;
