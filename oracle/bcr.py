@@ -15,6 +15,15 @@ until one block remains (Dinv of it is stored).  Solve:
     root:                  x_0 = Dinv_0 b_0
     backward, per level:   x_t = Dinv_t (b_t - L_t x_{t-1} - U_t x_{t+1})  (t odd)
 
+With tail=True (what the device runs) the last level (two blocks, t = 0, 1)
+and the root are one step: with G_1 = Dinv_1 L_1, P = Dinv_0 F_0,
+R = G_1 Dinv_0 and Q = Dinv_1 + G_1 P,
+
+    x_0 = Dinv_0 b_0 - P b_1,    x_1 = Q b_1 - R b_0
+
+(substituting the last forward step into the root and the root into the
+last backward step).
+
 It solves the same linear system as the reference's direct solve
 (bin/solve.py:123, :227), so it is pinned by agreement with
 ``blocktri.BlockThomas`` and scipy's splu on assembled systems, and by the
@@ -39,7 +48,7 @@ def pow2_row_scale(D, L, U):
 
 
 class BCR:
-    def __init__(self, D, L, U, equilibrate=True, dtype=complex):
+    def __init__(self, D, L, U, equilibrate=True, dtype=complex, tail=True):
         n = len(D)
         self.dtype = dtype
         if equilibrate:
@@ -80,6 +89,12 @@ class BCR:
             D, L, U = nD, nL, nU
             ids = ids[0::2]
         self.root = np.linalg.inv(D[0])
+        self.tail = None
+        if tail and self.levels:
+            last = self.levels[-1]
+            G1 = last["Dinv"][1] @ last["L"][1]
+            P = self.root @ last["F"][0]
+            self.tail = dict(P=P, R=G1 @ self.root, Q=last["Dinv"][1] + G1 @ P)
 
     def solve(self, b, off):
         b = np.asarray(b, dtype=complex)
@@ -89,7 +104,8 @@ class BCR:
         bb = [v.astype(self.dtype) for v in bb]
         stack = []
         cur = bb
-        for lev in self.levels:
+        levels = self.levels[:-1] if self.tail else self.levels
+        for lev in levels:
             T = len(cur)
             new = []
             for t in range(0, T, 2):
@@ -101,8 +117,12 @@ class BCR:
                 new.append(v)
             stack.append(cur)
             cur = new
-        x = [self.root @ cur[0]]
-        for lev, bl in zip(reversed(self.levels), reversed(stack)):
+        if self.tail:
+            tl = self.tail
+            x = [self.root @ cur[0] - tl["P"] @ cur[1], tl["Q"] @ cur[1] - tl["R"] @ cur[0]]
+        else:
+            x = [self.root @ cur[0]]
+        for lev, bl in zip(reversed(levels), reversed(stack)):
             T = len(bl)
             out = [None] * T
             for t in range(0, T, 2):

@@ -237,10 +237,39 @@ def test_oracle_solvers_vs_superlu(hydro_small, case):
     rng = np.random.default_rng(0)
     b = rng.standard_normal(M.shape[0]) + 1j * rng.standard_normal(M.shape[0])
     xs = sla.splu(M.tocsc()).solve(b)
-    for sol in (blocktri.BlockThomas(D, L, U), bcr.BCR(D, L, U)):
+    for sol in (blocktri.BlockThomas(D, L, U), bcr.BCR(D, L, U), bcr.BCR(D, L, U, tail=False)):
         x = sol.solve(b, off)
         assert blocktri.relres(M, x, b) <= 1e-16
         assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-10
+
+
+@pytest.mark.parametrize("nblk", [2, 3, 5, 6])
+def test_oracle_bcr_tail_on_few_blocks(nblk):
+    """The two-block tail (last level and root in one step) on systems whose
+    reduction has one or a few levels: same solution as SuperLU, and a sign
+    error in its R term (a plausible slip in the substitution) is caught."""
+    rng = np.random.default_rng(nblk)
+    sizes = [7 + (k % 3) for k in range(nblk)]
+    D = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 6 * np.eye(n) for n in sizes]
+    L = [None] + [rng.standard_normal((sizes[k], sizes[k - 1])) + 0j for k in range(1, nblk)]
+    U = [rng.standard_normal((sizes[k], sizes[k + 1])) + 0j for k in range(nblk - 1)] + [None]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    M = np.zeros((off[-1], off[-1]), dtype=complex)
+    for k in range(nblk):
+        M[off[k]:off[k + 1], off[k]:off[k + 1]] = D[k]
+        if L[k] is not None:
+            M[off[k]:off[k + 1], off[k - 1]:off[k]] = L[k]
+        if U[k] is not None:
+            M[off[k]:off[k + 1], off[k + 1]:off[k + 2]] = U[k]
+    b = rng.standard_normal(off[-1]) + 1j * rng.standard_normal(off[-1])
+    xs = np.linalg.solve(M, b)
+    sol = bcr.BCR(D, L, U)
+    assert sol.tail is not None
+    x = sol.solve(b, off)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-12
+    sol.tail["R"] = -sol.tail["R"]
+    x = sol.solve(b, off)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) > 1e-6
 
 
 def test_oracle_bcr_planted_mistake_is_caught(hydro_small):
