@@ -406,7 +406,7 @@ struct kore_ws {
 };
 
 static constexpr int MAXR = 16;
-static constexpr int DOT_MAXCHUNK = 512;   // split-K chunks of the workspace inner products
+static constexpr int DOT_MAXCHUNK = 1024;   // split-K chunks of the workspace inner products
 static constexpr int DOT_ROWS = 64;        // minimum rows per chunk
 
 template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
