@@ -6,6 +6,7 @@
 //   skinny        batched Y = beta Y + alpha (A1 X1 + A2 X2), nrhs <= 16,
 //                 the block sweeps of the cyclic-reduction solve
 //   gather/scatter/axpy/norms/zero/copy   vector plumbing
+#include <type_traits>
 #include "kore_device.hpp"
 #include "kore_kernels.hpp"
 
@@ -178,10 +179,12 @@ template <int NR> struct SkLds {
 };
 
 // PRE (launches of few workgroups, where each product's load latency is the
-// cost): a product with K <= min(SK_PRE_K, chunk) has all of this thread's A entries and
+// cost): a product with K <= min(SkPreK, X chunk) -- 512 for one or two
+// right-hand sides, 256 otherwise -- has all of this thread's A entries and
 // its share of the X chunk requested before the first wait, so the product
 // costs one memory round trip instead of one per unrolled group.
 constexpr int SK_PRE_K = 256;
+template <int NR> struct SkPreK { static constexpr int value = NR <= 2 ? 512 : SK_PRE_K; };   // k entries preloaded
 template <class T, int NR, bool PRE>
 __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
@@ -204,9 +207,10 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
     const int64_t lda = d.lda[p], ldx = d.ldx[p];
     const T al = d.alpha[p];
     if constexpr (PRE) {
-      constexpr int KP = SK_PRE_K < KC ? SK_PRE_K : KC;  // k entries staged at once
-      if (K <= KP) {                      // uniform
-        constexpr int NA = KP / SK_KS;                   // A entries per thread
+      constexpr int KP = SkPreK<NR>::value < KC ? SkPreK<NR>::value : KC;  // k entries staged at once
+      // the product, with NA A entries per thread (NA * 16 >= K)
+      auto product = [&](auto na) {
+        constexpr int NA = decltype(na)::value;
         constexpr int NXL = (KC * NR + 255) / 256;       // X entries per thread
         cx<T> xr[NXL];
 #pragma unroll
@@ -240,6 +244,13 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
 #pragma unroll
           for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a[t], sX[kk][c]);
         }
+      };
+      if (K <= SK_PRE_K && SK_PRE_K <= KP) {          // uniform: the short form when it suffices
+        product(std::integral_constant<int, SK_PRE_K / SK_KS>{});
+        continue;
+      }
+      if (K <= KP) {
+        product(std::integral_constant<int, KP / SK_KS>{});
         continue;
       }
     }
