@@ -932,7 +932,7 @@ static int32_t solve_dev(kore_sys* s, const cx<double>* b, int64_t ldb, cx<doubl
 // restart Hessenberg matrix and its Givens rotations on the host.  Converges
 // where plain refinement with a complex64 factor stalls (condition numbers up
 // to about 1/eps_fp32), because the factor only has to be a preconditioner.
-static constexpr int GM_MAXRESTART = 30, GM_CHUNKS = 512;
+static constexpr int GM_MAXRESTART = 30, GM_CHUNKS = 1024;
 
 static int32_t prec_apply(kore_sys* s, const cx<double>* v, cx<double>* z) {
   if (s->f->prec == KORE_PREC_C64)
