@@ -157,7 +157,7 @@ def residual_device(S, req: Request, x=None, lam=None, X=None, b=None):
 class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
-    def __init__(self, device=0, slots=4, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
+    def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60):
         self.device = device
