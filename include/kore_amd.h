@@ -177,6 +177,11 @@ int32_t kore_ws_gemm(kore_ws* ws, int32_t d0, int32_t nd, int32_t s0, int32_t ns
                      double alpha, double beta);
 /* V[:, dst+j] = V[:, src+j]. */
 int32_t kore_ws_copy(kore_ws* ws, int32_t src, int32_t dst, int32_t nc);
+/* V[:, c0:c0+nc] = pseudo-random entries (re, im uniform in [-1, 1)), a
+ * function of seed, column and row only, generated on the device.
+ * Replaces: SLEPc's random initial vector of EPS (BVSetRandomColumn), which
+ * Krylov-Schur uses when no initial space is given (bin/solve.py:123). */
+int32_t kore_ws_random(kore_ws* ws, int32_t c0, int32_t nc, uint64_t seed);
 /* Block Arnoldi expansion of the Krylov-Schur basis from column j0 to j1 in
  * blocks of bs: for j = j0, j0+bs, ...: W = OP V[:, j:j+bs] (in the scratch
  * block V[:, cw:cw+bs], cw >= j1+bs), two passes of classical Gram-Schmidt of

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("kore_ws_dot", _I32, [_P, _I32, _I32, _I32, _I32, _PD]),
     ("kore_ws_gemm", _I32, [_P, _I32, _I32, _I32, _I32, _PD, _D, _D]),
     ("kore_ws_copy", _I32, [_P, _I32, _I32, _I32]),
+    ("kore_ws_random", _I32, [_P, _I32, _I32, C.c_uint64]),
     ("kore_ws_expand", _I32, [_P, _I32, _I32, _I32, _I32, _D, _I32, _I32, _PD, _I64, C.POINTER(C.c_int32),
                               C.POINTER(SolveInfo)]),
 ]
@@ -341,6 +342,10 @@ class Workspace:
     def set(self, c0, X):
         X = np.asfortranarray(np.asarray(X, dtype=np.complex128).reshape(self.n, -1))
         _check(lib().kore_ws_set(self._p, int(c0), X.shape[1], _dptr(X), self.n))
+
+    def random(self, c0, nc, seed):
+        """Pseudo-random columns generated on the device (Krylov start block)."""
+        _check(lib().kore_ws_random(self._p, int(c0), int(nc), int(seed) & 0xFFFFFFFFFFFFFFFF))
 
     def get(self, c0, nc):
         X = np.zeros((self.n, nc), dtype=np.complex128, order="F")

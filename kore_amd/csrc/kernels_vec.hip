@@ -793,6 +793,34 @@ hipError_t launch_zero(cx<T>* p, int64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 template hipError_t launch_zero<double>(cx<double>*, int64_t, hipStream_t);
+
+// Pseudo-random columns: entry i of column c is (u0, u1), u uniform in
+// [-1, 1), from the SplitMix64 finaliser (Steele, Lea & Flood 2014) of
+// seed, c and i -- counter based, so the values do not depend on the launch
+// shape.  For Krylov start vectors, which need only be generic.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(256) void rand_cols_kernel(cx<double>* __restrict__ V, int64_t ldv, int64_t n,
+                                                        uint64_t seed) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  const uint64_t k = splitmix64(seed ^ splitmix64(((uint64_t)c << 40) ^ (uint64_t)i));
+  const double s = 1.0 / 2147483648.0;   // 2^-31
+  const double re = (double)(int32_t)(uint32_t)k * s, im = (double)(int32_t)(uint32_t)(k >> 32) * s;
+  V[(int64_t)c * ldv + i] = cx<double>{re, im};
+}
+
+hipError_t launch_rand_cols(cx<double>* V, int64_t ldv, int64_t n, int nc, uint64_t seed, hipStream_t s) {
+  if (n <= 0 || nc <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rand_cols_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nc), dim3(256), 0, s, V, ldv, n,
+                     seed);
+  return hipGetLastError();
+}
 template hipError_t launch_zero<float>(cx<float>*, int64_t, hipStream_t);
 
 __global__ void copy_cols_kernel(const cx<double>* __restrict__ x, int64_t ldx, cx<double>* __restrict__ y,

@@ -111,6 +111,7 @@ hipError_t launch_chol_inv(const cx<double>* G, int ldg, int n, cx<double>* R, c
 // fill complex buffer with zeros (graph-capture-safe clear)
 template <class T>
 hipError_t launch_zero(cx<T>* p, int64_t n, hipStream_t s);
+hipError_t launch_rand_cols(cx<double>* V, int64_t ldv, int64_t n, int nc, uint64_t seed, hipStream_t s);
 // copy columns (complex128)
 hipError_t launch_copy_cols(const cx<double>* x, int64_t ldx, cx<double>* y, int64_t ldy, int64_t n,
                             int nrhs, hipStream_t s);

@@ -1835,6 +1835,13 @@ int32_t kore_ws_gemm(kore_ws* w, int32_t d0, int32_t nd, int32_t s0, int32_t ns,
   return ws_gemm_dev(w, d0, nd, s0, ns, q, ns, alpha, beta);
 }
 
+int32_t kore_ws_random(kore_ws* w, int32_t c0, int32_t nc, uint64_t seed) {
+  TRY(ws_check(w, c0, nc));
+  kore_sys* s = w->sys;
+  HIP_TRY(launch_rand_cols(wcol(w, c0), s->n, s->n, nc, seed, s->ctx->stream));
+  return KORE_OK;
+}
+
 int32_t kore_ws_copy(kore_ws* w, int32_t src, int32_t dst, int32_t nc) {
   TRY(ws_check(w, src, nc));
   TRY(ws_check(w, dst, nc));

@@ -294,14 +294,17 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     host = {}
     ws = _Timed(ws_raw, host)
     rng = np.random.default_rng(seed)
-    if v0 is None:
-        X0 = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
+    if v0 is None and hasattr(ws_raw, "random"):
+        ws.random(cW, bs, seed)                  # generated on the device: no host pass, no upload
     else:
-        X0 = np.asarray(v0, dtype=complex).reshape(n, -1)
-        if X0.shape[1] < bs:
-            X0 = np.hstack([X0, rng.standard_normal((n, bs - X0.shape[1])) + 0j])
-        X0 = X0[:, :bs]
-    ws.set(cW, X0)
+        if v0 is None:
+            X0 = rng.standard_normal((n, bs)) + 1j * rng.standard_normal((n, bs))
+        else:
+            X0 = np.asarray(v0, dtype=complex).reshape(n, -1)
+            if X0.shape[1] < bs:
+                X0 = np.hstack([X0, rng.standard_normal((n, bs - X0.shape[1])) + 0j])
+            X0 = X0[:, :bs]
+        ws.set(cW, X0)
     _cholqr(ws, cW, 0, bs)
     H = np.zeros((m + bs, m), dtype=complex, order="F")
     j = 0

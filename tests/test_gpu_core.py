@@ -62,6 +62,23 @@ def test_ws_gemm_and_dot(ctx, hydro_small):
     S.close()
 
 
+def test_ws_random_start_block(ctx, hydro_small):
+    """Device-generated start columns: entries in [-1, 1) (re and im), the
+    same for the same seed, different across seeds and columns."""
+    par, A, B = hydro_small
+    S = _system(ctx, par, A, B)
+    ws = S.get_workspace(8)
+    ws.random(0, 3, 7)
+    ws.random(3, 1, 7)
+    ws.random(4, 1, 8)
+    V = ws.get(0, 5)
+    assert np.all(np.abs(V.real) <= 1.0) and np.all(np.abs(V.imag) <= 1.0)
+    assert np.allclose(V[:, 0], V[:, 3])
+    assert not np.allclose(V[:, 0], V[:, 4]) and not np.allclose(V[:, 0], V[:, 1])
+    assert abs(np.mean(V[:, :3].real)) < 0.1 and 0.25 < np.var(V[:, :3].imag) < 0.42
+    S.close()
+
+
 def _solve_case(ctx, par, A, B, sigma, precision, nrhs=1, seed=0):
     from kore_amd import backend
     S = _system(ctx, par, A, B)
