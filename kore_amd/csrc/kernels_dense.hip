@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>*
         for (int q = 0; q < 4; ++q) {
           const int gi = i0 + ii + q;
           ra[q] = czero<T>();
-          if (gk < K && gi < d.m) ra[q] = A[(int64_t)gk * lda + gi];
+          if (gk < K && gi < d.m) ra[q] = gld(A + (int64_t)gk * lda + gi);
         }
       } else {                 // A_eff(i,k) = conj(A_store(k,i)), A_store column-major K x m
         const int gi = i0 + (tid >> 2), kk = (tid & 3) * 4;
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>*
           const int gk = k0 + kk + q;
           ra[q] = czero<T>();
           if (gk < K && gi < d.m) {
-            ra[q] = A[(int64_t)gi * lda + gk];
+            ra[q] = gld(A + (int64_t)gi * lda + gk);
             ra[q].im = -ra[q].im;
           }
         }
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>*
       for (int q = 0; q < 4; ++q) {
         const int gk = k0 + kk + q;
         rb[q] = czero<T>();
-        if (gk < K && gj < d.n) rb[q] = B[(int64_t)gj * ldb + gk];
+        if (gk < K && gj < d.n) rb[q] = gld(B + (int64_t)gj * ldb + gk);
       }
     };
     auto valid = [&](int p) { return d.A[p] != nullptr && d.k[p] > 0; };
@@ -147,11 +147,11 @@ __global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>*
             cx<T>* c = d.C + (int64_t)gj * d.ldc + gi;
             cx<T> v{d.alpha * cre[rs][cs][r], d.alpha * cim[rs][cs][r]};
             if (d.beta != T(0)) {
-              cx<T> o = *c;
+              cx<T> o = gld(c);
               v.re += d.beta * o.re;
               v.im += d.beta * o.im;
             }
-            *c = v;
+            gst(c, v);
           }
         }
   }

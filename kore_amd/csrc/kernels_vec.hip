@@ -178,13 +178,76 @@ template <int NR> struct SkLds {
   static constexpr int value = x > p ? x : p;
 };
 
-// PRE (launches of few workgroups, where each product's load latency is the
-// cost): a product with K <= min(SkPreK, X chunk) -- 512 for one or two
-// right-hand sides, 256 otherwise -- has all of this thread's A entries and
-// its share of the X chunk requested before the first wait, so the product
-// costs one memory round trip instead of one per unrolled group.
+// PRE (the batched sweep kernel; the single-product kernel streams): a product
+// with K <= min(SkPreK, X chunk) -- 512 for one or two right-hand sides, 256
+// otherwise -- has all of this thread's A entries and its share of the X chunk
+// requested before the first wait, so the product costs one memory round trip
+// instead of one per unrolled group.
 constexpr int SK_PRE_K = 256;
 template <int NR> struct SkPreK { static constexpr int value = NR <= 2 ? 512 : SK_PRE_K; };   // k entries preloaded
+
+// PRE with one or two right-hand sides and every product's K <= 256 (the
+// sweeps of blocks up to 256 wide): the A entries and X chunks of all (up to
+// three) products are requested before the first wait, so a launch costs one
+// memory round trip instead of one per product.  Products are accumulated in
+// the same order, with the same alpha-scaled X values, as the per-product form.
+template <class T, int NR>
+__device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrhs, int row, int ks, cx<T> (&acc)[NR]) {
+  constexpr int NA = SK_PRE_K / SK_KS;               // A entries per thread and product
+  constexpr int NXL = (SK_PRE_K * NR + 255) / 256;   // X entries per thread and product
+  __shared__ cx<T> sXf[3][SK_PRE_K][NR];
+  cx<T> xr[3][NXL];
+  cx<T> a[3][NA];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const cx<T>* __restrict__ X = d.X[p];
+    const int K = d.A[p] != nullptr ? d.k[p] : 0;
+    const int64_t ldx = d.ldx[p];
+#pragma unroll
+    for (int t = 0; t < NXL; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      const int kk = idx / NR, c = idx - kk * NR;
+      xr[p][t] = czero<T>();
+      if (idx < SK_PRE_K * NR && kk < K && c < nrhs) xr[p][t] = gld(X + (int64_t)c * ldx + kk);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int K = d.A[p] != nullptr ? d.k[p] : 0;
+    const cx<T>* __restrict__ Ac = d.A[p] + row;
+    const int64_t lda = d.lda[p];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      const int kk = ks + SK_KS * t;
+      a[p][t] = czero<T>();
+      if (row < d.m && kk < K) a[p][t] = gld(Ac + (int64_t)kk * lda);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const T al = d.alpha[p];
+#pragma unroll
+    for (int t = 0; t < NXL; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      if (idx < SK_PRE_K * NR) {
+        const int kk = idx / NR, c = idx - kk * NR;
+        sXf[p][kk][c] = cx<T>{xr[p][t].re * al, xr[p][t].im * al};
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    if (d.A[p] == nullptr || d.k[p] <= 0) continue;   // uniform
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      const int kk = ks + SK_KS * t;                  // entries past K are zero on both sides
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a[p][t], sXf[p][kk][c]);
+    }
+  }
+}
+
 template <class T, int NR, bool PRE>
 __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
@@ -199,7 +262,14 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   cx<T> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = czero<T>();
-  for (int p = 0; p < 3; ++p) {
+  bool fused = false;
+  if constexpr (PRE && NR <= 2) {
+    fused = true;                                       // uniform over the workgroup
+    for (int p = 0; p < 3; ++p)
+      if (d.A[p] != nullptr && d.k[p] > SK_PRE_K) fused = false;
+    if (fused) skinny_fused_pre<T, NR>(d, nrhs, row, ks, acc);
+  }
+  for (int p = 0; p < 3 && !fused; ++p) {
     const cx<T>* __restrict__ A = d.A[p];
     const cx<T>* __restrict__ X = d.X[p];
     const int K = d.k[p];
@@ -218,7 +288,7 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
           const int idx = threadIdx.x + 256 * t;
           const int kk = idx / NR, c = idx - kk * NR;
           xr[t] = czero<T>();
-          if (idx < KC * NR && kk < K && c < nrhs) xr[t] = X[(int64_t)c * ldx + kk];
+          if (idx < KC * NR && kk < K && c < nrhs) xr[t] = gld(X + (int64_t)c * ldx + kk);
         }
         cx<T> a[NA];
         const cx<T>* __restrict__ Ac = A + row;
@@ -226,7 +296,7 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
         for (int t = 0; t < NA; ++t) {
           const int kk = ks + SK_KS * t;
           a[t] = czero<T>();
-          if (row < d.m && kk < K) a[t] = Ac[(int64_t)kk * lda];
+          if (row < d.m && kk < K) a[t] = gld(Ac + (int64_t)kk * lda);
         }
         __syncthreads();                  // earlier readers of the X chunk are done
 #pragma unroll
@@ -260,7 +330,7 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
         const int kk = idx / NR, c = idx - kk * NR;
         cx<T> v = czero<T>();
         if (kc0 + kk < K && c < nrhs) {
-          v = X[(int64_t)c * ldx + kc0 + kk];
+          v = gld(X + (int64_t)c * ldx + kc0 + kk);
           v.re *= al;
           v.im *= al;
         }
@@ -272,10 +342,10 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
         const cx<T>* __restrict__ Ac = A + (int64_t)kc0 * lda + row;
         int kk = ks;
         for (; kk + 3 * SK_KS < kend; kk += 4 * SK_KS) {
-          const cx<T> a0 = Ac[(int64_t)kk * lda];
-          const cx<T> a1 = Ac[(int64_t)(kk + SK_KS) * lda];
-          const cx<T> a2 = Ac[(int64_t)(kk + 2 * SK_KS) * lda];
-          const cx<T> a3 = Ac[(int64_t)(kk + 3 * SK_KS) * lda];
+          const cx<T> a0 = gld(Ac + (int64_t)kk * lda);
+          const cx<T> a1 = gld(Ac + (int64_t)(kk + SK_KS) * lda);
+          const cx<T> a2 = gld(Ac + (int64_t)(kk + 2 * SK_KS) * lda);
+          const cx<T> a3 = gld(Ac + (int64_t)(kk + 3 * SK_KS) * lda);
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
             acc[c] = cfma(acc[c], a0, sX[kk][c]);
@@ -285,7 +355,7 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
           }
         }
         for (; kk < kend; kk += SK_KS) {
-          const cx<T> a = Ac[(int64_t)kk * lda];
+          const cx<T> a = gld(Ac + (int64_t)kk * lda);
 #pragma unroll
           for (int c = 0; c < NR; ++c) acc[c] = cfma(acc[c], a, sX[kk][c]);
         }
@@ -308,19 +378,19 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
         cx<T>* y = d.Y + (int64_t)c * d.ldy + row;
         cx<T> o = v;
         if (d.beta != T(0)) {
-          const cx<T> old = *y;
+          const cx<T> old = gld(y);
           o.re += d.beta * old.re;
           o.im += d.beta * old.im;
         }
-        *y = o;
+        gst(y, o);
       }
     }
   }
 }
 
-template <class T, int NR, bool PRE>
+template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
-  skinny_tile<T, NR, PRE>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
+  skinny_tile<T, NR, true>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
 }
 
 // one product, descriptor passed by value (no descriptor upload)
@@ -353,30 +423,19 @@ hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int
   if (ndesc <= 0 || max_m <= 0) return hipSuccess;
   if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((max_m + SK_ROWS - 1) / SK_ROWS), (unsigned)ndesc);
-  // fewer workgroups than compute units: latency-bound, take the preloading form
-  if ((int64_t)grid.x * grid.y < 256) {
-    if (nrhs == 1)
-      hipLaunchKernelGGL((skinny_kernel<T, 1, true>), grid, dim3(256), 0, s, d_descs, nrhs);
-    else if (nrhs <= 2)
-      hipLaunchKernelGGL((skinny_kernel<T, 2, true>), grid, dim3(256), 0, s, d_descs, nrhs);
-    else if (nrhs <= 4)
-      hipLaunchKernelGGL((skinny_kernel<T, 4, true>), grid, dim3(256), 0, s, d_descs, nrhs);
-    else if (nrhs <= 8)
-      hipLaunchKernelGGL((skinny_kernel<T, 8, true>), grid, dim3(256), 0, s, d_descs, nrhs);
-    else
-      hipLaunchKernelGGL((skinny_kernel<T, 16, true>), grid, dim3(256), 0, s, d_descs, nrhs);
-    return hipGetLastError();
-  }
+  // the preloading form at every size: measured no slower than the streaming
+  // loop on the largest sweep levels (thousands of workgroups) and faster on
+  // the deep levels, where a launch holds a handful of blocks
   if (nrhs == 1)
-    hipLaunchKernelGGL((skinny_kernel<T, 1, false>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 1>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 2)
-    hipLaunchKernelGGL((skinny_kernel<T, 2, false>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 2>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 4)
-    hipLaunchKernelGGL((skinny_kernel<T, 4, false>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 4>), grid, dim3(256), 0, s, d_descs, nrhs);
   else if (nrhs <= 8)
-    hipLaunchKernelGGL((skinny_kernel<T, 8, false>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 8>), grid, dim3(256), 0, s, d_descs, nrhs);
   else
-    hipLaunchKernelGGL((skinny_kernel<T, 16, false>), grid, dim3(256), 0, s, d_descs, nrhs);
+    hipLaunchKernelGGL((skinny_kernel<T, 16>), grid, dim3(256), 0, s, d_descs, nrhs);
   return hipGetLastError();
 }
 template hipError_t launch_skinny<double>(const SkinnyDesc<double>*, int, int, int, hipStream_t);
@@ -580,7 +639,9 @@ hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<doubl
   if (na <= 0 || nb <= 0) return hipSuccess;
   if (nb > 16) return hipErrorInvalidValue;
   const dim3 grid((unsigned)nchunk, (unsigned)((na + 63) / 64));
-  if (nb <= 4)
+  if (nb == 1)        // one column (single-vector Krylov): a quarter of the accumulators, twice the occupancy
+    hipLaunchKernelGGL(tsdot_kernel<1>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);
+  else if (nb <= 4)
     hipLaunchKernelGGL(tsdot_kernel<4>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);
   else if (nb <= 8)
     hipLaunchKernelGGL(tsdot_kernel<8>, grid, dim3(256), 0, s, V, ldv, na, W, ldw, nb, n, rows_per_chunk, partial);

@@ -35,6 +35,24 @@ template <class T> __device__ __forceinline__ cx<T> cfma_conj(cx<T> acc, cx<T> a
   return acc;
 }
 template <class T> __device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
+
+// Global-memory access through pointers that the compiler cannot prove global
+// (those read from descriptor arrays in device memory).  Through a generic
+// pointer it emits flat loads, which count against the LDS/scalar wait counter
+// as well as the vector-memory one, so every LDS read or descriptor-field wait
+// would also wait for all outstanding loads and serialise their latencies.
+template <class T> struct cx_vec;
+template <> struct cx_vec<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct cx_vec<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <class T> __device__ __forceinline__ cx<T> gld(const cx<T>* p) {
+  typedef typename cx_vec<T>::type V;
+  const V v = *(const __attribute__((address_space(1))) V*)p;
+  return cx<T>{v.x, v.y};
+}
+template <class T> __device__ __forceinline__ void gst(cx<T>* p, cx<T> x) {
+  typedef typename cx_vec<T>::type V;
+  *(__attribute__((address_space(1))) V*)p = V{x.re, x.im};
+}
 template <class T> __device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.re) + fabs(a.im); }
 
 typedef double d4v __attribute__((ext_vector_type(4)));
