@@ -498,6 +498,7 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   // roles); afterwards any pivot of an earlier panel is "used"
   int role = -2;
   if (live) role = (p0 == 0 || roles[r] == -1) ? -1 : -2;
+  cx<T> myipv = czero<T>();               // this row's inverse pivot, if it becomes a pivot row
   cx<T> v[CPT];
 #pragma unroll
   for (int jj = 0; jj < CPT; ++jj) {
@@ -555,19 +556,19 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
       // Rows past n and columns past w hold zeros (and the pivot row's copy
       // too), so the update needs no row or column guard: zeros stay zeros.
       // Column c is held by slice c / CPT at register c % CPT.
-      if (r == p) {
+      // The pivot row is not scaled here: its column c becomes 1 and the row
+      // is multiplied by its inverse pivot once, after the panel (a later
+      // step updates it like any other row, which commutes with that scale).
+      // So every lane runs the same code, the pivot row with multiplier zero.
+      const bool isp = r == p;
+      if (isp) myipv = ipv;
+      cx<T> l = cmul(vc, ipv);
+      if (isp) l = czero<T>();
+      const cx<T> nl{-l.re, -l.im}, sc = isp ? cx<T>{T(1), T(0)} : nl;
 #pragma unroll
-        for (int jj = 0; jj < CPT; ++jj) {
-          const bool isc = jj == c % CPT && h == c / CPT;
-          v[jj] = isc ? ipv : cmul(v[jj], ipv);
-        }
-      } else {
-        const cx<T> l = cmul(vc, ipv), nl{-l.re, -l.im};
-#pragma unroll
-        for (int jj = 0; jj < CPT; ++jj) {
-          const cx<T> u = prow[h * SL + jj];
-          v[jj] = (jj == c % CPT && h == c / CPT) ? nl : cfma(v[jj], nl, u);   // four FMAs
-        }
+      for (int jj = 0; jj < CPT; ++jj) {
+        const cx<T> u = prow[h * SL + jj];
+        v[jj] = (jj == c % CPT && h == c / CPT) ? sc : cfma(v[jj], nl, u);   // four FMAs
       }
     }
   };
@@ -575,6 +576,10 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   for (int c = 0; c < WP / 2; ++c) step(c);
 #pragma unroll
   for (int c = WP / 2; c < WP; ++c) step(c);
+  if (role >= 0) {                        // pivot rows of this panel: the deferred scale
+#pragma unroll
+    for (int jj = 0; jj < CPT; ++jj) v[jj] = cmul(v[jj], myipv);
+  }
   __syncthreads();
   // C -> M[:, J]; R = M[P, :] outside J (disjoint columns, so the order of the
   // two does not matter; the update kernel has not run yet)
