@@ -193,12 +193,22 @@ int32_t kore_ws_random(kore_ws* ws, int32_t c0, int32_t nc, uint64_t seed);
  * converge (KORE_WARN_IR_NOT_CONVERGED).  KORE_WARN_BREAKDOWN: the Cholesky
  * factor of block j_done broke down (its Gram-Schmidt coefficients are in H,
  * the orthogonalised block in V[:, cw:cw+bs]).  *j_done = first column not
- * completed (j1 on success).
+ * completed (j1 on success, less after a stop request, see below).
  * Replaces: the Arnoldi expansion inside SLEPc's Krylov-Schur (STApply +
  * BVOrthogonalize per step) under E.solve() (bin/solve.py:123). */
 int32_t kore_ws_expand(kore_ws* ws, int32_t j0, int32_t j1, int32_t bs, int32_t cw, double ir_tol,
                        int32_t ir_maxit, int32_t ir_stop, double* H, int64_t ldh, int32_t* j_done,
                        kore_solve_info* info);
+/* j_end >= 0: a kore_ws_expand running on another thread (or started later)
+ * starts no block at a column >= j_end: it ends at j_end, or after the block in
+ * flight if it is already past (KORE_OK, *j_done = first column not
+ * completed); j_end < 0 clears the request, which stays set until then (clear
+ * it before each expansion).  Lets the host end an expansion when a
+ * convergence check made beside it has succeeded (j_end = 0), or at the column
+ * where the residual trend of that check predicts convergence.  No
+ * counterpart in the reference: SLEPc's Krylov-Schur checks convergence only
+ * at the end of each expansion to ncv (bin/solve.py:123). */
+int32_t kore_ws_stop_expand(kore_ws* ws, int32_t j_end);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@ There is no fallback: if ``libkore_amd.so`` is missing or cannot be loaded,
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
@@ -91,6 +92,7 @@ SIGNATURES = [
     ("kore_ws_gemm", _I32, [_P, _I32, _I32, _I32, _I32, _PD, _D, _D]),
     ("kore_ws_copy", _I32, [_P, _I32, _I32, _I32]),
     ("kore_ws_random", _I32, [_P, _I32, _I32, C.c_uint64]),
+    ("kore_ws_stop_expand", _I32, [_P, _I32]),
     ("kore_ws_expand", _I32, [_P, _I32, _I32, _I32, _I32, _D, _I32, _I32, _PD, _I64, C.POINTER(C.c_int32),
                               C.POINTER(SolveInfo)]),
 ]
@@ -159,6 +161,21 @@ def factor_bytes(lay: Layout, precision: int) -> int:
     return int(lib().kore_factor_bytes(C.byref(lay), int(precision)))
 
 
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """Contexts still open at interpreter exit are closed (with their systems
+    and workspaces) while the HIP runtime is intact, not left to object
+    finalisation at an unspecified point of the shutdown."""
+    for ctx in list(_LIVE_CONTEXTS):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """A device context: one HIP stream, a memory budget, kernel timers."""
 
@@ -168,6 +185,7 @@ class Context:
         _check(L.kore_ctx_create(int(device), int(mem_budget), C.byref(p)))
         self._p = p
         self._children = weakref.WeakSet()
+        _LIVE_CONTEXTS.add(self)
 
     @property
     def handle(self):
@@ -388,6 +406,12 @@ class Workspace:
                                   int(ir_stop), _dptr(H), H.shape[0], C.byref(jd), C.byref(info))
         _check(rc)
         return rc, int(jd.value), info
+
+    def stop_expand(self, j_end=0):
+        """Ask an expand() running on another thread to start no block at a
+        column >= j_end (0: end after the block in flight); None clears the
+        request (kore_ws_stop_expand)."""
+        _check(lib().kore_ws_stop_expand(self._p, -1 if j_end is None else int(j_end)))
 
     def close(self):
         if getattr(self, "_p", None):

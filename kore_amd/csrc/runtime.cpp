@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -398,6 +399,7 @@ struct kore_ws {
   cx<double>* hh = nullptr;  // pinned host copies of them, two slots (blocks in flight)
   hipEvent_t hev[2] = {nullptr, nullptr};
   Uploader up;
+  std::atomic<int> stop_at{INT32_MAX};  // set from another thread: the running expansion starts no block at or past it
   ~kore_ws() {
     if (hh) hipHostFree(hh);
     for (hipEvent_t e : hev)
@@ -1885,6 +1887,12 @@ static int32_t ws_expand_finish(kore_ws* w, int j, int bs, int slot, double* H, 
   return KORE_OK;
 }
 
+int32_t kore_ws_stop_expand(kore_ws* w, int32_t j_end) {
+  if (!w) return fail(KORE_ERR_ARG, "NULL workspace");
+  w->stop_at.store(j_end < 0 ? INT32_MAX : j_end, std::memory_order_release);
+  return KORE_OK;
+}
+
 int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t cw, double ir_tol, int32_t ir_maxit,
                        int32_t ir_stop, double* H, int64_t ldh, int32_t* j_done, kore_solve_info* info) {
   if (!w || !H || !j_done) return fail(KORE_ERR_ARG, "NULL argument");
@@ -1916,6 +1924,7 @@ int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t c
   int pending = -1, pslot = 0;   // block queued on the device whose coefficients are not yet read
   bool bd = false;
   for (int j = j0, slot = 0; j < j1; j += bs, slot ^= 1) {
+    if (j >= w->stop_at.load(std::memory_order_acquire)) break;   // the caller no longer needs the rest
     kore_solve_info li{};
     TRY(ws_apply_op_dev(w, j, cw, bs, ir_tol, ir_maxit, &li));
     if (info) {

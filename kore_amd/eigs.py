@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import concurrent.futures
 import dataclasses
+import threading
 import time
 
 import numpy as np
@@ -316,6 +317,7 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
     breakdowns = 0
     mixed = sysm.precision != PREC_FP64
     expand = getattr(ws, "expand", None) if hasattr(ws_raw, "expand") else None
+    stop_expand = getattr(ws_raw, "stop_expand", None) if expand is not None else None
     if expand is None:
         def expand(j0, j1, bs_, cw, Hm, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
             return expand_host(ws, j0, j1, bs_, cw, Hm, ir_tol, ir_maxit, ir_stop)
@@ -348,23 +350,30 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
                         # a chunk long enough to cover the check (Schur cost ~ j^3) ...
                         if t_col and t_rr:
                             chunk = max(chunk, int(np.ceil(t_rr * (j / j_rr) ** 3 / t_col / bs)) * bs)
-                        # ... and, once the worst wanted residual has been seen
-                        # falling at two checks, one that ends a base step short
-                        # of the column where the trend reaches tol, then base
-                        # steps (the chunk beside the final check is wasted)
+                        # ... unless the worst wanted residual, seen falling at
+                        # two checks, reaches tol on its log-linear trend before
+                        # that: then the chunk ends at the trend's column, where
+                        # the next check is the one expected to succeed (the
+                        # expansion beside a successful check is stopped).  The
+                        # trend is pessimistic as convergence speeds up.  A
+                        # trend column beyond the cover: a base step short of it.
                         if len(trend) >= 2 and trend[-1][1] < trend[-2][1]:
                             (ja, ra), (jb, rb) = trend[-2], trend[-1]
                             slope = (np.log(rb) - np.log(ra)) / (jb - ja)
                             j_conv = jb + (np.log(tol) - np.log(rb)) / slope
-                            chunk = check_every if j_conv - j <= 2 * check_every else \
-                                max(chunk, int((j_conv - j - check_every) / bs) * bs)
+                            to_conv = int(np.ceil((j_conv - j) / bs)) * bs
+                            chunk = max(bs, to_conv) if to_conv <= chunk else max(chunk, to_conv - check_every)
                         chunk = min(chunk, 4 * check_every)
                     j_stop = min(m, j + chunk)
                 elif check_every and not can_check:
                     j_stop = min(m, max(j_first, j + check_every))   # grow to where a check is worth it
+                if stop_expand is not None:
+                    stop_expand(None)
                 if threaded:
                     Hs = np.asfortranarray(H[:j + bs, :j])
+                    started = threading.Event()
                     def work(j0=j, j1=j_stop):
+                        started.set()
                         t = time.perf_counter()
                         r = expand(j0, j1, bs, cW, H, ir_tol=ir_tol, ir_maxit=ir_maxit if mixed else 0,
                                    ir_stop=fallback_sweeps or 0)
@@ -372,10 +381,28 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
                     if worker is None:                      # one thread per eigensolve, reused by every chunk
                         worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
                     fut = worker.submit(work)
+                    # the worker enters the library call (which releases the GIL)
+                    # before the check starts: LAPACK holds the GIL, and a worker
+                    # still waiting for it would leave the device idle
+                    started.wait()
                     try:
                         t = time.perf_counter()
                         early = _rayleigh_ritz(Hs, j, bs, sigma, which, tol, hidden)
                         t_rr, j_rr = time.perf_counter() - t, j
+                        if stop_expand is not None:
+                            if early.nconv >= nev:
+                                stop_expand(0)              # the rest of the chunk is not needed
+                            elif adaptive and trend:
+                                # this check's residual extends the trend: when it
+                                # now reaches tol before the chunk's end, the chunk
+                                # ends there instead (the next check is made there)
+                                wr = float(np.max(early.res[:nev] / np.abs(early.theta[:nev])))
+                                (ja, ra) = trend[-1]
+                                if np.isfinite(wr) and 0 < wr < ra:
+                                    slope = (np.log(wr) - np.log(ra)) / (j - ja)
+                                    j_end = j + max(1, int(np.ceil((np.log(tol) - np.log(wr)) / slope / bs))) * bs
+                                    if j_end < j_stop:
+                                        stop_expand(j_end)
                     finally:
                         (st, jd, ws_info), t_exp = fut.result()    # re-raises the worker's exception
                     if jd > j:

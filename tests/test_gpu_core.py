@@ -216,6 +216,72 @@ def test_expand_matches_host_driven(ctx, hydro_small, bs):
     S.close()
 
 
+def test_expand_stop_request(ctx, hydro_small):
+    """kore_ws_stop_expand from another thread ends a running expansion after
+    its current block: status OK, a whole number of blocks done, and those
+    columns (basis and projected matrix) equal an expansion that ran to the
+    end; a request made before the call ends it at once, an end column ends
+    it exactly there, and clearing the request lets the next call run in
+    full."""
+    import threading
+    import time
+    from kore_amd import backend
+    par, A, B = hydro_small
+    say("case: expand stop request")
+    S = _system(ctx, par, A, B)
+    S.factor(par.tau, 0)
+    bs, m = 1, 48
+    ncol = m + 2 * bs
+    rng = np.random.default_rng(12)
+    X0 = rng.standard_normal((S.n, bs)) + 1j * rng.standard_normal((S.n, bs))
+    X0 /= np.linalg.norm(X0)
+    ws = backend.Workspace(S, ncol)
+    ws.set(0, X0)
+    Hf = np.zeros((m + bs, m), dtype=complex, order="F")
+    st, jd, _ = ws.expand(0, m, bs, m + bs, Hf)
+    assert st == 0 and jd == m
+    Vf = ws.get(0, m + bs)
+    # stop requested before the call: nothing is expanded
+    ws.stop_expand(0)
+    H0 = np.zeros_like(Hf)
+    st, jd, _ = ws.expand(0, m, bs, m + bs, H0)
+    assert st == 0 and jd == 0
+    # stop requested while the expansion runs on another thread
+    ws.stop_expand(None)
+    ws.set(0, X0)
+    Hs = np.zeros_like(Hf)
+    res = {}
+    th = threading.Thread(target=lambda: res.update(r=ws.expand(0, m, bs, m + bs, Hs)))
+    th.start()
+    time.sleep(0.002)
+    ws.stop_expand(0)
+    th.join(timeout=60)
+    assert not th.is_alive()
+    st, jd, _ = res["r"]
+    say("  stopped at column", jd, "of", m)
+    assert st == 0 and 0 <= jd <= m and jd % bs == 0
+    if jd > 0:
+        Vs = ws.get(0, jd + bs)
+        assert np.abs(Hs[:jd + bs, :jd] - Hf[:jd + bs, :jd]).max() <= 1e-12 * np.abs(Hf).max()
+        assert np.abs(Vs - Vf[:, :jd + bs]).max() <= 1e-12
+    # an end column: the expansion stops exactly there
+    ws.stop_expand(20)
+    ws.set(0, X0)
+    He = np.zeros_like(Hf)
+    st, jd, _ = ws.expand(0, m, bs, m + bs, He)
+    assert st == 0 and jd == 20
+    assert np.abs(He[:21, :20] - Hf[:21, :20]).max() <= 1e-12 * np.abs(Hf).max()
+    # cleared: the next expansion runs in full
+    ws.stop_expand(None)
+    ws.set(0, X0)
+    H2 = np.zeros_like(Hf)
+    st, jd, _ = ws.expand(0, m, bs, m + bs, H2)
+    assert st == 0 and jd == m
+    assert np.abs(H2 - Hf).max() <= 1e-12 * np.abs(Hf).max()
+    ws.close()
+    S.close()
+
+
 def test_expand_breakdown_recovers(ctx, hydro_small):
     """Start block in the null space of B': the device Cholesky flags the
     breakdown, the host continues with a random block, and the eigenvalues
