@@ -227,6 +227,58 @@ struct Uploader {
   }
 };
 
+// Pinned staging for downloads: device -> pinned at DMA rate, then a copy into
+// the caller's (pageable) array split over a few host threads.
+struct Downloader {
+  char* host = nullptr;
+  size_t cap = 0;
+  ~Downloader() {
+    if (host) hipHostFree(host);
+  }
+  int32_t reserve(size_t bytes) {
+    if (bytes <= cap) return KORE_OK;
+    if (host) {
+      HIP_TRY(hipHostFree(host));
+      host = nullptr;
+      cap = 0;
+    }
+    HIP_TRY(hipHostMalloc((void**)&host, bytes, hipHostMallocDefault));
+    cap = bytes;
+    return KORE_OK;
+  }
+};
+
+struct CopySeg {
+  char* dst;
+  const char* src;
+  size_t bytes;
+};
+
+// memcpy of a list of segments, the total split evenly over up to 8 threads
+// (at least 1 MiB each)
+static void copy_parallel(const std::vector<CopySeg>& segs) {
+  size_t total = 0;
+  for (const CopySeg& g : segs) total += g.bytes;
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, total >> 20));
+  auto range = [&](size_t lo, size_t hi) {
+    size_t base = 0;
+    for (const CopySeg& g : segs) {
+      const size_t s0 = std::max(lo, base), s1 = std::min(hi, base + g.bytes);
+      if (s0 < s1) memcpy(g.dst + (s0 - base), g.src + (s0 - base), s1 - s0);
+      base += g.bytes;
+    }
+  };
+  if (nt <= 1) {
+    range(0, total);
+    return;
+  }
+  const size_t per = (total + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(range, t * per, std::min(total, (t + 1) * per));
+  range(0, per);
+  for (std::thread& x : th) x.join();
+}
+
 // ---------------------------------------------------------------------------
 // layout
 // ---------------------------------------------------------------------------
@@ -387,6 +439,7 @@ struct kore_sys {
   double sre = 0, sim = 0;
   std::vector<double> h_scale;
   Uploader up;
+  Downloader down;           // pinned staging of kore_ws_get
   DevBuf gm;                 // GMRES basis (restart+1 vectors), dot partials, coefficients
   int gm_cols = 0;
 };
@@ -1771,13 +1824,18 @@ int32_t kore_ws_get(kore_ws* w, int32_t c0, int32_t nc, double* host, int64_t ld
   kore_sys* s = w->sys;
   const int64_t n = s->n;
   if (ld < n) return fail(KORE_ERR_ARG, "ld < sizmat");
+  const size_t colb = (size_t)n * sizeof(zd);
+  TRY(s->down.reserve(colb * (size_t)std::min(nc, MAXR)));
+  std::vector<CopySeg> segs;
   for (int a = 0; a < nc; a += MAXR) {
     const int nr = std::min(MAXR, nc - a);
     HIP_TRY(launch_scatter(wcol(w, c0 + a), n, s->st, n, n, nr, s->d_perm.as<int64_t>(), s->ctx->stream));
-    for (int j = 0; j < nr; ++j)
-      HIP_TRY(hipMemcpyAsync(host + 2 * ld * (int64_t)(a + j), s->st + (size_t)j * n, n * sizeof(zd),
-                             hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s->down.host, s->st, nr * colb, hipMemcpyDeviceToHost, s->ctx->stream));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    segs.clear();
+    for (int j = 0; j < nr; ++j)
+      segs.push_back(CopySeg{reinterpret_cast<char*>(host + 2 * ld * (int64_t)(a + j)), s->down.host + j * colb, colb});
+    copy_parallel(segs);
   }
   return KORE_OK;
 }

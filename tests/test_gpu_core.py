@@ -62,6 +62,26 @@ def test_ws_gemm_and_dot(ctx, hydro_small):
     S.close()
 
 
+def test_ws_get_large_columns(ctx):
+    """Downloads large enough to take the pinned staging and the multi-thread
+    host copy (several MiB per chunk of 16 columns, a partial last chunk)
+    return the uploaded columns exactly, in Kore's ordering."""
+    from kore_amd.params import Params
+    from kore_amd import assembly, backend
+    par = Params(m=1, symm=1, Ek=1e-4, N=100, lmax=150)
+    A, B, _ = assembly.assemble(par)
+    S = _system(ctx, par, A, B)
+    say("case: ws get, sizmat", S.n)
+    ws = backend.Workspace(S, 40)
+    rng = np.random.default_rng(4)
+    V = rng.standard_normal((S.n, 37)) + 1j * rng.standard_normal((S.n, 37))
+    ws.set(2, V)
+    assert np.array_equal(ws.get(2, 37), V)
+    assert np.array_equal(ws.get(20, 3), V[:, 18:21])
+    ws.close()
+    S.close()
+
+
 def test_ws_random_start_block(ctx, hydro_small):
     """Device-generated start columns: entries in [-1, 1) (re and im), the
     same for the same seed, different across seeds and columns."""
