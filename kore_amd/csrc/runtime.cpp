@@ -1825,7 +1825,7 @@ int32_t kore_ws_get(kore_ws* w, int32_t c0, int32_t nc, double* host, int64_t ld
   const int64_t n = s->n;
   if (ld < n) return fail(KORE_ERR_ARG, "ld < sizmat");
   const size_t colb = (size_t)n * sizeof(zd);
-  TRY(s->down.reserve(colb * (size_t)std::min(nc, MAXR)));
+  TRY(s->down.reserve(colb * MAXR));   // once per system: a pinned allocation is not cheap
   std::vector<CopySeg> segs;
   for (int a = 0; a < nc; a += MAXR) {
     const int nr = std::min(MAXR, nc - a);
