@@ -95,6 +95,35 @@ class NumpyWorkspace:
         pass
 
 
+class StoppableNumpyWorkspace(NumpyWorkspace):
+    """NumpyWorkspace with the library's expansion contract: expand() block by
+    block (host-driven), honouring stop_expand(j_end) requests made from
+    another thread between blocks, as kore_ws_expand / kore_ws_stop_expand."""
+
+    def __init__(self, sysm, ncol):
+        super().__init__(sysm, ncol)
+        self.stop_at = None
+        self.requests = []
+        self.stopped = []
+
+    def stop_expand(self, j_end=0):
+        self.stop_at = j_end
+        if j_end is not None:
+            self.requests.append(j_end)
+
+    def expand(self, j0, j1, bs, cw, H, ir_tol=1e-14, ir_maxit=0, ir_stop=0):
+        j = j0
+        while j < j1:
+            if self.stop_at is not None and j >= self.stop_at:
+                self.stopped.append((j0, j1, j))
+                break
+            st, jd, info = eigs.expand_host(self, j, j + bs, bs, cw, H, ir_tol, ir_maxit, ir_stop)
+            if st != 0:
+                return st, jd, info
+            j = jd
+        return 0, j, eigs._Info(1, True)
+
+
 class NumpySystem:
     def __init__(self, A, B, sigma):
         self.n = A.shape[0]
@@ -171,6 +200,43 @@ def test_early_convergence_checks():
         for i in range(4):
             assert abs(r.lam[i] - w[i]) / abs(w[i]) <= 1e-10
         assert eigs.residuals(A, B, r.lam[:4], r.X[:, :4]).max() <= 1e-12
+
+
+def test_expansion_stop_requests():
+    """With a workspace that honours stop requests (as the device library
+    does), the adaptive checks end an expansion chunk once a check beside it
+    has converged, or at the column where the residual trend reaches tol:
+    requests are made, the eigenpairs still match dense QZ with residuals at
+    the tolerance, and fewer OP columns are applied than a full expansion."""
+    par = Params(m=1, symm=1, Ek=1e-3, N=16, lmax=20)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.9j
+    w = la.eig(A.toarray(), B.toarray(), right=False)
+    w = w[np.isfinite(w)]
+    w = w[np.argsort(np.abs(w - sigma))]
+    S = NumpySystem(A, B, sigma)
+    made = []
+
+    def ws_factory(sysm, ncol):
+        ws = StoppableNumpyWorkspace(sysm, ncol)
+        made.append(ws)
+        return ws
+
+    full = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=1, check_every=0, workspace=NumpyWorkspace)
+    r = eigs.eigs(S, sigma, nev=4, ncv=120, tol=1e-12, block=1, workspace=ws_factory)
+    ws = made[-1]
+    assert ws.requests and ws.requests[-1] == 0          # the converged check stops its chunk
+    assert r.nconv >= 4 and r.restarts == 0 and r.op_columns < full.op_columns
+    for i in range(4):
+        assert abs(r.lam[i] - w[i]) / abs(w[i]) <= 1e-10
+    assert eigs.residuals(A, B, r.lam[:4], r.X[:, :4]).max() <= 1e-12
+    # an end column set before the call ends the expansion exactly there
+    ws2 = StoppableNumpyWorkspace(S, 12)
+    ws2.set(0, np.ones((S.n, 1)) / np.sqrt(S.n))
+    H = np.zeros((11, 10), dtype=complex, order="F")
+    ws2.stop_expand(6)
+    st, jd, _ = ws2.expand(0, 10, 1, 11, H)
+    assert st == 0 and jd == 6 and ws2.stopped == [(0, 10, 6)]
 
 
 def test_triangular_eigenvectors():
