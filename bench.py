@@ -112,7 +112,7 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
 # HBM bytes per solve-sweep dispatch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE (scaled by a plain 1 GiB copy) on the default workload (FP64,
 # Krylov block 1); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
-MEASURED_SWEEP_TRAFFIC = {("hydro_E1e-6_N200", 0, 1): 57.72e6}
+MEASURED_SWEEP_TRAFFIC = {}
 
 
 def roofline(run, precision, workload=None, block=None):
@@ -128,6 +128,7 @@ def roofline(run, precision, workload=None, block=None):
         cand["skinny"] = (ms, dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
                                    kernel="skinny_kernel (cyclic-reduction solve sweep)",
                                    per_launch_bytes=f.solve_bytes / max(1, f.solve_launches),
+                                   launches_per_sweep=f.solve_launches,
                                    avg_launch_ms=ms / cnt))
     peak = PEAK_FP32 if precision else PEAK_FP64
     for key, flops in (("inv", f.flops_inv), ("gemm", f.flops_gemm)):

@@ -23,6 +23,7 @@
 #include <unordered_map>
 #include <vector>
 #include <thread>
+#include <type_traits>
 
 #include "kore_amd.h"
 #include "kore_device.hpp"
@@ -401,6 +402,16 @@ template <class T> struct Factor : FactorBase {
   GemmDesc<T>* tail2 = nullptr;     // Q
   int tail_tiles1 = 0, tail_tiles2 = 0, tail_m = 0;
   SkinnyDesc<T>* tail_solve = nullptr;   // two descriptors
+  // Level 0 of a sweep with one right-hand side and an FP64 factor (at least
+  // two levels before the tail): its couplings L, U are the sparse
+  // off-diagonal blocks of the matrix itself, so the sweep applies them from
+  // the CSR and reads only the level's inverse blocks, twice:
+  //   forward   y_odd = Dinv_odd b_odd;  b_even -= L y  (sparse)
+  //   backward  b_odd -= L x_even (sparse);  x_odd = Dinv_odd b_odd
+  // instead of E, F (forward) and Dinv, G, H (backward) once each.
+  bool lev0_sparse = false;
+  SkinnyDesc<T>* lev0_dinv = nullptr;    // x_odd (or y_odd) = Dinv_odd b_odd, one descriptor per odd block
+  int lev0_n = 0, lev0_m = 0;
   cx<T>** dD = nullptr;
   cx<T>** dL = nullptr;
   cx<T>** dU = nullptr;
@@ -421,6 +432,8 @@ struct kore_sys {
   std::vector<int64_t> perm;
   std::vector<int32_t> h_ip, h_blk;
   int64_t nnz = 0;                 // entries of the union pattern (device arrays only)
+  int64_t off_nnz[2] = {0, 0};     // entries outside the diagonal block, rows of even / odd blocks
+  int64_t off_rows[2] = {0, 0};    // rows of even / odd blocks
   bool has_b = false;
   DevBuf d_ip, d_ix, d_va, d_vb, d_perm, d_blk, d_boff, d_scale, d_scr, d_part;
   CsrDev csr{};
@@ -600,6 +613,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     d.Y = Y; d.ldy = ldv; d.m = m; d.beta = beta;
     return d;
   };
+  double lev0_bytes = 0;          // sweep bytes of level 0 (forward and backward)
   for (int l = 0; l < nlev; ++l) {
     const auto& id = ids[l];
     const int T_ = (int)id.size();
@@ -678,6 +692,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     }
     rg2[l].n = vg.size() - rg2[l].off;
     // solve: forward  B_t -= E_t B_{t-1} + F_t B_{t+1}   (t even)
+    const double sb_level = F->solve_bytes;
     rfw[l].off = vs.size();
     for (int t = 0; t < T_; t += 2) {
       const int k = id[t];
@@ -708,6 +723,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       L.mbw = std::max(L.mbw, m);
     }
     rbw[l].n = vs.size() - rbw[l].off;
+    if (l == 0) lev0_bytes = F->solve_bytes - sb_level;
   }
   // root
   const int kr = ids[nlev][0];
@@ -748,6 +764,29 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   }
   F->flops = flops;
   F->solve_launches = F->tail ? 2 * nlev - 1 : 2 * nlev + 1;
+  size_t rlev0 = 0;
+  if constexpr (std::is_same<T, double>::value) {
+    if (F->tail && nlev >= 2) {
+      const double sb = F->solve_bytes;
+      const auto& id = ids[0];
+      rlev0 = vs.size();
+      for (int t = 1; t < (int)id.size(); t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        vs.push_back(mkskin(vecp(F->Xv, k), m, {Prod{F->D[k], m, vecp(F->Bv, k), m, T(1)}}, T(0)));
+        F->lev0_m = std::max(F->lev0_m, m);
+      }
+      F->lev0_n = (int)(vs.size() - rlev0);
+      const double dinv_bytes = F->solve_bytes - sb;
+      // sparse launches: each off-block entry's index, two values and the
+      // vector entry it multiplies; per row its pointer, scale, block id and y
+      const double sparse_bytes = 52.0 * (double)(s->off_nnz[0] + s->off_nnz[1]) +
+                                  52.0 * (double)(s->off_rows[0] + s->off_rows[1]);
+      F->solve_bytes = sb - lev0_bytes + 2 * dinv_bytes + sparse_bytes;
+      F->solve_launches += 2;
+      F->lev0_sparse = true;
+    }
+  }
 
   // upload descriptors
   const size_t bi = vinv.size() * sizeof(InvDesc<T>);
@@ -780,6 +819,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   }
   F->root_inv = dinv + rinv[nlev].off;
   F->root_solve = dsk + rroot;
+  if (F->lev0_sparse) F->lev0_dinv = dsk + rlev0;
   if (F->tail) {
     F->tail1 = dg + rtail1;
     F->tail2 = dg + rtail2;
@@ -856,8 +896,22 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int ltop = F->tail ? F->nlev - 1 : F->nlev;   // levels swept one by one
+  const bool sp0 = F->lev0_sparse && nrhs == 1;        // level 0 through the sparse couplings
   for (int l = 0; l < ltop; ++l) {
     Level<T>& L = F->lev[l];
+    if constexpr (std::is_same<T, double>::value) {
+      if (l == 0 && sp0) {
+        {
+          Timed tm(c, KORE_TIMER_SKINNY);
+          HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // y_odd in Xv
+        }
+        Timed tm(c, KORE_TIMER_SKINNY);
+        HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_blk.as<int32_t>(),
+                               s->d_boff.as<int64_t>(), 0, F->Xv,
+                               F->Bv, st));
+        continue;
+      }
+    }
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
   }
@@ -870,6 +924,19 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
   }
   for (int l = ltop - 1; l >= 0; --l) {
     Level<T>& L = F->lev[l];
+    if constexpr (std::is_same<T, double>::value) {
+      if (l == 0 && sp0) {
+        {
+          Timed tm(c, KORE_TIMER_SKINNY);
+          HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_blk.as<int32_t>(),
+                                 s->d_boff.as<int64_t>(), 1, F->Xv,
+                                 F->Bv, st));
+        }
+        Timed tm(c, KORE_TIMER_SKINNY);
+        HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));     // x_odd in Xv
+        continue;
+      }
+    }
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.bw, L.nbw, L.mbw, nrhs, st));
   }
@@ -1307,7 +1374,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   int32_t* pix = reinterpret_cast<int32_t*>(pvbb + capb);
   int32_t* pixb = pix + cap;
   std::vector<size_t> tcnt(nth, 0), tcntb(nth, 0);
-  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0);
+  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
   auto build_rows = [&](int t) {
@@ -1363,6 +1430,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         ovb[cntt] = b;
         ++cntt;
         ++cnt;
+        if (kc != kb) ++rowoff[i];
         if (b != zd(0, 0)) {
           oixb[cnttb] = col;
           ovbb[cnttb] = b;
@@ -1391,6 +1459,10 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
                   (long long)bad_row[t], bad_kb[t], bad_kc[t]);
   s->h_ip.assign(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) s->h_ip[i + 1] = s->h_ip[i] + rowcnt[i];
+  for (int64_t i = 0; i < n; ++i) {
+    s->off_nnz[s->h_blk[i] & 1] += rowoff[i];
+    s->off_rows[s->h_blk[i] & 1] += 1;
+  }
   const size_t nnz = (size_t)s->h_ip[n];
   s->nnz = (int64_t)nnz;
   TRY(s->d_ip.alloc(ctx, (n + 1) * sizeof(int32_t)));

@@ -136,11 +136,13 @@ def test_solve_hydro_small(ctx, hydro_small, precision):
 
 @pytest.mark.parametrize("case", ["hydro_N100", "hydro_N200", "thermal_N60", "hydro_N300"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_solve_large_blocks(ctx, case, precision):
+@pytest.mark.parametrize("nrhs", [1, 2])
+def test_solve_large_blocks(ctx, case, precision, nrhs):
     """Degree blocks of 96-512 rows take the multi-workgroup inversion (panel
     kernel + tiled MFMA update) and GEMM edge tiles of sizes that are not
     multiples of 64: 100, 200, 300, and thermal 120 / 60 in one level list.
-    Solves match SuperLU to 1e-10."""
+    One right-hand side with the FP64 factor takes level 0 of the sweep
+    through the sparse couplings.  Solves match SuperLU to 1e-10."""
     from kore_amd.params import Params
     from kore_amd import assembly
     kw = dict(hydro_N100=dict(N=100, lmax=14), hydro_N200=dict(N=200, lmax=10), hydro_N300=dict(N=300, lmax=8),
@@ -148,7 +150,7 @@ def test_solve_large_blocks(ctx, case, precision):
     par = Params(m=1, symm=1, Ek=1e-4, **kw)
     A, B, _ = assembly.assemble(par)
     say("case: large-block solve", case, "sizmat", A.shape[0], "precision", precision)
-    err, info = _solve_case(ctx, par, A, B, 0.4j, precision, nrhs=2, seed=7)
+    err, info = _solve_case(ctx, par, A, B, 0.4j, precision, nrhs=nrhs, seed=7)
     say("  rel err", err, "sweeps", info.iterations)
     assert err.max() <= 1e-10
 
@@ -161,14 +163,15 @@ def test_solve_many_rhs(ctx, hydro_small):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_eigs_hydro_small(ctx, hydro_small, precision):
+@pytest.mark.parametrize("block", [1, 4])
+def test_eigs_hydro_small(ctx, hydro_small, precision, block):
     from kore_amd import eigs
     from oracle import krylov_schur
     par, A, B = hydro_small
-    say("case: hydro_small eigs precision", precision)
+    say("case: hydro_small eigs precision", precision, "block", block)
     S = _system(ctx, par, A, B)
     S.factor(par.tau, precision)
-    r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=4)
+    r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=block)
     say("  nconv", r.nconv, "restarts", r.restarts, "ops", r.op_columns)
     lu = sla.splu((A - par.tau * B).tocsc())
     lam_o, X_o, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], par.tau, 5, tol=1e-13,
