@@ -93,21 +93,21 @@ hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re
 // i.e. y_t -= L_t v_{t-1} + U_t v_{t+1} with L, U the off-diagonal blocks of
 // the row-equilibrated level-0 system R (A - cs B).  16 lanes per row.
 __global__ __launch_bounds__(256) void offdiag_kernel(CsrDev m, cx<double> cs, const double* __restrict__ scale,
-                                                      const int32_t* __restrict__ blk,
-                                                      const int64_t* __restrict__ boff, int parity,
+                                                      const int32_t* __restrict__ split, int parity,
                                                       const cx<double>* __restrict__ v, cx<double>* __restrict__ y) {
   const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
   const int64_t row = (int64_t)blockIdx.x * 16 + g;
   if (row >= m.n) return;
-  const int kb = blk[row];
-  if ((kb & 1) != parity) return;                   // uniform over the row's 16 lanes
-  const int64_t lo = boff[kb], hi = boff[kb + 1];   // the row's diagonal block: columns [lo, hi)
+  const int32_t sp = split[row];
+  if (((sp >> 30) & 1) != parity) return;          // uniform over the row's 16 lanes
+  const int nlo = sp & 0x7FFF, nhi = (sp >> 15) & 0x7FFF;
   const int s = m.indptr[row], e = m.indptr[row + 1];
+  // the row's entries are sorted by column: those of the neighbouring blocks
+  // are its first nlo and last nhi
   cx<double> acc = czero<double>();
-  for (int k = s + gl; k < e; k += 16) {
-    const int j = m.indices[k];
-    if (j >= lo && j < hi) continue;
-    acc = cfma(acc, csub(m.va[k], cmul(cs, m.vb[k])), v[j]);
+  for (int q = gl; q < nlo + nhi; q += 16) {
+    const int k = q < nlo ? s + q : e - nhi + (q - nlo);
+    acc = cfma(acc, csub(m.va[k], cmul(cs, m.vb[k])), v[m.indices[k]]);
   }
 #pragma unroll
   for (int off = 8; off >= 1; off >>= 1) {
@@ -121,11 +121,11 @@ __global__ __launch_bounds__(256) void offdiag_kernel(CsrDev m, cx<double> cs, c
   }
 }
 
-hipError_t launch_offdiag(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* blk,
-                          const int64_t* boff, int parity, const cx<double>* v, cx<double>* y, hipStream_t s) {
+hipError_t launch_offdiag(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,
+                          int parity, const cx<double>* v, cx<double>* y, hipStream_t s) {
   if (m.n == 0) return hipSuccess;
   hipLaunchKernelGGL(offdiag_kernel, dim3((unsigned)((m.n + 15) / 16)), dim3(256), 0, s, m, cx<double>{sre, sim},
-                     scale, blk, boff, parity, v, y);
+                     scale, split, parity, v, y);
   return hipGetLastError();
 }
 

@@ -67,8 +67,8 @@ struct CsrDev {
 
 // Y[:, c] = beta*Y[:, c] + (ca*A + cb*B) X[:, c],  c < nrhs  (complex128)
 // rows of blocks of the given parity: y -= scale * (off-diagonal blocks of A - (sre + i sim) B) v
-hipError_t launch_offdiag(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* blk,
-                          const int64_t* boff, int parity, const cx<double>* v, cx<double>* y, hipStream_t s);
+hipError_t launch_offdiag(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,
+                          int parity, const cx<double>* v, cx<double>* y, hipStream_t s);
 hipError_t launch_spmv(const CsrDev& m, double ca_re, double ca_im, double cb_re, double cb_im,
                        const cx<double>* X, int64_t ldx, cx<double>* Y, int64_t ldy, int nrhs,
                        double beta, hipStream_t s, const double* rscale = nullptr);

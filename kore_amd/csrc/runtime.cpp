@@ -434,6 +434,10 @@ struct kore_sys {
   int64_t nnz = 0;                 // entries of the union pattern (device arrays only)
   int64_t off_nnz[2] = {0, 0};     // entries outside the diagonal block, rows of even / odd blocks
   int64_t off_rows[2] = {0, 0};    // rows of even / odd blocks
+  // per row: entries before its diagonal block (bits 0-14), after it (bits
+  // 15-29), block parity (bit 30); the row's entries are sorted by column
+  std::vector<int32_t> h_split;
+  DevBuf d_split;
   bool has_b = false;
   DevBuf d_ip, d_ix, d_va, d_vb, d_perm, d_blk, d_boff, d_scale, d_scr, d_part;
   CsrDev csr{};
@@ -906,8 +910,7 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
           HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // y_odd in Xv
         }
         Timed tm(c, KORE_TIMER_SKINNY);
-        HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_blk.as<int32_t>(),
-                               s->d_boff.as<int64_t>(), 0, F->Xv,
+        HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 0, F->Xv,
                                F->Bv, st));
         continue;
       }
@@ -928,8 +931,7 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
       if (l == 0 && sp0) {
         {
           Timed tm(c, KORE_TIMER_SKINNY);
-          HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_blk.as<int32_t>(),
-                                 s->d_boff.as<int64_t>(), 1, F->Xv,
+          HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 1, F->Xv,
                                  F->Bv, st));
         }
         Timed tm(c, KORE_TIMER_SKINNY);
@@ -1374,7 +1376,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   int32_t* pix = reinterpret_cast<int32_t*>(pvbb + capb);
   int32_t* pixb = pix + cap;
   std::vector<size_t> tcnt(nth, 0), tcntb(nth, 0);
-  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0);
+  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0), rowlo(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
   auto build_rows = [&](int t) {
@@ -1431,6 +1433,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         ++cntt;
         ++cnt;
         if (kc != kb) ++rowoff[i];
+        if (kc < kb) ++rowlo[i];
         if (b != zd(0, 0)) {
           oixb[cnttb] = col;
           ovbb[cnttb] = b;
@@ -1459,9 +1462,14 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
                   (long long)bad_row[t], bad_kb[t], bad_kc[t]);
   s->h_ip.assign(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) s->h_ip[i + 1] = s->h_ip[i] + rowcnt[i];
+  s->h_split.resize(n);
   for (int64_t i = 0; i < n; ++i) {
     s->off_nnz[s->h_blk[i] & 1] += rowoff[i];
     s->off_rows[s->h_blk[i] & 1] += 1;
+    const int32_t lo = rowlo[i], hi = rowoff[i] - rowlo[i];
+    if (lo >= (1 << 15) || hi >= (1 << 15))
+      return fail(KORE_ERR_UNSUPPORTED, "row %lld has more than 32767 entries in a neighbouring block", (long long)i);
+    s->h_split[i] = lo | (hi << 15) | ((s->h_blk[i] & 1) << 30);
   }
   const size_t nnz = (size_t)s->h_ip[n];
   s->nnz = (int64_t)nnz;
@@ -1479,6 +1487,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   }
   TRY(s->d_perm.alloc(ctx, n * sizeof(int64_t)));
   TRY(s->d_blk.alloc(ctx, n * sizeof(int32_t)));
+  TRY(s->d_split.alloc(ctx, n * sizeof(int32_t)));
   TRY(s->d_boff.alloc(ctx, (s->nblk + 1) * sizeof(int64_t)));
   TRY(s->d_scale.alloc(ctx, n * sizeof(double)));
   TRY(s->d_scr.alloc(ctx, 4 * (size_t)n * MAXR * sizeof(cx<double>)));
@@ -1515,6 +1524,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   }
   HIP_TRY(hipMemcpyAsync(s->d_perm.p, s->perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_blk.p, s->h_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_split.p, s->h_split.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_boff.p, s->boff.data(), (s->nblk + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   s->csr = CsrDev{n, s->d_ip.as<int32_t>(), s->d_ix.as<int32_t>(), s->d_va.as<cx<double>>(), s->d_vb.as<cx<double>>()};
