@@ -67,8 +67,9 @@ typedef struct {
   double flops;            /* algorithmic real flops of the factorisation actually run */
   double flops_inv;        /*   of which block inversions (8 n^3 each)     */
   double flops_gemm;       /*   of which Schur-complement GEMMs (8 m n k)  */
-  double solve_bytes;      /* factor bytes one solve sweep reads          */
-  int32_t solve_launches;  /* kernel launches of one solve sweep          */
+  double solve_bytes;      /* bytes one single-column solve sweep reads: factor blocks, plus, with an
+                              FP64 factor, level 0's sparse couplings read from the matrix */
+  int32_t solve_launches;  /* kernel launches of one single-column solve sweep */
   int64_t bytes;           /* device bytes held by the factor */
   double ms;               /* device time, HIP events on the library stream */
 } kore_factor_info;
