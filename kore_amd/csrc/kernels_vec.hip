@@ -236,15 +236,16 @@ template <int NR> struct SkPreK { static constexpr int value = NR <= 2 ? 512 : S
 // three) products are requested before the first wait, so a launch costs one
 // memory round trip instead of one per product.  Products are accumulated in
 // the same order, with the same alpha-scaled X values, as the per-product form.
-template <class T, int NR>
+// NP: products held in registers (3; 1 for launches whose descriptors hold one).
+template <class T, int NR, int NP>
 __device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrhs, int row, int ks, cx<T> (&acc)[NR]) {
   constexpr int NA = SK_PRE_K / SK_KS;               // A entries per thread and product
   constexpr int NXL = (SK_PRE_K * NR + 255) / 256;   // X entries per thread and product
-  __shared__ cx<T> sXf[3][SK_PRE_K][NR];
-  cx<T> xr[3][NXL];
-  cx<T> a[3][NA];
+  __shared__ cx<T> sXf[NP][SK_PRE_K][NR];
+  cx<T> xr[NP][NXL];
+  cx<T> a[NP][NA];
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
+  for (int p = 0; p < NP; ++p) {
     const cx<T>* __restrict__ X = d.X[p];
     const int K = d.A[p] != nullptr ? d.k[p] : 0;
     const int64_t ldx = d.ldx[p];
@@ -257,7 +258,7 @@ __device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrh
     }
   }
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
+  for (int p = 0; p < NP; ++p) {
     const int K = d.A[p] != nullptr ? d.k[p] : 0;
     const cx<T>* __restrict__ Ac = d.A[p] + row;
     const int64_t lda = d.lda[p];
@@ -269,7 +270,7 @@ __device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrh
     }
   }
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
+  for (int p = 0; p < NP; ++p) {
     const T al = d.alpha[p];
 #pragma unroll
     for (int t = 0; t < NXL; ++t) {
@@ -282,7 +283,7 @@ __device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrh
   }
   __syncthreads();
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
+  for (int p = 0; p < NP; ++p) {
     if (d.A[p] == nullptr || d.k[p] <= 0) continue;   // uniform
 #pragma unroll
     for (int t = 0; t < NA; ++t) {
@@ -293,7 +294,7 @@ __device__ __forceinline__ void skinny_fused_pre(const SkinnyDesc<T>& d, int nrh
   }
 }
 
-template <class T, int NR, bool PRE>
+template <class T, int NR, bool PRE, int NP = 3>
 __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   constexpr int KC = SkChunk<NR>::value;
   const int r0 = blockIdx.x * SK_ROWS;
@@ -311,8 +312,8 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
   if constexpr (PRE && NR <= 2) {
     fused = true;                                       // uniform over the workgroup
     for (int p = 0; p < 3; ++p)
-      if (d.A[p] != nullptr && d.k[p] > SK_PRE_K) fused = false;
-    if (fused) skinny_fused_pre<T, NR>(d, nrhs, row, ks, acc);
+      if (d.A[p] != nullptr && (d.k[p] > SK_PRE_K || p >= NP)) fused = false;
+    if (fused) skinny_fused_pre<T, NR, NP>(d, nrhs, row, ks, acc);
   }
   for (int p = 0; p < 3 && !fused; ++p) {
     const cx<T>* __restrict__ A = d.A[p];
@@ -322,7 +323,10 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
     const int64_t lda = d.lda[p], ldx = d.ldx[p];
     const T al = d.alpha[p];
     if constexpr (PRE) {
-      constexpr int KP = SkPreK<NR>::value < KC ? SkPreK<NR>::value : KC;  // k entries staged at once
+      // k entries staged at once (one-product launches keep to the short form,
+      // so that their register count stays low; longer products stream)
+      constexpr int KPP = NP == 1 ? SK_PRE_K : SkPreK<NR>::value;
+      constexpr int KP = KPP < KC ? KPP : KC;
       // the product, with NA A entries per thread (NA * 16 >= K)
       auto product = [&](auto na) {
         constexpr int NA = decltype(na)::value;
@@ -437,6 +441,23 @@ template <class T, int NR>
 __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __restrict__ descs, int nrhs) {
   skinny_tile<T, NR, true>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
 }
+
+// one right-hand side and one product per descriptor: a third of the
+// registers of the three-product form, so more workgroups per CU keep loads
+// in flight
+template <class T>
+__global__ __launch_bounds__(256) void skinny_one_kernel(const SkinnyDesc<T>* __restrict__ descs) {
+  skinny_tile<T, 1, true, 1>(descs[blockIdx.y], 1);
+}
+
+template <class T>
+hipError_t launch_skinny_one(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, hipStream_t s) {
+  if (ndesc <= 0 || max_m <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((max_m + SK_ROWS - 1) / SK_ROWS), (unsigned)ndesc);
+  hipLaunchKernelGGL((skinny_one_kernel<T>), grid, dim3(256), 0, s, d_descs);
+  return hipGetLastError();
+}
+template hipError_t launch_skinny_one<double>(const SkinnyDesc<double>*, int, int, hipStream_t);
 
 // one product, descriptor passed by value (no descriptor upload)
 template <class T, int NR>

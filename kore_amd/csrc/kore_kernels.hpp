@@ -50,6 +50,9 @@ template <class T>
 hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmin, int nmax, hipStream_t s);
 template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s);
+// the same for one right-hand side and descriptors of one product each
+template <class T>
+hipError_t launch_skinny_one(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, hipStream_t s);
 // single product with the descriptor passed by value
 template <class T>
 hipError_t launch_skinny1(const SkinnyDesc<T>& d, int nrhs, hipStream_t s);

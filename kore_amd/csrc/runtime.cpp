@@ -907,7 +907,7 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
       if (l == 0 && sp0) {
         {
           Timed tm(c, KORE_TIMER_SKINNY);
-          HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // y_odd in Xv
+          HIP_TRY(launch_skinny_one<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, st));   // y_odd in Xv
         }
         Timed tm(c, KORE_TIMER_SKINNY);
         HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 0, F->Xv,
@@ -935,7 +935,7 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
                                  F->Bv, st));
         }
         Timed tm(c, KORE_TIMER_SKINNY);
-        HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));     // x_odd in Xv
+        HIP_TRY(launch_skinny_one<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, st));     // x_odd in Xv
         continue;
       }
     }
