@@ -112,7 +112,7 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
 # HBM bytes per solve-sweep dispatch measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE (scaled by a plain 1 GiB copy) on the default workload (FP64,
 # Krylov block 1); see profiles/hbm_traffic_eigensolve_m1_E1e-6_N200_fp64.md.
-MEASURED_SWEEP_TRAFFIC = {("hydro_E1e-6_N200", 0, 1): 40.03e6}
+MEASURED_SWEEP_TRAFFIC = {("hydro_E1e-6_N200", 0, 1): 40.02e6}
 
 
 def roofline(run, precision, workload=None, block=None):
