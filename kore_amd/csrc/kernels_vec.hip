@@ -323,9 +323,10 @@ __device__ __forceinline__ void skinny_tile(const SkinnyDesc<T>& d, int nrhs) {
     const int64_t lda = d.lda[p], ldx = d.ldx[p];
     const T al = d.alpha[p];
     if constexpr (PRE) {
-      // k entries staged at once (one-product launches keep to the short form,
-      // so that their register count stays low; longer products stream)
-      constexpr int KPP = NP == 1 ? SK_PRE_K : SkPreK<NR>::value;
+      // k entries staged at once (launches of fewer than three products keep
+      // to the short form, so that their register count stays low; longer
+      // products stream)
+      constexpr int KPP = NP < 3 ? SK_PRE_K : SkPreK<NR>::value;
       constexpr int KP = KPP < KC ? KPP : KC;
       // the product, with NA A entries per thread (NA * 16 >= K)
       auto product = [&](auto na) {
@@ -442,22 +443,26 @@ __global__ __launch_bounds__(256) void skinny_kernel(const SkinnyDesc<T>* __rest
   skinny_tile<T, NR, true>(descs[blockIdx.y], nrhs);   // descriptor read in place (uniform loads, no private copy)
 }
 
-// one right-hand side and one product per descriptor: a third of the
-// registers of the three-product form, so more workgroups per CU keep loads
-// in flight
-template <class T>
-__global__ __launch_bounds__(256) void skinny_one_kernel(const SkinnyDesc<T>* __restrict__ descs) {
-  skinny_tile<T, 1, true, 1>(descs[blockIdx.y], 1);
+// one right-hand side and at most NP products per descriptor, blocks up to
+// 256 wide: registers for NP products only, so more workgroups per CU keep
+// loads in flight (one product: a third of the three-product form; only
+// NP = 1 is launched)
+template <class T, int NP>
+__global__ __launch_bounds__(256) void skinny_np_kernel(const SkinnyDesc<T>* __restrict__ descs) {
+  skinny_tile<T, 1, true, NP>(descs[blockIdx.y], 1);
 }
 
 template <class T>
-hipError_t launch_skinny_one(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, hipStream_t s) {
+hipError_t launch_skinny_np(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int np, hipStream_t s) {
   if (ndesc <= 0 || max_m <= 0) return hipSuccess;
   const dim3 grid((unsigned)((max_m + SK_ROWS - 1) / SK_ROWS), (unsigned)ndesc);
-  hipLaunchKernelGGL((skinny_one_kernel<T>), grid, dim3(256), 0, s, d_descs);
+  if (np == 1)
+    hipLaunchKernelGGL((skinny_np_kernel<T, 1>), grid, dim3(256), 0, s, d_descs);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
-template hipError_t launch_skinny_one<double>(const SkinnyDesc<double>*, int, int, hipStream_t);
+template hipError_t launch_skinny_np<double>(const SkinnyDesc<double>*, int, int, int, hipStream_t);
 
 // one product, descriptor passed by value (no descriptor upload)
 template <class T, int NR>

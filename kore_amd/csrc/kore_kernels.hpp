@@ -50,9 +50,10 @@ template <class T>
 hipError_t launch_gj_inverse(const InvDesc<T>* d_descs, int ndesc, int nmin, int nmax, hipStream_t s);
 template <class T>
 hipError_t launch_skinny(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int nrhs, hipStream_t s);
-// the same for one right-hand side and descriptors of one product each
+// the same for one right-hand side, descriptors of at most np (= 1)
+// product each and blocks up to 256 wide (wider ones stream)
 template <class T>
-hipError_t launch_skinny_one(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, hipStream_t s);
+hipError_t launch_skinny_np(const SkinnyDesc<T>* d_descs, int ndesc, int max_m, int np, hipStream_t s);
 // single product with the descriptor passed by value
 template <class T>
 hipError_t launch_skinny1(const SkinnyDesc<T>& d, int nrhs, hipStream_t s);

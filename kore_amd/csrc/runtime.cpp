@@ -901,13 +901,20 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
   hipStream_t st = c->stream;
   const int ltop = F->tail ? F->nlev - 1 : F->nlev;   // levels swept one by one
   const bool sp0 = F->lev0_sparse && nrhs == 1;        // level 0 through the sparse couplings
+  // one column, blocks up to 256 wide: level 0's inverse blocks (one product
+  // each) take the kernel with registers for one product (a two-product
+  // kernel for the forward levels measured no faster)
+  const bool fewp = std::is_same<T, double>::value && nrhs == 1 && s->nmax <= 256;
   for (int l = 0; l < ltop; ++l) {
     Level<T>& L = F->lev[l];
     if constexpr (std::is_same<T, double>::value) {
       if (l == 0 && sp0) {
         {
           Timed tm(c, KORE_TIMER_SKINNY);
-          HIP_TRY(launch_skinny_one<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, st));   // y_odd in Xv
+          if (fewp)
+            HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // y_odd in Xv
+          else
+            HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
         }
         Timed tm(c, KORE_TIMER_SKINNY);
         HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 0, F->Xv,
@@ -935,7 +942,10 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
                                  F->Bv, st));
         }
         Timed tm(c, KORE_TIMER_SKINNY);
-        HIP_TRY(launch_skinny_one<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, st));     // x_odd in Xv
+        if (fewp)
+          HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // x_odd in Xv
+        else
+          HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
         continue;
       }
     }
