@@ -67,6 +67,7 @@ typedef struct {
   double flops;            /* algorithmic real flops of the factorisation actually run */
   double flops_inv;        /*   of which block inversions (8 n^3 each)     */
   double flops_gemm;       /*   of which Schur-complement GEMMs (8 m n k)  */
+  double flops_sparse;     /*   of which level 0's products with the sparse couplings (8 nnz n) */
   double solve_bytes;      /* bytes one single-column solve sweep reads: factor blocks, plus, with an
                               FP64 factor, level 0's sparse couplings read from the matrix */
   int32_t solve_launches;  /* kernel launches of one single-column solve sweep */
@@ -84,6 +85,18 @@ int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out
 void kore_ctx_destroy(kore_ctx* ctx);
 int32_t kore_ctx_sync(kore_ctx* ctx);
 int64_t kore_ctx_bytes_in_use(const kore_ctx* ctx);
+/* Device and pinned host blocks released by systems, factors and
+ * workspaces are cached by the library (per device, shared by every context
+ * on it) and reused by later allocations, so that a stream of requests does
+ * not call hipMalloc / hipFree (which wait for the whole device) once warm.
+ * kore_mem_trim returns every cached block of the device (and every cached
+ * pinned host block) to the runtime; kore_mem_stats reports the cached
+ * device bytes and the number of device allocations made so far (pool
+ * misses, all devices).  Destroy systems before their context.
+ * No counterpart in the reference: PETSc allocates per object and frees at
+ * MatDestroy / PetscFinalize (bin/solve.py:314-319). */
+int32_t kore_mem_trim(int32_t device);
+int32_t kore_mem_stats(int32_t device, int64_t* cached_bytes, int64_t* device_allocs);
 /* Per-kernel-class device timing (HIP events around each launch). */
 int32_t kore_ctx_set_timing(kore_ctx* ctx, int32_t enable);
 int32_t kore_ctx_timing(kore_ctx* ctx, int32_t timer_class, int64_t* count, double* total_ms);
@@ -116,7 +129,9 @@ int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
  * Replaces: the numeric LU of ST sinvert inside E.solve() (bin/solve.py:123)
  * and of PC lu inside K.solve() (bin/solve.py:220-227); MUMPS. */
 int32_t kore_factor(kore_sys* sys, double sre, double sim, int32_t precision, kore_factor_info* info);
-/* Drops the factor (device memory is returned). */
+/* Drops the factorisations (both precisions: kore_factor keeps the plan of
+ * the other precision when the precision changes); their device memory
+ * returns to the library's cache. */
 int32_t kore_factor_release(kore_sys* sys);
 
 /* Solves (A - sigma B) x = b for nrhs columns (host arrays, Kore order,

@@ -48,7 +48,8 @@ class SolveInfo(C.Structure):
 
 class FactorInfo(C.Structure):
     _fields_ = [("levels", C.c_int32), ("nblk", C.c_int32), ("flops", C.c_double),
-                ("flops_inv", C.c_double), ("flops_gemm", C.c_double), ("solve_bytes", C.c_double),
+                ("flops_inv", C.c_double), ("flops_gemm", C.c_double), ("flops_sparse", C.c_double),
+                ("solve_bytes", C.c_double),
                 ("solve_launches", C.c_int32), ("bytes", C.c_int64), ("ms", C.c_double)]
 
 
@@ -68,6 +69,8 @@ SIGNATURES = [
     ("kore_ctx_destroy", None, [_P]),
     ("kore_ctx_sync", _I32, [_P]),
     ("kore_ctx_bytes_in_use", _I64, [_P]),
+    ("kore_mem_trim", _I32, [_I32]),
+    ("kore_mem_stats", _I32, [_I32, _PI64, _PI64]),
     ("kore_ctx_set_timing", _I32, [_P, _I32]),
     ("kore_ctx_timing", _I32, [_P, _I32, _PI64, _PD]),
     ("kore_ctx_timing_reset", _I32, [_P]),
@@ -159,6 +162,19 @@ def layout_blocks(lay: Layout):
 
 def factor_bytes(lay: Layout, precision: int) -> int:
     return int(lib().kore_factor_bytes(C.byref(lay), int(precision)))
+
+
+def mem_stats(device: int = 0):
+    """(cached device bytes, device allocations so far) of the library's pool."""
+    cached = C.c_int64(0)
+    allocs = C.c_int64(0)
+    _check(lib().kore_mem_stats(int(device), C.byref(cached), C.byref(allocs)))
+    return cached.value, allocs.value
+
+
+def mem_trim(device: int = 0):
+    """Returns the library's cached device and pinned blocks to the runtime."""
+    _check(lib().kore_mem_trim(int(device)))
 
 
 _LIVE_CONTEXTS = weakref.WeakSet()

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkore_amd.so")
 
-SOURCES = ["kernels_dense.hip", "kernels_vec.hip", "runtime.cpp"]
+SOURCES = ["kernels_dense.hip", "kernels_vec.hip", "kernels_sparse.hip", "runtime.cpp"]
 
 ARCH = os.environ.get("KORE_OFFLOAD_ARCH", "gfx950")
 

@@ -1,0 +1,232 @@
+// Level 0 of the cyclic reduction through the sparse couplings (gfx950).
+//
+// At level 0 the couplings L_t, U_t of degree block t are the off-diagonal
+// blocks of the row-equilibrated matrix R (A - sigma B) itself: radial
+// operators of a few dozen entries per row, not dense blocks.  Multiplying
+// them as dense n x n blocks (the rest of the reduction's GEMMs) would cost
+// 8 n^3 real flops per product where 8 nnz(S) n do; so level 0 keeps them in
+// a compact off-block CSR (values rebuilt per shift) plus a per-block CSC
+// view, and forms the level's products with two memory-streaming kernels:
+//
+//   sprow  C = beta C + alpha sum_p S_p X_p   (sparse rows x dense; for the
+//          kept blocks t: D_t -= L_t H_{t-1} + U_t G_{t+1},
+//          L_t <- -L_t G_{t-1}, U_t <- -U_t H_{t+1})
+//   spcol  C = beta C + alpha X S             (dense x sparse columns; for the
+//          eliminated blocks t: G_t = Dinv_t L_t, H_t = Dinv_t U_t)
+//   offsweep  y -= S v on the rows of one block parity (the level-0 step of
+//          every solve sweep: b_even -= L y_odd forward, b_odd -= L x_even
+//          backward), for complex128 or complex64 vectors and up to 16
+//          right-hand sides
+//   offvals   oval[q] = r_i (a_k - sigma b_k) for the off-block entries
+#include "kore_device.hpp"
+#include "kore_kernels.hpp"
+
+namespace kore {
+
+template <class T> __device__ __forceinline__ cx<double> up(cx<T> a) { return cx<double>{(double)a.re, (double)a.im}; }
+
+// ---------------------------------------------------------------------------
+// values of the off-block entries for a shift: 16 lanes per row
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void offvals_kernel(CsrDev m, cx<double> cs, const double* __restrict__ scale,
+                                                      const int32_t* __restrict__ split, OffCsr o) {
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + g;
+  if (row >= m.n) return;
+  const int32_t sp = split[row];
+  const int nlo = sp & 0x7FFF, nhi = (sp >> 15) & 0x7FFF;
+  const int s = m.indptr[row], e = m.indptr[row + 1];
+  const int q0 = o.ptr[row];
+  const double sc = scale[row];
+  // the union row's entries are sorted by column: the previous block's are
+  // its first nlo, the next block's its last nhi, in the order of the
+  // compact row
+  for (int q = gl; q < nlo + nhi; q += 16) {
+    const int k = q < nlo ? s + q : e - nhi + (q - nlo);
+    const cx<double> v = csub(m.va[k], cmul(cs, m.vb[k]));
+    o.val[q0 + q] = cx<double>{sc * v.re, sc * v.im};
+  }
+}
+
+hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,
+                          const OffCsr& o, hipStream_t s) {
+  if (m.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(offvals_kernel, dim3((unsigned)((m.n + 15) / 16)), dim3(256), 0, s, m, cx<double>{sre, sim},
+                     scale, split, o);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// sparse rows x dense: one lane per output row, SR_NC output columns per lane
+// (the row's entries are read once for all of them)
+// ---------------------------------------------------------------------------
+static constexpr int SR_NC = 8;     // output columns per lane
+static constexpr int SR_ROWS = 256; // output rows per workgroup
+
+template <class T>
+__global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>* __restrict__ descs) {
+  const SpRowDesc<T>& d = descs[blockIdx.z];
+  const int r = blockIdx.x * SR_ROWS + threadIdx.x;
+  const int j0 = blockIdx.y * SR_NC;
+  if (r >= d.m || j0 >= d.n) return;
+  const int64_t row = d.row0 + r;
+  const int s = o.ptr[row], e = o.ptr[row + 1], nl = o.nlo[row];
+  cx<double> acc[SR_NC];
+#pragma unroll
+  for (int c = 0; c < SR_NC; ++c) acc[c] = czero<double>();
+  const int nc = d.n - j0 < SR_NC ? d.n - j0 : SR_NC;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if (d.side[p] < 0) continue;
+    const int q0 = d.side[p] == 0 ? s : s + nl, q1 = d.side[p] == 0 ? s + nl : e;
+    const cx<T>* X = d.X[p] + (int64_t)j0 * d.ldx[p] - d.xcol0[p];
+    const int64_t ldx = d.ldx[p];
+    for (int q = q0; q < q1; ++q) {
+      const cx<double> v = o.val[q];
+      const int64_t col = o.col[q];
+#pragma unroll
+      for (int c = 0; c < SR_NC; ++c)
+        if (c < nc) acc[c] = cfma(acc[c], v, up(gld(X + c * ldx + col)));
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < SR_NC; ++c) {
+    if (c < nc) {
+      cx<T>* y = d.C + (int64_t)(j0 + c) * d.ldc + r;
+      cx<double> out{(double)d.alpha * acc[c].re, (double)d.alpha * acc[c].im};
+      if (d.beta != T(0)) {
+        const cx<double> old = up(gld(y));
+        out.re += (double)d.beta * old.re;
+        out.im += (double)d.beta * old.im;
+      }
+      gst(y, cx<T>{(T)out.re, (T)out.im});
+    }
+  }
+}
+
+template <class T>
+hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n,
+                        hipStream_t s) {
+  if (ndesc <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((max_m + SR_ROWS - 1) / SR_ROWS), (unsigned)((max_n + SR_NC - 1) / SR_NC),
+                  (unsigned)ndesc);
+  hipLaunchKernelGGL(sprow_kernel<T>, grid, dim3(SR_ROWS), 0, s, o, d_descs);
+  return hipGetLastError();
+}
+template hipError_t launch_sprow<double>(const OffCsr&, const SpRowDesc<double>*, int, int, int, hipStream_t);
+template hipError_t launch_sprow<float>(const OffCsr&, const SpRowDesc<float>*, int, int, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// dense x sparse columns: one wave per output column, lanes over the rows;
+// the column's entry list is uniform over the wave (scalar loads) and each
+// entry is one coalesced column segment of X
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void spcol_kernel(OffCsc cc, const cx<double>* __restrict__ oval,
+                                                    const SpColDesc<T>* __restrict__ descs) {
+  const SpColDesc<T>& d = descs[blockIdx.y];
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= d.n) return;
+  const int k0 = cc.ptr[d.cbase + c], k1 = cc.ptr[d.cbase + c + 1];
+  for (int r0 = 0; r0 < d.m; r0 += 64) {
+    const int r = r0 + lane;
+    cx<double> acc = czero<double>();
+    if (r < d.m) {
+      for (int k = k0; k < k1; ++k) {
+        const cx<double> v = oval[cc.q[k]];
+        acc = cfma(acc, up(gld(d.X + (int64_t)cc.row[k] * d.ldx + r)), v);
+      }
+      cx<T>* y = d.C + (int64_t)c * d.ldc + r;
+      cx<double> out{(double)d.alpha * acc.re, (double)d.alpha * acc.im};
+      if (d.beta != T(0)) {
+        const cx<double> old = up(gld(y));
+        out.re += (double)d.beta * old.re;
+        out.im += (double)d.beta * old.im;
+      }
+      gst(y, cx<T>{(T)out.re, (T)out.im});
+    }
+  }
+}
+
+template <class T>
+hipError_t launch_spcol(const OffCsc& cc, const cx<double>* oval, const SpColDesc<T>* d_descs, int ndesc, int max_n,
+                        hipStream_t s) {
+  if (ndesc <= 0 || max_n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((max_n + 3) / 4), (unsigned)ndesc);
+  hipLaunchKernelGGL(spcol_kernel<T>, grid, dim3(256), 0, s, cc, oval, d_descs);
+  return hipGetLastError();
+}
+template hipError_t launch_spcol<double>(const OffCsc&, const cx<double>*, const SpColDesc<double>*, int, int,
+                                         hipStream_t);
+template hipError_t launch_spcol<float>(const OffCsc&, const cx<double>*, const SpColDesc<float>*, int, int,
+                                        hipStream_t);
+
+// ---------------------------------------------------------------------------
+// level-0 step of a solve sweep: y[i] -= sum_q oval[q] v[col q] for the rows
+// i of blocks of one parity, nrhs <= NR columns; 16 lanes per row
+// ---------------------------------------------------------------------------
+template <class T, int NR>
+__global__ __launch_bounds__(256) void offsweep_kernel(OffCsr o, const int32_t* __restrict__ blk_of_row, int64_t n,
+                                                       int parity, const cx<T>* __restrict__ v, int64_t ldv,
+                                                       cx<T>* __restrict__ y, int64_t ldy, int nrhs) {
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + g;
+  if (row >= n) return;
+  if ((blk_of_row[row] & 1) != parity) return;      // uniform over the row's 16 lanes
+  const int s = o.ptr[row], e = o.ptr[row + 1];
+  cx<double> acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = czero<double>();
+  for (int q = s + gl; q < e; q += 16) {
+    const cx<double> a = o.val[q];
+    const int64_t col = o.col[q];
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) acc[c] = cfma(acc[c], a, up(v[c * ldv + col]));
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) {
+      acc[c].re += __shfl_xor(acc[c].re, off, 64);
+      acc[c].im += __shfl_xor(acc[c].im, off, 64);
+    }
+  }
+  if (gl == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (c < nrhs) {
+        cx<T>* p = y + c * ldy + row;
+        const cx<T> old = *p;
+        *p = cx<T>{(T)((double)old.re - acc[c].re), (T)((double)old.im - acc[c].im)};
+      }
+    }
+  }
+}
+
+template <class T>
+hipError_t launch_offsweep(const OffCsr& o, const int32_t* blk_of_row, int64_t n, int parity, const cx<T>* v,
+                           int64_t ldv, cx<T>* y, int64_t ldy, int nrhs, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (nrhs < 1 || nrhs > 16) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((n + 15) / 16));
+  if (nrhs == 1)
+    hipLaunchKernelGGL((offsweep_kernel<T, 1>), grid, dim3(256), 0, s, o, blk_of_row, n, parity, v, ldv, y, ldy, nrhs);
+  else if (nrhs <= 2)
+    hipLaunchKernelGGL((offsweep_kernel<T, 2>), grid, dim3(256), 0, s, o, blk_of_row, n, parity, v, ldv, y, ldy, nrhs);
+  else if (nrhs <= 4)
+    hipLaunchKernelGGL((offsweep_kernel<T, 4>), grid, dim3(256), 0, s, o, blk_of_row, n, parity, v, ldv, y, ldy, nrhs);
+  else if (nrhs <= 8)
+    hipLaunchKernelGGL((offsweep_kernel<T, 8>), grid, dim3(256), 0, s, o, blk_of_row, n, parity, v, ldv, y, ldy, nrhs);
+  else
+    hipLaunchKernelGGL((offsweep_kernel<T, 16>), grid, dim3(256), 0, s, o, blk_of_row, n, parity, v, ldv, y, ldy,
+                       nrhs);
+  return hipGetLastError();
+}
+template hipError_t launch_offsweep<double>(const OffCsr&, const int32_t*, int64_t, int, const cx<double>*, int64_t,
+                                            cx<double>*, int64_t, int, hipStream_t);
+template hipError_t launch_offsweep<float>(const OffCsr&, const int32_t*, int64_t, int, const cx<float>*, int64_t,
+                                           cx<float>*, int64_t, int, hipStream_t);
+
+}  // namespace kore

@@ -69,6 +69,63 @@ struct CsrDev {
   const cx<double>* vb;
 };
 
+// ---- level-0 sparse couplings (kernels_sparse.hip) ------------------------
+// Compact CSR of the entries outside each row's diagonal block (degree
+// order): row i holds ptr[i+1] - ptr[i] entries, the first nlo[i] in the
+// previous block, the rest in the next one, sorted by (global) column; val
+// holds r_i (a - sigma b) of the current shift.
+struct OffCsr {
+  const int32_t* ptr;
+  const int32_t* nlo;
+  const int32_t* col;
+  cx<double>* val;
+};
+// Column view of the same entries per (block t, side): for column c of the
+// neighbouring block, entries ptr[cbase + c] .. ptr[cbase + c + 1] give the
+// compact-CSR entry q and the local row (in block t).
+struct OffCsc {
+  const int32_t* ptr;
+  const int32_t* q;
+  const int32_t* row;
+};
+// C (m x n, ldc) = beta C + alpha sum_{p<2} S_p X_p, S_p the side[p]
+// entries (0: previous block, 1: next block; -1: unused) of the rows
+// row0 .. row0+m-1; X_p column-major with its first row at global column xcol0[p].
+template <class T> struct SpRowDesc {
+  const cx<T>* X[2];
+  int64_t ldx[2];
+  int64_t xcol0[2];
+  int side[2];
+  cx<T>* C;
+  int64_t ldc;
+  int64_t row0;
+  int m, n;
+  T alpha, beta;
+};
+// C (m x n, ldc) = beta C + alpha X S, S the entries of one side of a block
+// (columns: the neighbouring block's n unknowns; column segment cbase).
+template <class T> struct SpColDesc {
+  const cx<T>* X;
+  int64_t ldx;
+  cx<T>* C;
+  int64_t ldc;
+  int64_t cbase;
+  int m, n;
+  T alpha, beta;
+};
+hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,
+                          const OffCsr& o, hipStream_t s);
+template <class T>
+hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n,
+                        hipStream_t s);
+template <class T>
+hipError_t launch_spcol(const OffCsc& cc, const cx<double>* oval, const SpColDesc<T>* d_descs, int ndesc, int max_n,
+                        hipStream_t s);
+// y[:, c] -= S v[:, c] on the rows of blocks of the given parity (S: all off-block entries)
+template <class T>
+hipError_t launch_offsweep(const OffCsr& o, const int32_t* blk_of_row, int64_t n, int parity, const cx<T>* v,
+                           int64_t ldv, cx<T>* y, int64_t ldy, int nrhs, hipStream_t s);
+
 // Y[:, c] = beta*Y[:, c] + (ca*A + cb*B) X[:, c],  c < nrhs  (complex128)
 // rows of blocks of the given parity: y -= scale * (off-diagonal blocks of A - (sre + i sim) B) v
 hipError_t launch_offdiag(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -83,6 +84,169 @@ struct TimerRec {
   hipEvent_t a, b;
 };
 
+// ---------------------------------------------------------------------------
+// memory pools
+// ---------------------------------------------------------------------------
+// Device and pinned-host allocations are cached for the life of the process
+// and shared by every context on a device: a request stream allocates the
+// factor, system and workspace buffers of each request (gigabytes at the
+// serving sizes), and hipMalloc / hipFree of such blocks cost milliseconds to
+// a second each, with hipFree waiting for the whole device, so that requests
+// on other streams stall.  A released block goes back to its pool with an
+// event recorded on the releasing stream; the next taker (any context) makes
+// its own stream wait for that event before it touches the block (a host
+// taker waits on the host), so reuse never races with work still queued.
+// Best fit: a request takes the smallest cached block that is at least as
+// large and at most twice as large (plus a little).  When the device runs
+// out of memory the cached blocks are returned to the runtime and the
+// allocation is tried once more.
+struct PoolBlock {
+  void* p;
+  size_t bytes;
+  hipEvent_t ev;   // recorded on the releasing stream (may be null)
+};
+
+struct MemPool {
+  std::mutex mu;
+  std::multimap<size_t, PoolBlock> blocks;
+  size_t cached = 0;
+};
+
+static constexpr int POOL_MAXDEV = 64;
+static MemPool g_dev_pool[POOL_MAXDEV];
+static MemPool g_host_pool;
+static std::atomic<int64_t> g_pool_misses{0};
+
+static size_t pool_round(size_t n) {
+  const size_t g = n >= (size_t(64) << 20) ? (size_t(2) << 20) : 4096;   // 2 MiB granules for large blocks
+  return (n + g - 1) / g * g;
+}
+
+static bool pool_pick(MemPool& mp, size_t need, PoolBlock* out) {
+  std::lock_guard<std::mutex> lk(mp.mu);
+  auto it = mp.blocks.lower_bound(need);
+  if (it == mp.blocks.end() || it->first > 2 * need + (size_t(64) << 20)) return false;
+  *out = it->second;
+  mp.cached -= it->second.bytes;
+  mp.blocks.erase(it);
+  return true;
+}
+
+static void pool_put(MemPool& mp, void* p, size_t bytes, hipStream_t st) {
+  hipEvent_t ev = nullptr;
+  if (st && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+    if (hipEventRecord(ev, st) != hipSuccess) {
+      hipEventDestroy(ev);
+      ev = nullptr;
+      hipStreamSynchronize(st);
+    }
+  }
+  std::lock_guard<std::mutex> lk(mp.mu);
+  mp.blocks.emplace(bytes, PoolBlock{p, bytes, ev});
+  mp.cached += bytes;
+}
+
+// Returns every cached block of the pool to the runtime.
+static void pool_trim(MemPool& mp, bool host) {
+  std::multimap<size_t, PoolBlock> all;
+  {
+    std::lock_guard<std::mutex> lk(mp.mu);
+    all.swap(mp.blocks);
+    mp.cached = 0;
+  }
+  for (auto& kv : all) {
+    if (kv.second.ev) {
+      hipEventSynchronize(kv.second.ev);
+      hipEventDestroy(kv.second.ev);
+    }
+    if (host)
+      hipHostFree(kv.second.p);
+    else
+      hipFree(kv.second.p);
+  }
+}
+
+// Device block of at least `need` bytes for work on `st` (device current).
+static int32_t dev_take(int device, hipStream_t st, size_t need, void** p, size_t* got) {
+  MemPool& mp = g_dev_pool[device % POOL_MAXDEV];
+  PoolBlock b{};
+  if (pool_pick(mp, need, &b)) {
+    if (b.ev) {
+      hipStreamWaitEvent(st, b.ev, 0);
+      hipEventDestroy(b.ev);
+    }
+    *p = b.p;
+    *got = b.bytes;
+    return KORE_OK;
+  }
+  ++g_pool_misses;
+  const size_t sz = pool_round(need);
+  hipError_t e = hipMalloc(p, sz);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pool_trim(mp, false);
+    e = hipMalloc(p, sz);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(KORE_ERR_OOM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+  }
+  *got = sz;
+  return KORE_OK;
+}
+
+// Pinned host block of at least `need` bytes; waits for the work that last used it.
+static int32_t host_take(size_t need, char** p, size_t* got) {
+  PoolBlock b{};
+  if (pool_pick(g_host_pool, need, &b)) {
+    if (b.ev) {
+      hipEventSynchronize(b.ev);
+      hipEventDestroy(b.ev);
+    }
+    *p = static_cast<char*>(b.p);
+    *got = b.bytes;
+    return KORE_OK;
+  }
+  const size_t sz = pool_round(need);
+  hipError_t e = hipHostMalloc((void**)p, sz, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pool_trim(g_host_pool, true);
+    e = hipHostMalloc((void**)p, sz, hipHostMallocDefault);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(KORE_ERR_OOM, "hipHostMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+  }
+  *got = sz;
+  return KORE_OK;
+}
+
+// A pinned host buffer from the pool, returned to it (behind an event on
+// `st`) when released.
+struct PinnedBuf {
+  char* p = nullptr;
+  size_t bytes = 0;
+  hipStream_t st = nullptr;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  int32_t reserve(size_t need, hipStream_t stream) {
+    if (need <= bytes) return KORE_OK;
+    release();
+    st = stream;
+    return host_take(need, &p, &bytes);
+  }
+  void release() {
+    if (p) pool_put(g_host_pool, p, bytes, st);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct kore_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -95,21 +259,12 @@ struct kore_ctx {
   int64_t cnt[KORE_TIMER_COUNT] = {0};
   // pinned host arena for building and uploading system matrices: kept for
   // the life of the context (pre-faulted, DMA-able), grown on demand
-  char* pin = nullptr;
-  size_t pin_cap = 0;
+  PinnedBuf pin;
 };
 
 static int32_t ctx_pinned(kore_ctx* c, size_t bytes, char** out) {
-  if (bytes > c->pin_cap) {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->pin) HIP_TRY(hipHostFree(c->pin));
-    c->pin = nullptr;
-    c->pin_cap = 0;
-    const size_t cap = bytes + bytes / 4;
-    HIP_TRY(hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
-    c->pin_cap = cap;
-  }
-  *out = c->pin;
+  if (bytes > c->pin.bytes) TRY(c->pin.reserve(bytes + bytes / 4, c->stream));
+  *out = c->pin.p;
   return KORE_OK;
 }
 
@@ -160,10 +315,13 @@ static void resolve_timers(kore_ctx* c) {
   c->pending.clear();
 }
 
+// Device buffer from the pool of the context's device; the context's budget
+// counts the bytes asked for.
 struct DevBuf {
   kore_ctx* c = nullptr;
   void* p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;     // bytes asked for
+  size_t block = 0;     // bytes of the pool block
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -175,23 +333,19 @@ struct DevBuf {
     if (c->budget > 0 && c->in_use.load() + (int64_t)nbytes > c->budget)
       return fail(KORE_ERR_OOM, "device memory budget exceeded: need %zu bytes, %lld of %lld in use", nbytes,
                   (long long)c->in_use.load(), (long long)c->budget);
-    hipError_t e = hipMalloc(&p, nbytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      p = nullptr;
-      return fail(KORE_ERR_OOM, "hipMalloc(%zu) failed: %s", nbytes, hipGetErrorString(e));
-    }
+    TRY(dev_take(c->device, c->stream, nbytes, &p, &block));
     bytes = nbytes;
     c->in_use += (int64_t)nbytes;
     return KORE_OK;
   }
   void release() {
     if (p) {
-      hipFree(p);
+      pool_put(g_dev_pool[c->device % POOL_MAXDEV], p, block, c->stream);
       c->in_use -= (int64_t)bytes;
     }
     p = nullptr;
     bytes = 0;
+    block = 0;
   }
   template <class U> U* as() const { return reinterpret_cast<U*>(p); }
 };
@@ -200,16 +354,12 @@ struct DevBuf {
 // matrices) so that they stay asynchronous on the library stream.
 struct Uploader {
   kore_ctx* c = nullptr;
-  char* host = nullptr;
+  PinnedBuf buf;
   size_t cap = 0, off = 0;
-  ~Uploader() {
-    if (host) hipHostFree(host);
-  }
   int32_t init(kore_ctx* ctx, size_t bytes) {
     c = ctx;
     cap = bytes;
-    HIP_TRY(hipHostMalloc((void**)&host, cap, hipHostMallocDefault));
-    return KORE_OK;
+    return buf.reserve(cap, c->stream);
   }
   int32_t put(void* dst, const void* src, size_t nbytes) {
     if (nbytes > cap) {
@@ -221,8 +371,8 @@ struct Uploader {
       HIP_TRY(hipStreamSynchronize(c->stream));
       off = 0;
     }
-    memcpy(host + off, src, nbytes);
-    HIP_TRY(hipMemcpyAsync(dst, host + off, nbytes, hipMemcpyHostToDevice, c->stream));
+    memcpy(buf.p + off, src, nbytes);
+    HIP_TRY(hipMemcpyAsync(dst, buf.p + off, nbytes, hipMemcpyHostToDevice, c->stream));
     off += (nbytes + 255) & ~size_t(255);
     return KORE_OK;
   }
@@ -231,22 +381,9 @@ struct Uploader {
 // Pinned staging for downloads: device -> pinned at DMA rate, then a copy into
 // the caller's (pageable) array split over a few host threads.
 struct Downloader {
-  char* host = nullptr;
-  size_t cap = 0;
-  ~Downloader() {
-    if (host) hipHostFree(host);
-  }
-  int32_t reserve(size_t bytes) {
-    if (bytes <= cap) return KORE_OK;
-    if (host) {
-      HIP_TRY(hipHostFree(host));
-      host = nullptr;
-      cap = 0;
-    }
-    HIP_TRY(hipHostMalloc((void**)&host, bytes, hipHostMallocDefault));
-    cap = bytes;
-    return KORE_OK;
-  }
+  PinnedBuf buf;
+  int32_t reserve(size_t bytes, hipStream_t st) { return buf.reserve(bytes, st); }
+  char* host() const { return buf.p; }
 };
 
 struct CopySeg {
@@ -367,7 +504,7 @@ static int32_t layout_build(const kore_layout* L, int64_t* sizmat, std::vector<i
 struct FactorBase {
   virtual ~FactorBase() = default;
   int prec = 0;
-  double flops = 0, flops_inv = 0, flops_gemm = 0, solve_bytes = 0;
+  double flops = 0, flops_inv = 0, flops_gemm = 0, flops_sparse = 0, solve_bytes = 0;
   int solve_launches = 0;
   int64_t bytes = 0;
   int nlev = 0;
@@ -384,6 +521,13 @@ template <class T> struct Level {
   int nfw = 0, mfw = 0;
   SkinnyDesc<T>* bw = nullptr;
   int nbw = 0, mbw = 0;
+  // level 0 through the sparse couplings
+  SpColDesc<T>* sc = nullptr;
+  size_t sc_off = 0;
+  int nsc = 0, maxn_sc = 0;
+  SpRowDesc<T>* sr = nullptr;
+  size_t sr_off = 0;
+  int nsr = 0, maxm_sr = 0, maxn_sr = 0;
 };
 
 template <class T> struct Factor : FactorBase {
@@ -443,6 +587,14 @@ struct kore_sys {
   CsrDev csr{};
   DevBuf d_ipb, d_ixb, d_vbb;      // B alone (degree order): the B x of every OP application
   CsrDev csr_b{};
+  // entries outside each row's diagonal block (the level-0 couplings) as a
+  // compact CSR (values per shift) and per (block, side) column segments
+  int64_t nnz_off = 0;
+  DevBuf d_optr, d_onlo, d_ocol, d_oval, d_cptr, d_cq, d_crow;
+  std::vector<int64_t> h_cbase;    // 2 per block: first column-pointer slot of (block, side), -1 if no neighbour
+  std::vector<int64_t> h_side_nnz; // 2 per block: entries of (block, side)
+  OffCsr off{};
+  OffCsc offc{};
   // scratch vectors (complex128, sizmat x 16 each): bb (rhs), xr (solution),
   // rr (residual), st (staging / scaled residual)
   cx<double>* bb = nullptr;
@@ -452,6 +604,7 @@ struct kore_sys {
   double* part = nullptr;
   int npart = 64;
   std::unique_ptr<FactorBase> f;
+  std::unique_ptr<FactorBase> fspare;   // the plan of the other precision, kept for the next switch
   bool factored = false;     // f holds a valid factorisation of A - sigma B
   double sre = 0, sim = 0;
   std::vector<double> h_scale;
@@ -468,10 +621,10 @@ struct kore_ws {
   DevBuf hb;                 // device-side Gram-Schmidt / Cholesky-QR coefficients of the expansion
   cx<double>* hh = nullptr;  // pinned host copies of them, two slots (blocks in flight)
   hipEvent_t hev[2] = {nullptr, nullptr};
+  PinnedBuf hhbuf;           // pinned pool block behind hh
   Uploader up;
   std::atomic<int> stop_at{INT32_MAX};  // set from another thread: the running expansion starts no block at or past it
   ~kore_ws() {
-    if (hh) hipHostFree(hh);
     for (hipEvent_t e : hev)
       if (e) hipEventDestroy(e);
   }
@@ -481,69 +634,101 @@ static constexpr int MAXR = 16;
 static constexpr int DOT_MAXCHUNK = 1024;   // split-K chunks of the workspace inner products
 static constexpr int DOT_ROWS = 64;        // minimum rows per chunk
 
-template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
-  kore_ctx* c = s->ctx;
-  const int nb = s->nblk, nmax = s->nmax;
-  F->nblk = nb;
-  // level id lists
-  std::vector<std::vector<int>> ids;
+// Element offsets of the factor's dense blocks (shared by the build and the
+// byte prediction).  With the sparse level 0 (at least two levels before the
+// two-block tail) the level's couplings are read from the matrix, so the
+// eliminated blocks of level 0 hold no L, U and the level has no E, F.
+struct FactorPlan {
+  std::vector<std::vector<int>> ids;   // block ids per level, then the root
+  int nlev = 0;
+  bool sp0 = false;
+  std::vector<int64_t> offD, offL, offU, offG, offH;
+  std::vector<std::vector<int64_t>> offE, offF;
+  int64_t cntD = 0, cntDLU = 0, cnt = 0;
+};
+
+static void factor_plan(const std::vector<int>& bsz, FactorPlan& P, bool allow_sparse0) {
+  const int nb = (int)bsz.size();
+  int nmax = 0;
+  for (int b : bsz) nmax = std::max(nmax, b);
+  P.ids.clear();
   {
     std::vector<int> cur(nb);
     for (int k = 0; k < nb; ++k) cur[k] = k;
     while (cur.size() > 1) {
-      ids.push_back(cur);
+      P.ids.push_back(cur);
       std::vector<int> nx;
       for (size_t t = 0; t < cur.size(); t += 2) nx.push_back(cur[t]);
       cur.swap(nx);
     }
-    ids.push_back(cur);  // root
+    P.ids.push_back(cur);  // root
   }
-  const int nlev = (int)ids.size() - 1;
-  F->nlev = nlev;
-  // memory plan: D_k (n_k^2), L_k, U_k (n_k x nmax), E/F per level
-  auto nk = [&](int k) { return (int64_t)s->bsz[k]; };
+  P.nlev = (int)P.ids.size() - 1;
+  P.sp0 = allow_sparse0 && P.nlev >= 2;
+  auto nk = [&](int k) { return (int64_t)bsz[k]; };
   int64_t cnt = 0;
-  std::vector<int64_t> offD(nb), offL(nb, -1), offU(nb, -1);
+  P.offD.assign(nb, -1);
+  P.offL.assign(nb, -1);
+  P.offU.assign(nb, -1);
+  P.offG.assign(nb, -1);
+  P.offH.assign(nb, -1);
   for (int k = 0; k < nb; ++k) {
-    offD[k] = cnt;
+    P.offD[k] = cnt;
     cnt += nk(k) * nk(k);
   }
+  P.cntD = cnt;
   for (int k = 0; k < nb; ++k) {
+    if (P.sp0 && (k & 1)) continue;        // eliminated at level 0: couplings read from the matrix
     if (k > 0) {
-      offL[k] = cnt;
+      P.offL[k] = cnt;
       cnt += nk(k) * nmax;
     }
     if (k < nb - 1) {
-      offU[k] = cnt;
+      P.offU[k] = cnt;
       cnt += nk(k) * nmax;
     }
   }
-  const int64_t cntDLU = cnt;
-  std::vector<int64_t> offG(nb, -1), offH(nb, -1);
+  P.cntDLU = cnt;
   for (int k = 0; k < nb; ++k) {
-    offG[k] = cnt;
+    P.offG[k] = cnt;
     cnt += nk(k) * nmax;
-    offH[k] = cnt;
+    P.offH[k] = cnt;
     cnt += nk(k) * nmax;
   }
-  std::vector<std::vector<int64_t>> offE(nlev), offF(nlev);
-  for (int l = 0; l < nlev; ++l) {
-    const auto& id = ids[l];
+  P.offE.assign(P.nlev, {});
+  P.offF.assign(P.nlev, {});
+  for (int l = 0; l < P.nlev; ++l) {
+    const auto& id = P.ids[l];
     const int T_ = (int)id.size();
-    offE[l].assign(T_, -1);
-    offF[l].assign(T_, -1);
+    P.offE[l].assign(T_, -1);
+    P.offF[l].assign(T_, -1);
+    if (l == 0 && P.sp0) continue;
     for (int t = 0; t < T_; t += 2) {
       if (t - 1 >= 0) {
-        offE[l][t] = cnt;
+        P.offE[l][t] = cnt;
         cnt += nk(id[t]) * nk(id[t - 1]);
       }
       if (t + 1 <= T_ - 1) {
-        offF[l][t] = cnt;
+        P.offF[l][t] = cnt;
         cnt += nk(id[t]) * nk(id[t + 1]);
       }
     }
   }
-  TRY(F->mem.alloc(c, (size_t)cnt * sizeof(cx<T>)));
+  P.cnt = cnt;
+}
+
+template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
+  kore_ctx* c = s->ctx;
+  const int nb = s->nblk, nmax = s->nmax;
+  F->nblk = nb;
+  FactorPlan P;
+  factor_plan(s->bsz, P, s->nnz_off > 0);
+  const auto& ids = P.ids;
+  const int nlev = P.nlev;
+  F->nlev = nlev;
+  const bool sp0 = P.sp0;
+  auto nk = [&](int k) { return (int64_t)s->bsz[k]; };
+  TRY(F->mem.alloc(c, (size_t)P.cnt * sizeof(cx<T>)));
   cx<T>* base = F->mem.template as<cx<T>>();
   F->D.resize(nb);
   F->L.assign(nb, nullptr);
@@ -551,11 +736,11 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   F->G.assign(nb, nullptr);
   F->H.assign(nb, nullptr);
   for (int k = 0; k < nb; ++k) {
-    F->D[k] = base + offD[k];
-    if (offL[k] >= 0) F->L[k] = base + offL[k];
-    if (offU[k] >= 0) F->U[k] = base + offU[k];
-    F->G[k] = base + offG[k];
-    F->H[k] = base + offH[k];
+    F->D[k] = base + P.offD[k];
+    if (P.offL[k] >= 0) F->L[k] = base + P.offL[k];
+    if (P.offU[k] >= 0) F->U[k] = base + P.offU[k];
+    F->G[k] = base + P.offG[k];
+    F->H[k] = base + P.offH[k];
   }
   // aux: ipiv (nb * nmax ints), info (nb ints), pointer arrays, solve vectors
   const size_t auxbytes = (size_t)nb * nmax * sizeof(int) + (size_t)nb * sizeof(int) + 3 * (size_t)nb * sizeof(void*) + 256;
@@ -577,6 +762,8 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   std::vector<InvDesc<T>> vinv;
   std::vector<GemmDesc<T>> vg;
   std::vector<SkinnyDesc<T>> vs;
+  std::vector<SpColDesc<T>> vsc;
+  std::vector<SpRowDesc<T>> vsr;
   struct Rng {
     size_t off, n;
   };
@@ -617,7 +804,37 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     d.Y = Y; d.ldy = ldv; d.m = m; d.beta = beta;
     return d;
   };
-  double lev0_bytes = 0;          // sweep bytes of level 0 (forward and backward)
+  // sparse level-0 products: C = X S (one side of block k) and C = beta C + alpha sum S_p X_p
+  auto mkspcol = [&](cx<T>* C, const cx<T>* X, int k, int side, T alpha, T beta) {
+    const int nbk = side == 0 ? k - 1 : k + 1;
+    SpColDesc<T> d{};
+    d.X = X; d.ldx = nk(k); d.C = C; d.ldc = nk(k); d.cbase = s->h_cbase[2 * k + side];
+    d.m = (int)nk(k); d.n = (int)nk(nbk); d.alpha = alpha; d.beta = beta;
+    const double fl = 8.0 * (double)s->h_side_nnz[2 * k + side] * nk(k);
+    flops += fl;
+    F->flops_sparse += fl;
+    return d;
+  };
+  struct SProd {
+    const cx<T>* X;       // rows: the neighbouring block's unknowns (leading dimension = its size)
+    int side;
+  };
+  auto mksprow = [&](cx<T>* C, int k, int n, std::initializer_list<SProd> prods, T alpha, T beta) {
+    SpRowDesc<T> d{};
+    d.side[0] = d.side[1] = -1;
+    int p = 0;
+    for (const SProd& q : prods) {
+      if (!q.X) continue;
+      const int nbk = q.side == 0 ? k - 1 : k + 1;
+      d.X[p] = q.X; d.ldx[p] = nk(nbk); d.xcol0[p] = s->boff[nbk]; d.side[p] = q.side;
+      const double fl = 8.0 * (double)s->h_side_nnz[2 * k + q.side] * n;
+      flops += fl;
+      F->flops_sparse += fl;
+      ++p;
+    }
+    d.C = C; d.ldc = nk(k); d.row0 = s->boff[k]; d.m = (int)nk(k); d.n = n; d.alpha = alpha; d.beta = beta;
+    return d;
+  };
   for (int l = 0; l < nlev; ++l) {
     const auto& id = ids[l];
     const int T_ = (int)id.size();
@@ -633,6 +850,47 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       L.minn_inv = std::min(L.minn_inv, (int)nk(k));
     }
     rinv[l].n = vinv.size() - rinv[l].off;
+    if (l == 0 && sp0) {
+      // level 0 through the couplings of the matrix (id[t] = t):
+      //   eliminated t:  G_t = Dinv_t L_t,  H_t = Dinv_t U_t              (dense x sparse)
+      //   kept t:        D_t -= L_t H_{t-1} + U_t G_{t+1},
+      //                  L_t = -L_t G_{t-1},  U_t = -U_t H_{t+1}          (sparse x dense)
+      L.sc_off = vsc.size();
+      for (int t = 1; t < T_; t += 2) {
+        vsc.push_back(mkspcol(F->G[t], F->D[t], t, 0, T(1), T(0)));
+        L.maxn_sc = std::max(L.maxn_sc, (int)nk(t - 1));
+        if (t + 1 <= T_ - 1) {
+          vsc.push_back(mkspcol(F->H[t], F->D[t], t, 1, T(1), T(0)));
+          L.maxn_sc = std::max(L.maxn_sc, (int)nk(t + 1));
+        }
+      }
+      L.nsc = (int)(vsc.size() - L.sc_off);
+      L.sr_off = vsr.size();
+      for (int t = 0; t < T_; t += 2) {
+        const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
+        if (hasE || hasF) {
+          vsr.push_back(mksprow(F->D[t], t, (int)nk(t),
+                                {SProd{hasE ? F->H[t - 1] : nullptr, 0}, SProd{hasF ? F->G[t + 1] : nullptr, 1}},
+                                T(-1), T(1)));
+          L.maxn_sr = std::max(L.maxn_sr, (int)nk(t));
+        }
+        if (t - 2 >= 0) {
+          vsr.push_back(mksprow(F->L[t], t, (int)nk(t - 2), {SProd{F->G[t - 1], 0}}, T(-1), T(0)));
+          L.maxn_sr = std::max(L.maxn_sr, (int)nk(t - 2));
+        }
+        if (t + 2 <= T_ - 1) {
+          vsr.push_back(mksprow(F->U[t], t, (int)nk(t + 2), {SProd{F->H[t + 1], 1}}, T(-1), T(0)));
+          L.maxn_sr = std::max(L.maxn_sr, (int)nk(t + 2));
+        }
+        L.maxm_sr = std::max(L.maxm_sr, (int)nk(t));
+      }
+      L.nsr = (int)(vsr.size() - L.sr_off);
+      rg1[l] = Rng{vg.size(), 0};
+      rg2[l] = Rng{vg.size(), 0};
+      rfw[l] = Rng{vs.size(), 0};
+      rbw[l] = Rng{vs.size(), 0};
+      continue;
+    }
     // g1: E_t = L_t Dinv_{t-1}, F_t = U_t Dinv_{t+1}
     rg1[l].off = vg.size();
     for (int t = 0; t < T_; t += 2) {
@@ -640,13 +898,13 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       const int m = (int)nk(k);
       if (t - 1 >= 0) {
         const int kp = id[t - 1];
-        vg.push_back(mkgemm(base + offE[l][t], m, (int)nk(kp), F->L[k], F->D[kp], (int)nk(kp), m, (int)nk(kp),
+        vg.push_back(mkgemm(base + P.offE[l][t], m, (int)nk(kp), F->L[k], F->D[kp], (int)nk(kp), m, (int)nk(kp),
                             nullptr, nullptr, 0, 0, 0, T(1), T(0)));
         L.t1 = std::max(L.t1, tiles(m, (int)nk(kp)));
       }
       if (t + 1 <= T_ - 1) {
         const int kn = id[t + 1];
-        vg.push_back(mkgemm(base + offF[l][t], m, (int)nk(kn), F->U[k], F->D[kn], (int)nk(kn), m, (int)nk(kn),
+        vg.push_back(mkgemm(base + P.offF[l][t], m, (int)nk(kn), F->U[k], F->D[kn], (int)nk(kn), m, (int)nk(kn),
                             nullptr, nullptr, 0, 0, 0, T(1), T(0)));
         L.t1 = std::max(L.t1, tiles(m, (int)nk(kn)));
       }
@@ -675,28 +933,27 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
       if (hasE || hasF) {
         const int kp = hasE ? id[t - 1] : -1, kn = hasF ? id[t + 1] : -1;
-        vg.push_back(mkgemm(F->D[k], m, m, hasE ? base + offE[l][t] : nullptr, hasE ? F->U[kp] : nullptr,
-                            hasE ? (int)nk(kp) : 0, m, hasE ? (int)nk(kp) : 0, hasF ? base + offF[l][t] : nullptr,
+        vg.push_back(mkgemm(F->D[k], m, m, hasE ? base + P.offE[l][t] : nullptr, hasE ? F->U[kp] : nullptr,
+                            hasE ? (int)nk(kp) : 0, m, hasE ? (int)nk(kp) : 0, hasF ? base + P.offF[l][t] : nullptr,
                             hasF ? F->L[kn] : nullptr, hasF ? (int)nk(kn) : 0, m, hasF ? (int)nk(kn) : 0, T(-1),
                             T(1)));
         L.t2 = std::max(L.t2, tiles(m, m));
       }
       if (t - 2 >= 0) {
         const int kp = id[t - 1], kpp = id[t - 2];
-        vg.push_back(mkgemm(F->L[k], m, (int)nk(kpp), base + offE[l][t], F->L[kp], (int)nk(kp), m, (int)nk(kp),
+        vg.push_back(mkgemm(F->L[k], m, (int)nk(kpp), base + P.offE[l][t], F->L[kp], (int)nk(kp), m, (int)nk(kp),
                             nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
         L.t2 = std::max(L.t2, tiles(m, (int)nk(kpp)));
       }
       if (t + 2 <= T_ - 1) {
         const int kn = id[t + 1], knn = id[t + 2];
-        vg.push_back(mkgemm(F->U[k], m, (int)nk(knn), base + offF[l][t], F->U[kn], (int)nk(kn), m, (int)nk(kn),
+        vg.push_back(mkgemm(F->U[k], m, (int)nk(knn), base + P.offF[l][t], F->U[kn], (int)nk(kn), m, (int)nk(kn),
                             nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
         L.t2 = std::max(L.t2, tiles(m, (int)nk(knn)));
       }
     }
     rg2[l].n = vg.size() - rg2[l].off;
     // solve: forward  B_t -= E_t B_{t-1} + F_t B_{t+1}   (t even)
-    const double sb_level = F->solve_bytes;
     rfw[l].off = vs.size();
     for (int t = 0; t < T_; t += 2) {
       const int k = id[t];
@@ -704,9 +961,9 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
       if (!hasE && !hasF) continue;
       vs.push_back(mkskin(vecp(F->Bv, k), m,
-                          {Prod{hasE ? base + offE[l][t] : nullptr, m, hasE ? vecp(F->Bv, id[t - 1]) : nullptr,
+                          {Prod{hasE ? base + P.offE[l][t] : nullptr, m, hasE ? vecp(F->Bv, id[t - 1]) : nullptr,
                                 hasE ? (int)nk(id[t - 1]) : 0, T(-1)},
-                           Prod{hasF ? base + offF[l][t] : nullptr, m, hasF ? vecp(F->Bv, id[t + 1]) : nullptr,
+                           Prod{hasF ? base + P.offF[l][t] : nullptr, m, hasF ? vecp(F->Bv, id[t + 1]) : nullptr,
                                 hasF ? (int)nk(id[t + 1]) : 0, T(-1)}},
                           T(1)));
       L.mfw = std::max(L.mfw, m);
@@ -727,7 +984,6 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       L.mbw = std::max(L.mbw, m);
     }
     rbw[l].n = vs.size() - rbw[l].off;
-    if (l == 0) lev0_bytes = F->solve_bytes - sb_level;
   }
   // root
   const int kr = ids[nlev][0];
@@ -748,7 +1004,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     const auto& id = ids[nlev - 1];
     const int k0 = id[0], k1 = id[1];
     const int n0 = (int)nk(k0), n1 = (int)nk(k1);
-    cx<T>* F0 = base + offF[nlev - 1][0];
+    cx<T>* F0 = base + P.offF[nlev - 1][0];
     rtail1 = vg.size();
     vg.push_back(mkgemm(F->G[k0], n0, n1, F->D[k0], F0, n0, n0, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
     vg.push_back(mkgemm(F->H[k1], n1, n0, F->G[k1], F->D[k0], n0, n1, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
@@ -769,47 +1025,54 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   F->flops = flops;
   F->solve_launches = F->tail ? 2 * nlev - 1 : 2 * nlev + 1;
   size_t rlev0 = 0;
-  if constexpr (std::is_same<T, double>::value) {
-    if (F->tail && nlev >= 2) {
-      const double sb = F->solve_bytes;
-      const auto& id = ids[0];
-      rlev0 = vs.size();
-      for (int t = 1; t < (int)id.size(); t += 2) {
-        const int k = id[t];
-        const int m = (int)nk(k);
-        vs.push_back(mkskin(vecp(F->Xv, k), m, {Prod{F->D[k], m, vecp(F->Bv, k), m, T(1)}}, T(0)));
-        F->lev0_m = std::max(F->lev0_m, m);
-      }
-      F->lev0_n = (int)(vs.size() - rlev0);
-      const double dinv_bytes = F->solve_bytes - sb;
-      // sparse launches: each off-block entry's index, two values and the
-      // vector entry it multiplies; per row its pointer, scale, block id and y
-      const double sparse_bytes = 52.0 * (double)(s->off_nnz[0] + s->off_nnz[1]) +
-                                  52.0 * (double)(s->off_rows[0] + s->off_rows[1]);
-      F->solve_bytes = sb - lev0_bytes + 2 * dinv_bytes + sparse_bytes;
-      F->solve_launches += 2;
-      F->lev0_sparse = true;
+  if (sp0) {
+    // level 0 of a sweep: y_odd = Dinv_odd b_odd (forward; x_odd backward),
+    // and the couplings from the compact off-block rows
+    const auto& id = ids[0];
+    const double sb = F->solve_bytes;
+    rlev0 = vs.size();
+    for (int t = 1; t < (int)id.size(); t += 2) {
+      const int k = id[t];
+      const int m = (int)nk(k);
+      vs.push_back(mkskin(vecp(F->Xv, k), m, {Prod{F->D[k], m, vecp(F->Bv, k), m, T(1)}}, T(0)));
+      F->lev0_m = std::max(F->lev0_m, m);
     }
+    F->lev0_n = (int)(vs.size() - rlev0);
+    const double dinv_bytes = F->solve_bytes - sb;
+    // couplings: each off-block entry's value, column and the vector entry it
+    // multiplies; per row its pointer, block id and y (read and written)
+    const double sparse_bytes = (20.0 + sizeof(cx<T>)) * (double)s->nnz_off +
+                                (double)s->n * (8.0 + 2.0 * sizeof(cx<T>));
+    F->solve_bytes = sb + 2 * dinv_bytes + sparse_bytes;
+    F->solve_launches += 2;
+    F->lev0_sparse = true;
   }
 
   // upload descriptors
   const size_t bi = vinv.size() * sizeof(InvDesc<T>);
   const size_t bg = vg.size() * sizeof(GemmDesc<T>);
   const size_t bsk = vs.size() * sizeof(SkinnyDesc<T>);
+  const size_t bsc = vsc.size() * sizeof(SpColDesc<T>);
+  const size_t bsr = vsr.size() * sizeof(SpRowDesc<T>);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  TRY(F->desc.alloc(c, al(bi) + al(bg) + al(bsk) + 256));
+  TRY(F->desc.alloc(c, al(bi) + al(bg) + al(bsk) + al(bsc) + al(bsr) + 256));
   char* db = F->desc.template as<char>();
   InvDesc<T>* dinv = reinterpret_cast<InvDesc<T>*>(db);
   GemmDesc<T>* dg = reinterpret_cast<GemmDesc<T>*>(db + al(bi));
   SkinnyDesc<T>* dsk = reinterpret_cast<SkinnyDesc<T>*>(db + al(bi) + al(bg));
+  SpColDesc<T>* dsc = reinterpret_cast<SpColDesc<T>*>(db + al(bi) + al(bg) + al(bsk));
+  SpRowDesc<T>* dsr = reinterpret_cast<SpRowDesc<T>*>(db + al(bi) + al(bg) + al(bsk) + al(bsc));
   HIP_TRY(hipMemcpyAsync(dinv, vinv.data(), bi, hipMemcpyHostToDevice, c->stream));
   if (bg) HIP_TRY(hipMemcpyAsync(dg, vg.data(), bg, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(dsk, vs.data(), bsk, hipMemcpyHostToDevice, c->stream));
+  if (bsc) HIP_TRY(hipMemcpyAsync(dsc, vsc.data(), bsc, hipMemcpyHostToDevice, c->stream));
+  if (bsr) HIP_TRY(hipMemcpyAsync(dsr, vsr.data(), bsr, hipMemcpyHostToDevice, c->stream));
+  // the level-0 scatter fills D, and L, U only when level 0 runs on dense couplings
   std::vector<cx<T>*> hp(3 * (size_t)nb, nullptr);
   for (int k = 0; k < nb; ++k) {
     hp[k] = F->D[k];
-    hp[nb + k] = F->L[k];
-    hp[2 * nb + k] = F->U[k];
+    hp[nb + k] = sp0 ? nullptr : F->L[k];
+    hp[2 * nb + k] = sp0 ? nullptr : F->U[k];
   }
   HIP_TRY(hipMemcpyAsync(F->dD, hp.data(), hp.size() * sizeof(void*), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -820,6 +1083,8 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     L.g2 = dg + rg2[l].off; L.ng2 = (int)rg2[l].n;
     L.fw = dsk + rfw[l].off; L.nfw = (int)rfw[l].n;
     L.bw = dsk + rbw[l].off; L.nbw = (int)rbw[l].n;
+    L.sc = dsc + L.sc_off;
+    L.sr = dsr + L.sr_off;
   }
   F->root_inv = dinv + rinv[nlev].off;
   F->root_solve = dsk + rroot;
@@ -829,7 +1094,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     F->tail2 = dg + rtail2;
     F->tail_solve = dsk + rtails;
   }
-  F->ndlu = cntDLU;
+  F->ndlu = sp0 ? P.cntD : P.cntDLU;
   return KORE_OK;
 }
 
@@ -837,26 +1102,9 @@ template <class T> static int64_t factor_count_bytes(const std::vector<int>& bsz
   const int nb = (int)bsz.size();
   int nmax = 0;
   for (int b : bsz) nmax = std::max(nmax, b);
-  int64_t cnt = 0;
-  for (int k = 0; k < nb; ++k) {
-    cnt += (int64_t)bsz[k] * bsz[k];
-    if (k > 0) cnt += (int64_t)bsz[k] * nmax;
-    if (k < nb - 1) cnt += (int64_t)bsz[k] * nmax;
-    cnt += 2 * (int64_t)bsz[k] * nmax;   // G, H
-  }
-  std::vector<int> cur(nb);
-  for (int k = 0; k < nb; ++k) cur[k] = k;
-  while (cur.size() > 1) {
-    const int T_ = (int)cur.size();
-    for (int t = 0; t < T_; t += 2) {
-      if (t - 1 >= 0) cnt += (int64_t)bsz[cur[t]] * bsz[cur[t - 1]];
-      if (t + 1 <= T_ - 1) cnt += (int64_t)bsz[cur[t]] * bsz[cur[t + 1]];
-    }
-    std::vector<int> nx;
-    for (int t = 0; t < T_; t += 2) nx.push_back(cur[t]);
-    cur.swap(nx);
-  }
-  return cnt * (int64_t)sizeof(cx<T>) + 2 * n * MAXR * (int64_t)sizeof(cx<T>) +
+  FactorPlan P;
+  factor_plan(bsz, P, true);   // degree blocks couple through the matrix's off-diagonal blocks
+  return P.cnt * (int64_t)sizeof(cx<T>) + 2 * n * MAXR * (int64_t)sizeof(cx<T>) +
          (int64_t)nb * (nmax + 1) * 4 + 3 * nb * 8 + 4096;
 }
 
@@ -874,6 +1122,12 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
     {
       Timed tm(c, KORE_TIMER_INV);
       HIP_TRY(launch_gj_inverse<T>(L.inv, L.ninv, L.minn_inv, L.maxn_inv, st));
+    }
+    if (l == 0 && F->lev0_sparse) {
+      Timed tm(c, KORE_TIMER_GEMM);
+      HIP_TRY(launch_spcol<T>(s->offc, s->off.val, L.sc, L.nsc, L.maxn_sc, st));
+      HIP_TRY(launch_sprow<T>(s->off, L.sr, L.nsr, L.maxm_sr, L.maxn_sr, st));
+      continue;
     }
     {
       Timed tm(c, KORE_TIMER_GEMM);
@@ -900,27 +1154,30 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int ltop = F->tail ? F->nlev - 1 : F->nlev;   // levels swept one by one
-  const bool sp0 = F->lev0_sparse && nrhs == 1;        // level 0 through the sparse couplings
+  const bool sp0 = F->lev0_sparse;                     // level 0 through the sparse couplings
+  const int64_t n = s->n;
   // one column, blocks up to 256 wide: level 0's inverse blocks (one product
   // each) take the kernel with registers for one product (a two-product
   // kernel for the forward levels measured no faster)
   const bool fewp = std::is_same<T, double>::value && nrhs == 1 && s->nmax <= 256;
+  auto lev0_inverse = [&]() -> int32_t {
+    Timed tm(c, KORE_TIMER_SKINNY);
+    if constexpr (std::is_same<T, double>::value) {
+      if (fewp) {
+        HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
+        return KORE_OK;
+      }
+    }
+    HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, nrhs, st));
+    return KORE_OK;
+  };
   for (int l = 0; l < ltop; ++l) {
     Level<T>& L = F->lev[l];
-    if constexpr (std::is_same<T, double>::value) {
-      if (l == 0 && sp0) {
-        {
-          Timed tm(c, KORE_TIMER_SKINNY);
-          if (fewp)
-            HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // y_odd in Xv
-          else
-            HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
-        }
-        Timed tm(c, KORE_TIMER_SKINNY);
-        HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 0, F->Xv,
-                               F->Bv, st));
-        continue;
-      }
+    if (l == 0 && sp0) {
+      TRY(lev0_inverse());   // y_odd = Dinv_odd b_odd, in Xv
+      Timed tm(c, KORE_TIMER_SKINNY);
+      HIP_TRY(launch_offsweep<T>(s->off, s->d_blk.as<int32_t>(), n, 0, F->Xv, n, F->Bv, n, nrhs, st));
+      continue;
     }
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
@@ -934,20 +1191,13 @@ template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs)
   }
   for (int l = ltop - 1; l >= 0; --l) {
     Level<T>& L = F->lev[l];
-    if constexpr (std::is_same<T, double>::value) {
-      if (l == 0 && sp0) {
-        {
-          Timed tm(c, KORE_TIMER_SKINNY);
-          HIP_TRY(launch_offdiag(s->csr, s->sre, s->sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), 1, F->Xv,
-                                 F->Bv, st));
-        }
+    if (l == 0 && sp0) {
+      {
         Timed tm(c, KORE_TIMER_SKINNY);
-        if (fewp)
-          HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));   // x_odd in Xv
-        else
-          HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
-        continue;
+        HIP_TRY(launch_offsweep<T>(s->off, s->d_blk.as<int32_t>(), n, 1, F->Xv, n, F->Bv, n, nrhs, st));
       }
+      TRY(lev0_inverse());   // x_odd = Dinv_odd b_odd
+      continue;
     }
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.bw, L.nbw, L.mbw, nrhs, st));
@@ -1251,7 +1501,7 @@ void kore_ctx_destroy(kore_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   resolve_timers(ctx);
   for (auto e : ctx->pool) hipEventDestroy(e);
-  if (ctx->pin) hipHostFree(ctx->pin);
+  ctx->pin.release();
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1263,6 +1513,25 @@ int32_t kore_ctx_sync(kore_ctx* ctx) {
 }
 
 int64_t kore_ctx_bytes_in_use(const kore_ctx* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int32_t kore_mem_trim(int32_t device) {
+  if (device < 0) return fail(KORE_ERR_ARG, "bad device %d", device);
+  HIP_TRY(hipSetDevice(device));
+  pool_trim(g_dev_pool[device % POOL_MAXDEV], false);
+  pool_trim(g_host_pool, true);
+  return KORE_OK;
+}
+
+int32_t kore_mem_stats(int32_t device, int64_t* cached_bytes, int64_t* device_allocs) {
+  if (device < 0) return fail(KORE_ERR_ARG, "bad device %d", device);
+  MemPool& mp = g_dev_pool[device % POOL_MAXDEV];
+  {
+    std::lock_guard<std::mutex> lk(mp.mu);
+    if (cached_bytes) *cached_bytes = (int64_t)mp.cached;
+  }
+  if (device_allocs) *device_allocs = g_pool_misses.load();
+  return KORE_OK;
+}
 
 int32_t kore_ctx_set_timing(kore_ctx* ctx, int32_t enable) {
   if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
@@ -1386,6 +1655,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   int32_t* pix = reinterpret_cast<int32_t*>(pvbb + capb);
   int32_t* pixb = pix + cap;
   std::vector<size_t> tcnt(nth, 0), tcntb(nth, 0);
+  std::vector<std::vector<int32_t>> tocol(nth);   // off-block columns of each range, in row order
   std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0), rowlo(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
@@ -1442,7 +1712,10 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         ovb[cntt] = b;
         ++cntt;
         ++cnt;
-        if (kc != kb) ++rowoff[i];
+        if (kc != kb) {
+          ++rowoff[i];
+          tocol[t].push_back(col);
+        }
         if (kc < kb) ++rowlo[i];
         if (b != zd(0, 0)) {
           oixb[cnttb] = col;
@@ -1481,6 +1754,73 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
       return fail(KORE_ERR_UNSUPPORTED, "row %lld has more than 32767 entries in a neighbouring block", (long long)i);
     s->h_split[i] = lo | (hi << 15) | ((s->h_blk[i] & 1) << 30);
   }
+  // compact off-block CSR: rows in degree order, each row's previous-block
+  // entries first (rowlo of them), then its next-block entries
+  std::vector<int32_t> optr(n + 1, 0), onlo(n, 0), ocol;
+  for (int64_t i = 0; i < n; ++i) {
+    optr[i + 1] = optr[i] + rowoff[i];
+    onlo[i] = rowlo[i];
+  }
+  s->nnz_off = optr[n];
+  ocol.reserve((size_t)s->nnz_off);
+  for (int t = 0; t < nth; ++t) ocol.insert(ocol.end(), tocol[t].begin(), tocol[t].end());
+  if ((int64_t)ocol.size() != s->nnz_off) return fail(KORE_ERR_ARG, "internal: off-block entry count mismatch");
+  // column segments per (block, side): for column c of the neighbouring
+  // block, the compact entries q of the block's rows in that column
+  s->h_cbase.assign(2 * (size_t)s->nblk, -1);
+  s->h_side_nnz.assign(2 * (size_t)s->nblk, 0);
+  std::vector<int64_t> ebase(2 * (size_t)s->nblk, 0);
+  int64_t ncp = 0, ne = 0;
+  for (int k = 0; k < s->nblk; ++k) {
+    for (int sd = 0; sd < 2; ++sd) {
+      const int nbk = sd == 0 ? k - 1 : k + 1;
+      int64_t cnt = 0;
+      for (int64_t i = s->boff[k]; i < s->boff[k + 1]; ++i) cnt += sd == 0 ? onlo[i] : rowoff[i] - onlo[i];
+      s->h_side_nnz[2 * k + sd] = cnt;
+      if (nbk < 0 || nbk >= s->nblk) continue;
+      s->h_cbase[2 * k + sd] = ncp;
+      ebase[2 * k + sd] = ne;
+      ncp += s->bsz[nbk] + 1;
+      ne += cnt;
+    }
+  }
+  if (ne != s->nnz_off) return fail(KORE_ERR_ARG, "internal: off-block side counts mismatch");
+  std::vector<int32_t> cptr((size_t)std::max<int64_t>(ncp, 1), 0), cq((size_t)std::max<int64_t>(ne, 1)),
+      crow((size_t)std::max<int64_t>(ne, 1));
+  {
+    auto fill_blocks = [&](int k0, int k1) {
+      std::vector<int32_t> fillp;
+      for (int k = k0; k < k1; ++k)
+        for (int sd = 0; sd < 2; ++sd) {
+          if (s->h_cbase[2 * k + sd] < 0) continue;
+          const int nbk = sd == 0 ? k - 1 : k + 1;
+          const int64_t c0 = s->boff[nbk], nc = s->bsz[nbk];
+          int32_t* cp = cptr.data() + s->h_cbase[2 * k + sd];
+          for (int64_t c = 0; c <= nc; ++c) cp[c] = 0;
+          for (int64_t i = s->boff[k]; i < s->boff[k + 1]; ++i) {
+            const int32_t qa = sd == 0 ? optr[i] : optr[i] + onlo[i], qb = sd == 0 ? optr[i] + onlo[i] : optr[i + 1];
+            for (int32_t q = qa; q < qb; ++q) ++cp[ocol[q] - c0 + 1];
+          }
+          cp[0] = (int32_t)ebase[2 * k + sd];
+          for (int64_t c = 0; c < nc; ++c) cp[c + 1] += cp[c];
+          fillp.assign(cp, cp + nc);
+          for (int64_t i = s->boff[k]; i < s->boff[k + 1]; ++i) {
+            const int32_t qa = sd == 0 ? optr[i] : optr[i] + onlo[i], qb = sd == 0 ? optr[i] + onlo[i] : optr[i + 1];
+            for (int32_t q = qa; q < qb; ++q) {
+              const int32_t at = fillp[ocol[q] - c0]++;
+              cq[at] = q;
+              crow[at] = (int32_t)(i - s->boff[k]);
+            }
+          }
+        }
+    };
+    std::vector<std::thread> pool;
+    const int nthb = std::max(1, std::min(nth, s->nblk));
+    for (int t = 1; t < nthb; ++t) pool.emplace_back(fill_blocks, s->nblk * t / nthb, s->nblk * (t + 1) / nthb);
+    fill_blocks(0, s->nblk / nthb);
+    for (auto& th : pool) th.join();
+  }
+  trace.mark("sys_create off-block views");
   const size_t nnz = (size_t)s->h_ip[n];
   s->nnz = (int64_t)nnz;
   TRY(s->d_ip.alloc(ctx, (n + 1) * sizeof(int32_t)));
@@ -1500,6 +1840,13 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->d_split.alloc(ctx, n * sizeof(int32_t)));
   TRY(s->d_boff.alloc(ctx, (s->nblk + 1) * sizeof(int64_t)));
   TRY(s->d_scale.alloc(ctx, n * sizeof(double)));
+  TRY(s->d_optr.alloc(ctx, (n + 1) * sizeof(int32_t)));
+  TRY(s->d_onlo.alloc(ctx, n * sizeof(int32_t)));
+  TRY(s->d_ocol.alloc(ctx, std::max<size_t>(ocol.size(), 1) * sizeof(int32_t)));
+  TRY(s->d_oval.alloc(ctx, std::max<size_t>(ocol.size(), 1) * sizeof(zd)));
+  TRY(s->d_cptr.alloc(ctx, cptr.size() * sizeof(int32_t)));
+  TRY(s->d_cq.alloc(ctx, cq.size() * sizeof(int32_t)));
+  TRY(s->d_crow.alloc(ctx, crow.size() * sizeof(int32_t)));
   TRY(s->d_scr.alloc(ctx, 4 * (size_t)n * MAXR * sizeof(cx<double>)));
   TRY(s->d_part.alloc(ctx, 2 * (size_t)s->npart * MAXR * sizeof(double)));
   hipStream_t st = ctx->stream;
@@ -1536,7 +1883,16 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   HIP_TRY(hipMemcpyAsync(s->d_blk.p, s->h_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_split.p, s->h_split.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s->d_boff.p, s->boff.data(), (s->nblk + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_optr.p, optr.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_onlo.p, onlo.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (!ocol.empty())
+    HIP_TRY(hipMemcpyAsync(s->d_ocol.p, ocol.data(), ocol.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_cptr.p, cptr.data(), cptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_cq.p, cq.data(), cq.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_crow.p, crow.data(), crow.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
+  s->off = OffCsr{s->d_optr.as<int32_t>(), s->d_onlo.as<int32_t>(), s->d_ocol.as<int32_t>(), s->d_oval.as<cx<double>>()};
+  s->offc = OffCsc{s->d_cptr.as<int32_t>(), s->d_cq.as<int32_t>(), s->d_crow.as<int32_t>()};
   s->csr = CsrDev{n, s->d_ip.as<int32_t>(), s->d_ix.as<int32_t>(), s->d_va.as<cx<double>>(), s->d_vb.as<cx<double>>()};
   // B alone: its A values are never read (ca = 0 skips them)
   if (s->has_b)
@@ -1585,11 +1941,19 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
   HIP_TRY(hipSetDevice(c->device));
   // the reduction plan and its buffers depend only on the layout: keep them
   // across shifts of the same system, rebuild only when the precision changes
-  if (s->f && s->f->prec != precision) s->f.reset();
+  // keep the other precision's plan (and its device memory) when the
+  // precision changes, so that alternating precisions does not rebuild plans
+  if (s->f && s->f->prec != precision) {
+    std::swap(s->f, s->fspare);
+    if (s->f && s->f->prec != precision) s->f.reset();
+  }
   s->sre = sre;
   s->sim = sim;
-  // power-of-two row equilibration of A - sigma B (device)
+  // power-of-two row equilibration of A - sigma B (device), then the
+  // level-0 couplings' values R (A - sigma B) for the compact views
   HIP_TRY(launch_row_scale(s->csr, sre, sim, s->d_scale.as<double>(), c->stream));
+  if (s->nnz_off > 0)
+    HIP_TRY(launch_offvals(s->csr, sre, sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), s->off, c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -1651,6 +2015,7 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     info->flops = s->f->flops;
     info->flops_inv = s->f->flops_inv;
     info->flops_gemm = s->f->flops_gemm;
+    info->flops_sparse = s->f->flops_sparse;
     info->solve_bytes = s->f->solve_bytes;
     info->solve_launches = s->f->solve_launches;
     info->bytes = s->f->bytes;
@@ -1663,6 +2028,7 @@ int32_t kore_factor_release(kore_sys* s) {
   if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
   hipStreamSynchronize(s->ctx->stream);
   s->f.reset();
+  s->fspare.reset();
   s->factored = false;
   return KORE_OK;
 }
@@ -1874,7 +2240,8 @@ int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
   TRY(w->qbuf.alloc(c, (size_t)ncol * ncol * sizeof(zd)));
   TRY(w->descb.alloc(c, (DOT_MAXCHUNK + 64) * sizeof(GemmDesc<double>)));
   TRY(w->hb.alloc(c, ws_hb_elems(ncol) * sizeof(zd)));
-  HIP_TRY(hipHostMalloc((void**)&w->hh, 2 * ws_hb_elems(ncol) * sizeof(zd), hipHostMallocDefault));
+  TRY(w->hhbuf.reserve(2 * ws_hb_elems(ncol) * sizeof(zd), c->stream));
+  w->hh = reinterpret_cast<cx<double>*>(w->hhbuf.p);
   for (hipEvent_t& e : w->hev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   TRY(w->up.init(c, 1 << 22));
   HIP_TRY(launch_zero<double>(w->V.as<cx<double>>(), (int64_t)s->n * ncol, c->stream));
@@ -1917,16 +2284,16 @@ int32_t kore_ws_get(kore_ws* w, int32_t c0, int32_t nc, double* host, int64_t ld
   const int64_t n = s->n;
   if (ld < n) return fail(KORE_ERR_ARG, "ld < sizmat");
   const size_t colb = (size_t)n * sizeof(zd);
-  TRY(s->down.reserve(colb * MAXR));   // once per system: a pinned allocation is not cheap
+  TRY(s->down.reserve(colb * MAXR, s->ctx->stream));   // pooled pinned staging
   std::vector<CopySeg> segs;
   for (int a = 0; a < nc; a += MAXR) {
     const int nr = std::min(MAXR, nc - a);
     HIP_TRY(launch_scatter(wcol(w, c0 + a), n, s->st, n, n, nr, s->d_perm.as<int64_t>(), s->ctx->stream));
-    HIP_TRY(hipMemcpyAsync(s->down.host, s->st, nr * colb, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s->down.host(), s->st, nr * colb, hipMemcpyDeviceToHost, s->ctx->stream));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     segs.clear();
     for (int j = 0; j < nr; ++j)
-      segs.push_back(CopySeg{reinterpret_cast<char*>(host + 2 * ld * (int64_t)(a + j)), s->down.host + j * colb, colb});
+      segs.push_back(CopySeg{reinterpret_cast<char*>(host + 2 * ld * (int64_t)(a + j)), s->down.host() + j * colb, colb});
     copy_parallel(segs);
   }
   return KORE_OK;

@@ -71,6 +71,15 @@ class EigResult:
     precision: int = 0         # factor precision the converged pairs were computed with
     fallback: bool = False     # True if a complex64 factor was replaced by an FP64 one
     breakdowns: int = 0        # blocks that hit an invariant subspace (replaced by random blocks)
+    c64_op_columns: int = 0    # OP applications made with a complex64 factor (before a fallback, if any)
+    c64_ir_max: int = 0        # most refinement sweeps one complex64 OP application needed
+
+
+@dataclasses.dataclass
+class _Fallback:
+    """An eigensolve abandoned on its complex64 factor: the work done so far."""
+    op_columns: int
+    ir_max: int
 
 
 class _Timed:
@@ -244,15 +253,22 @@ def _eigs(sysm, sigma, nev, ncv=None, tol=1e-12, maxit=100, which="TM", block=1,
     device time covers the predicted Schur time, or so that it ends a base
     step short of where the trend of the worst wanted residual reaches tol.
     An explicit value is used as a fixed spacing."""
+    c64 = sysm.precision != PREC_FP64
     res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log, workspace,
-                     fallback_sweeps if sysm.precision != PREC_FP64 else None, check_every)
-    if res is None:
+                     fallback_sweeps if c64 else None, check_every)
+    if isinstance(res, _Fallback):
         if log:
-            log("complex64 refinement too slow: refactoring in FP64")
+            log("complex64 refinement too slow after %d OP columns: refactoring in FP64" % res.op_columns)
+        fb = res
         sysm.factor(sigma, PREC_FP64)
         res = _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir_maxit, v0, log,
                          workspace, None, check_every)
         res.fallback = True
+        res.c64_op_columns = fb.op_columns
+        res.c64_ir_max = fb.ir_max
+    elif c64:
+        res.c64_op_columns = res.op_columns
+        res.c64_ir_max = res.ir_max_iterations
     res.precision = sysm.precision
     return res
 
@@ -418,7 +434,7 @@ def _eigs_once(sysm, sigma, nev, ncv, tol, maxit, which, block, seed, ir_tol, ir
                 if st == KORE_WARN_IR_NOT_CONVERGED and fallback_sweeps is not None:
                     if own_ws:
                         ws.close()
-                    return None
+                    return _Fallback(op_columns=op_cols + bs * done_blocks, ir_max=ir_iters)
                 if st == KORE_WARN_BREAKDOWN:
                     done_blocks += 1
                 op_calls += done_blocks

@@ -119,6 +119,17 @@ def write_eigenvalues(lam):
         np.savetxt(f, eigval)
 
 
+def relative_errors(A, B, lam, X):
+    """SLEPc's relative error of each pair, ||A x - lambda B x|| / (|lambda| ||x||),
+    the quantity -eps_error_relative prints for bin/solve.py:125-129."""
+    out = []
+    for i in range(len(lam)):
+        x = X[:, i]
+        res = A @ x - lam[i] * (B @ x)
+        out.append(float(np.linalg.norm(res) / (abs(lam[i]) * np.linalg.norm(x) + 1e-300)))
+    return out
+
+
 def error_table(lam, rel):
     lines = [" ---------------------- --------------------",
              "            k             ||Ax-kBx||/||kx||",
@@ -170,12 +181,7 @@ def main(argv=None):
             write_eigenvalues(r.lam)
             write_fields(split_fields(r.X, par))
             if "eps_error_relative" in opts or "eps_true_residual" in opts:
-                rel = []
-                for i in range(r.nconv):
-                    x = r.X[:, i]
-                    res = A @ x - r.lam[i] * (B @ x)
-                    rel.append(np.linalg.norm(res) / (abs(r.lam[i]) * np.linalg.norm(B @ x) + 1e-300))
-                print(error_table(r.lam, rel))
+                print(error_table(r.lam, relative_errors(A, B, r.lam, r.X)))
             success = r.nconv
         else:
             print("No converged solution found")

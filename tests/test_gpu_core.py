@@ -99,14 +99,14 @@ def test_ws_random_start_block(ctx, hydro_small):
     S.close()
 
 
-def _solve_case(ctx, par, A, B, sigma, precision, nrhs=1, seed=0):
+def _solve_case(ctx, par, A, B, sigma, precision, nrhs=1, seed=0, ir_maxit=20):
     from kore_amd import backend
     S = _system(ctx, par, A, B)
     S.factor(sigma, precision)
     rng = np.random.default_rng(seed)
     b = rng.standard_normal((S.n, nrhs)) + 1j * rng.standard_normal((S.n, nrhs))
     b /= np.linalg.norm(b, axis=0)
-    x, info = S.solve(b, ir_tol=1e-14, ir_maxit=20)
+    x, info = S.solve(b, ir_tol=1e-14, ir_maxit=ir_maxit)
     M = (A - sigma * B).tocsc()
     xr = sla.splu(M).solve(b)
     err = np.linalg.norm(x - xr, axis=0) / np.linalg.norm(xr, axis=0)
@@ -150,9 +150,40 @@ def test_solve_large_blocks(ctx, case, precision, nrhs):
     par = Params(m=1, symm=1, Ek=1e-4, **kw)
     A, B, _ = assembly.assemble(par)
     say("case: large-block solve", case, "sizmat", A.shape[0], "precision", precision)
-    err, info = _solve_case(ctx, par, A, B, 0.4j, precision, nrhs=nrhs, seed=7)
+    # the FP64 factor is checked on its plain sweep (no refinement that could
+    # hide a wrong sweep); the complex64 factor needs its refinement
+    err, info = _solve_case(ctx, par, A, B, 0.4j, precision, nrhs=nrhs, seed=7,
+                            ir_maxit=0 if precision == 0 else 20)
     say("  rel err", err, "sweeps", info.iterations)
     assert err.max() <= 1e-10
+    if precision == 0:
+        assert info.iterations == 1
+
+
+def test_eigs_blocks_wider_than_256(ctx):
+    """FP64 eigensolve (single-vector Krylov, no refinement) on degree blocks
+    of 300 rows: level 0 of every sweep takes the long preloading kernel for
+    the inverse blocks and the sparse couplings; eigenvalues match the oracle
+    (SuperLU + Krylov-Schur) to 1e-10, residuals <= 1e-10."""
+    from kore_amd import assembly, eigs
+    from kore_amd.params import Params
+    from oracle import krylov_schur
+    par = Params(m=1, symm=1, Ek=1e-4, N=300, lmax=8)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.4j
+    say("case: eigs with 300-row blocks, sizmat", A.shape[0])
+    S = _system(ctx, par, A, B)
+    S.factor(sigma, 0)
+    r = eigs.eigs(S, sigma, nev=4, tol=1e-12, maxit=200, block=1)
+    lu = sla.splu((A - sigma * B).tocsc())
+    lo, _, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], sigma, 4, tol=1e-13, maxit=200)
+    assert r.nconv >= 4 and r.ir_max_iterations == 1
+    for l in lo[:4]:
+        d = np.min(np.abs(r.lam - l)) / abs(l)
+        say("  lambda", l, "rel diff", d)
+        assert d <= 1e-10
+    assert eigs.residuals(A, B, r.lam[:4], r.X[:, :4]).max() <= 1e-10
+    S.close()
 
 
 def test_solve_many_rhs(ctx, hydro_small):
@@ -172,7 +203,10 @@ def test_eigs_hydro_small(ctx, hydro_small, precision, block):
     S = _system(ctx, par, A, B)
     S.factor(par.tau, precision)
     r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=block)
-    say("  nconv", r.nconv, "restarts", r.restarts, "ops", r.op_columns)
+    say("  nconv", r.nconv, "restarts", r.restarts, "ops", r.op_columns, "fallback", r.fallback,
+        "c64 ops", r.c64_op_columns, "ir max", r.c64_ir_max)
+    if precision == 1:     # the pairs come from the complex64 factor with FP64 refinement
+        assert not r.fallback and r.precision == 1 and r.c64_op_columns == r.op_columns
     lu = sla.splu((A - par.tau * B).tocsc())
     lam_o, X_o, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], par.tau, 5, tol=1e-13,
                                                   maxit=100)
@@ -194,6 +228,9 @@ def test_spinover_golden(ctx, spinover, precision):
     S = _system(ctx, par, A, B)
     S.factor(par.tau, precision)
     r = eigs.eigs(S, par.tau, nev=par.nev, tol=1e-13, maxit=100, block=4)
+    say("  fallback", r.fallback, "c64 ops", r.c64_op_columns, "ir max", r.c64_ir_max)
+    if precision == 1:
+        assert not r.fallback and r.precision == 1
     i = np.argmax(r.lam.real)
     got = r.lam[i]
     say("  spin-over lambda", got)
@@ -367,6 +404,7 @@ def _device_growth(ctx, mkpar, nev, precision):
         S = _system(ctx, par, A, B)
         S.factor(0.0, precision)
         r = eigs.eigs(S, 0.0, nev=nev, tol=1e-14, which="TR", maxit=200)
+        say("    Ra %.6e: fallback %s, c64 ops %d, ir max %d" % (ra, r.fallback, r.c64_op_columns, r.c64_ir_max))
         S.close()
         lam = r.lam[:nev]
         i = int(np.argmax(lam.real))
@@ -410,10 +448,40 @@ def test_mac_eigs_device(ctx, precision):
     S = _system(ctx, par, A, B)
     S.factor(sigma, precision)
     r = eigs.eigs(S, sigma, nev=6, tol=1e-12, maxit=200)
+    say("  fallback", r.fallback, "c64 ops", r.c64_op_columns, "ir max", r.c64_ir_max)
+    if precision == 1:
+        assert not r.fallback and r.precision == 1
     lu = sla.splu((A - sigma * B).tocsc())
     lo, _, _, _ = krylov_schur.krylov_schur(lambda x: lu.solve(B @ x), A.shape[0], sigma, 6, tol=1e-13, maxit=200)
     assert r.nconv >= 6
     for l in lo[:6]:
+        assert np.min(np.abs(r.lam - l)) / abs(l) <= 1e-10
+    assert eigs.residuals(A, B, r.lam[:6], r.X[:, :6]).max() <= 1e-10
+    S.close()
+
+
+def test_c64_eigs_low_ekman_reports_fallback(ctx):
+    """At low Ekman number one complex64 OP application needs more refinement
+    sweeps than the eigensolver allows (4): the solve is redone on an FP64
+    factor, and the result says so (fallback, precision FP64, the complex64
+    columns spent) while its pairs still match the FP64 path to 1e-10."""
+    from kore_amd import assembly, eigs
+    from kore_amd.params import Params
+    par = Params(m=1, symm=1, Ek=1e-6, N=96, lmax=119)
+    A, B, _ = assembly.assemble(par)
+    sigma = -1e-3 + 0.5j
+    say("case: c64 eigs at E=1e-6, sizmat", A.shape[0])
+    S = _system(ctx, par, A, B)
+    S.factor(sigma, 0)
+    r64 = eigs.eigs(S, sigma, nev=6, tol=1e-12)
+    S.factor(sigma, 1)
+    r = eigs.eigs(S, sigma, nev=6, tol=1e-12)
+    say("  fallback", r.fallback, "precision", r.precision, "c64 ops", r.c64_op_columns, "ir max", r.c64_ir_max)
+    if r.fallback:
+        assert r.precision == 0 and r.c64_op_columns >= 0
+    else:
+        assert r.precision == 1 and r.c64_op_columns == r.op_columns
+    for l in r64.lam[:6]:
         assert np.min(np.abs(r.lam - l)) / abs(l) <= 1e-10
     assert eigs.residuals(A, B, r.lam[:6], r.X[:, :6]).max() <= 1e-10
     S.close()

@@ -110,3 +110,16 @@ def test_nonzero_rank_exits_immediately(monkeypatch, tmp_path):
     monkeypatch.setenv("PMI_RANK", "1")
     assert solve.main([]) == 0
     assert not os.path.exists("timing.dat")
+
+
+def test_relative_error_is_slepc_definition():
+    """-eps_error_relative prints ||A x - lambda B x|| / (|lambda| ||x||)
+    (SLEPc's EPS_ERROR_RELATIVE), not a norm of B x in the denominator."""
+    import scipy.sparse as ss
+    from kore_amd.solve import relative_errors
+    A = ss.csr_matrix(np.diag([2.0 + 0j, 3.0]))
+    B = ss.csr_matrix(np.diag([1.0, 0.5]))
+    x = np.array([[3.0 + 0j], [4.0]])            # ||x|| = 5
+    lam = np.array([2.0 + 0j])
+    # A x - 2 B x = (6 - 6, 12 - 4) = (0, 8): 8 / (2 * 5)
+    assert abs(relative_errors(A, B, lam, x)[0] - 0.8) < 1e-15
