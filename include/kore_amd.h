@@ -121,6 +121,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat,
                         const int32_t* a_indptr, const int32_t* a_indices, const double* a_data,
                         const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
                         int32_t b_is_complex, kore_sys** out);
+/* Also destroys the system's workspaces that are still alive. */
 void kore_sys_destroy(kore_sys* sys);
 /* ||A - sigma B||_F of the stored (unscaled) matrices. */
 int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
@@ -171,6 +172,7 @@ int32_t kore_residual(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx,
 /* A block of ncol device vectors of length sizmat (complex128).
  * Replaces: SLEPc's BV basis object inside EPS Krylov-Schur. */
 int32_t kore_ws_create(kore_sys* sys, int32_t ncol, kore_ws** out);
+/* Must precede kore_sys_destroy of its system (which destroys it otherwise). */
 void kore_ws_destroy(kore_ws* ws);
 /* Host <-> device columns, host side in Kore order.
  * Replaces: VecSetValues / Scatter.toZero gathers (bin/solve.py:143-158). */

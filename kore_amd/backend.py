@@ -431,7 +431,11 @@ class Workspace:
 
     def close(self):
         if getattr(self, "_p", None):
-            lib().kore_ws_destroy(self._p)
+            # a system and its cached workspace form a reference cycle, so the
+            # collector may finalise the system first: kore_sys_destroy then
+            # destroyed this workspace already
+            if getattr(self.sys, "_p", None):
+                lib().kore_ws_destroy(self._p)
             self._p = None
 
     def __del__(self):

@@ -612,6 +612,8 @@ struct kore_sys {
   Downloader down;           // pinned staging of kore_ws_get
   DevBuf gm;                 // GMRES basis (restart+1 vectors), dot partials, coefficients
   int gm_cols = 0;
+  std::mutex ws_mu;
+  std::vector<kore_ws*> wss; // live workspaces: destroyed with the system at the latest
 };
 
 struct kore_ws {
@@ -1914,6 +1916,12 @@ void kore_sys_destroy(kore_sys* sys) {
   if (!sys) return;
   hipSetDevice(sys->ctx->device);
   hipStreamSynchronize(sys->ctx->stream);
+  std::vector<kore_ws*> left;
+  {
+    std::lock_guard<std::mutex> lk(sys->ws_mu);
+    left.swap(sys->wss);
+  }
+  for (kore_ws* w : left) delete w;
   delete sys;
 }
 
@@ -2245,13 +2253,24 @@ int32_t kore_ws_create(kore_sys* s, int32_t ncol, kore_ws** out) {
   for (hipEvent_t& e : w->hev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   TRY(w->up.init(c, 1 << 22));
   HIP_TRY(launch_zero<double>(w->V.as<cx<double>>(), (int64_t)s->n * ncol, c->stream));
+  {
+    std::lock_guard<std::mutex> lk(s->ws_mu);
+    s->wss.push_back(w.get());
+  }
   *out = w.release();
   return KORE_OK;
 }
 
 void kore_ws_destroy(kore_ws* w) {
   if (!w) return;
-  hipStreamSynchronize(w->sys->ctx->stream);
+  kore_sys* s = w->sys;
+  hipStreamSynchronize(s->ctx->stream);
+  {
+    std::lock_guard<std::mutex> lk(s->ws_mu);
+    auto it = std::find(s->wss.begin(), s->wss.end(), w);
+    if (it == s->wss.end()) return;   // already destroyed with its system
+    s->wss.erase(it);
+  }
   delete w;
 }
 

@@ -78,6 +78,12 @@ class Result:
     ir_sweeps: int = 0
     error: str = ""
     stages: dict = dataclasses.field(default_factory=dict)   # seconds: setup, factor, solve, check
+    fallback: bool = False     # the complex64 factor was replaced by an FP64 one
+    n: int = 0                 # unknowns
+    factor_flops: dict = dataclasses.field(default_factory=dict)   # algorithmic flops of the factorisations run
+    sweeps: int = 0            # single-column solve sweeps (OP applications, GMRES steps)
+    solve_bytes: float = 0.0   # bytes one sweep reads (factor blocks + level-0 couplings)
+    kernel_ms: dict = dataclasses.field(default_factory=dict)      # device ms per kernel class (instrumented runs)
 
 
 def serving_trace(nreq=1024, seed=1234, e_range=(1e-7, 1e-5), n_cap=500, forced_frac=0.7, rate=None, nev=6,
@@ -158,8 +164,8 @@ class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
     def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
-                 assembly_workers=4, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
-                 gmres_tol=1e-15, gmres_maxit=60):
+                 assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
+                 gmres_tol=1e-15, gmres_maxit=60, instrument=False):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -171,7 +177,11 @@ class Server:
         self.gmres_maxit = gmres_maxit
         self.keep_results = keep_results
         self.log = log
+        self.instrument = instrument
         self.ctxs = [backend.Context(device) for _ in range(slots)]
+        if instrument:               # HIP events around every launch: per-class device times
+            for c in self.ctxs:
+                c.set_timing(True)
         self.free_slots = queue.Queue()
         for i in range(slots):
             self.free_slots.put(i)
@@ -192,8 +202,21 @@ class Server:
         ctx = self.ctxs[slot]
         t_start = time.perf_counter()
         st = {}
+        flops = {}
         lay = backend.layout_from_params(par)
         S = backend.System(ctx, lay, A0, B0)
+        if self.instrument:
+            ctx.timing_reset()
+
+        def factor(prec):
+            fi = S.factor(req.sigma, prec)
+            key = "c64" if prec == backend.PREC_C64 else "fp64"
+            f = flops.setdefault(key, dict(inv=0.0, gemm=0.0, sparse=0.0, n=0))
+            f["inv"] += fi.flops_inv
+            f["gemm"] += fi.flops_gemm
+            f["sparse"] += fi.flops_sparse
+            f["n"] += 1
+            return fi
         try:
             t1 = time.perf_counter()
             st["setup"] = t1 - t_start
@@ -201,28 +224,35 @@ class Server:
             # complex64 factor cannot shorten (each operator application then
             # needs refinement sweeps); they take the FP64 factor directly
             prec = self.precision if req.kind == "forced" else backend.PREC_FP64
-            S.factor(req.sigma, prec)
+            fi = factor(prec)
             t2 = time.perf_counter()
             st["factor"] = t2 - t1
+            fallback = False
             if req.kind == "forced":
                 b = rhs_for(req, S.n)
-                if self.precision == backend.PREC_FP64:
-                    x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=0)
-                else:
-                    # complex64 factor as the preconditioner of FP64 GMRES
-                    x, info = S.solve_gmres(b, tol=self.gmres_tol, maxit=self.gmres_maxit)
-                if self.precision != backend.PREC_FP64 and not info.converged:
-                    # refinement stalled (low Ekman number): FP64 refactorisation
+                # FP64 GMRES preconditioned by the factor: with an FP64 factor it
+                # ends after the first solve; with a complex64 factor it wins back
+                # FP64 accuracy.  Either way it reports the normwise backward
+                # error ||b - (A - sigma B) x|| / (||A - sigma B||_F ||x|| + ||b||)
+                # from the resident matrices, the check of every forced result.
+                x, info = S.solve_gmres(b, tol=self.gmres_tol, maxit=self.gmres_maxit)
+                sweeps = info.iterations
+                if not info.converged and S.precision != backend.PREC_FP64:
+                    # stalled (low Ekman number): FP64 refactorisation
+                    fallback = True
                     t3 = time.perf_counter()
-                    S.factor(req.sigma, backend.PREC_FP64)
+                    S.release_factor()
+                    fi = factor(backend.PREC_FP64)
                     st["factor"] += time.perf_counter() - t3
-                    x, info = S.solve(b, ir_tol=self.ir_tol, ir_maxit=0)
+                    x, info = S.solve_gmres(b, tol=self.gmres_tol, maxit=self.gmres_maxit)
+                    sweeps += info.iterations
                 t4 = time.perf_counter()
                 st["solve"] = t4 - t2 - (st["factor"] - (t2 - t1))
-                res = residual_device(S, req, x=x, b=b)
-                st["check"] = time.perf_counter() - t4
-                out = Result(req.rid, req.kind, 0.0, 0.0, res, res <= 1e-10, x=x if self.keep_results else None,
-                             precision=S.precision, ir_sweeps=info.iterations)
+                res = float(info.rel_correction)
+                st["check"] = 0.0
+                out = Result(req.rid, req.kind, 0.0, 0.0, res, bool(info.converged) and res <= 1e-10,
+                             x=x if self.keep_results else None, precision=S.precision, ir_sweeps=info.iterations,
+                             fallback=fallback, sweeps=sweeps)
             else:
                 r = eigs.eigs(S, req.sigma, nev=req.nev, tol=self.tol, block=self.block, ir_tol=self.ir_tol)
                 t4 = time.perf_counter()
@@ -232,13 +262,19 @@ class Server:
                 st["check"] = time.perf_counter() - t4
                 out = Result(req.rid, req.kind, 0.0, 0.0, res, (res <= 1e-10) and r.nconv >= req.nev,
                              lam=r.lam[:k], x=r.X[:, :k] if self.keep_results else None,
-                             precision=r.precision, ir_sweeps=r.ir_max_iterations)
+                             precision=r.precision, ir_sweeps=r.ir_max_iterations, fallback=r.fallback,
+                             sweeps=r.op_columns)
+            out.solve_bytes = float(S.last_factor.solve_bytes)
+            if self.instrument:
+                out.kernel_ms = {k: v[1] for k, v in ctx.timing().items()}
         finally:
             S.close()
         now = time.perf_counter()
         out.service = now - t_start
         out.latency = now - (t0 + req.arrival)
         out.stages = st
+        out.n = S.n
+        out.factor_flops = flops
         return out
 
     def run(self, trace: List[Request]) -> List[Result]:
@@ -306,10 +342,17 @@ def summarize(results: List[Result], wall: float):
     for r in results:
         for k, v in r.stages.items():
             stages[k] = stages.get(k, 0.0) + v / len(results)
-    return dict(requests=len(results), ok=int(ok.sum()), solves_per_s=len(results) / wall,
+    forced = [r for r in results if r.kind == "forced"]
+    eigen = [r for r in results if r.kind == "eigen"]
+    return dict(requests=len(results), ok=int(ok.sum()), solves_per_s=len(results) / wall, wall_s=wall,
                 p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_service_s=float(np.mean([r.service for r in results])),
                 mean_stage_s=stages,
                 fp64_results=int(sum(r.precision == backend.PREC_FP64 for r in results)),
-                mean_ir_sweeps=float(np.mean([r.ir_sweeps for r in results])),
+                c64_results=int(sum(r.precision == backend.PREC_C64 for r in results)),
+                fallbacks=int(sum(r.fallback for r in results)),
+                mean_forced_sweeps=float(np.mean([r.sweeps for r in forced])) if forced else 0.0,
+                mean_eigen_op_columns=float(np.mean([r.sweeps for r in eigen])) if eigen else 0.0,
+                forced_p50_s=float(np.percentile([r.latency for r in forced], 50)) if forced else 0.0,
+                eigen_p50_s=float(np.percentile([r.latency for r in eigen], 50)) if eigen else 0.0,
                 worst_residual=float(max(r.residual for r in results)))

@@ -21,9 +21,20 @@ Every quantity is a few complex numbers, so the fixture stays tiny while the
 device result is compared entry-free but complete (a random projection of a
 wrong vector disagrees with probability one).
 
+Conditioning (``--conditioning``, added to an existing fixture): for each
+eigenvalue its absolute condition number kappa = ||x|| ||y|| / |y^H B x|
+(x, y the right and left eigenvectors, each one step of inverse iteration
+with SuperLU's factor of A - lambda B at the computed lambda), so that a
+test can bound the difference of two backward-stable answers by
+kappa ||A||_F eps / |lambda| where that exceeds 1e-10; and for the forced
+solve the 1-norm condition number of A0 - i w B0 (scipy's onenormest on
+SuperLU solves), which bounds the forward error of a solution with a given
+backward error.
+
 Run once in the build container (a few minutes, one BLAS thread):
     python tests/golden/make_config_goldens.py [name ...]
-writes tests/golden/config_<name>.json.
+    python tests/golden/make_config_goldens.py --conditioning [name ...]
+writes (or augments) tests/golden/config_<name>.json.
 """
 import json
 import os
@@ -114,11 +125,53 @@ def make(name):
     print("%s: n %d, %d pairs, max residual %.2e, %.1f s -> %s" % (name, n, nev, max(resid), out["seconds"], path))
 
 
+def conditioning(name):
+    from kore_amd import assembly
+    from kore_amd.params import Params
+    path = os.path.join(HERE, "config_%s.json" % name)
+    with open(path) as f:
+        out = json.load(f)
+    case = CASES[name]
+    par = Params(**case["par"])
+    t0 = time.perf_counter()
+    A, B, _ = assembly.assemble(par)
+    n = A.shape[0]
+    nA = float(np.sqrt(np.sum(np.abs(A.data) ** 2)))
+    lam = np.array([complex(*v) for v in out["lam"]])
+    rng = np.random.default_rng(99)
+    kappa = []
+    for l in lam:
+        lu = spla.splu((A - l * B).tocsc())
+        r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        x = lu.solve(r)
+        y = lu.solve(r, trans="H")
+        x /= np.linalg.norm(x)
+        y /= np.linalg.norm(y)
+        kappa.append(float(1.0 / abs(np.vdot(y, B @ x))))
+        del lu
+    out["kappa_abs"] = kappa
+    out["normA_fro"] = nA
+    if case["forced"] is not None:
+        w = case["forced"]
+        A0, B0 = assembly.assemble(par, which="unscaled")
+        M = (A0 - 1j * w * B0).tocsc()
+        lu = spla.splu(M)
+        inv = spla.LinearOperator(M.shape, matvec=lu.solve, rmatvec=lambda v: lu.solve(v, trans="H"),
+                                  dtype=complex)
+        out["forced"]["cond1"] = float(spla.onenormest(inv) * spla.norm(M, 1))
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("%s: kappa_abs in [%.2e, %.2e], %.1f s" % (name, min(kappa), max(kappa), time.perf_counter() - t0))
+
+
 if __name__ == "__main__":
     try:
         from threadpoolctl import threadpool_limits
         threadpool_limits(limits=1)
     except ImportError:
         pass
-    for nm in (sys.argv[1:] or sorted(CASES)):
-        make(nm)
+    args = sys.argv[1:]
+    cond = "--conditioning" in args
+    args = [a for a in args if a != "--conditioning"]
+    for nm in (args or sorted(CASES)):
+        (conditioning if cond else make)(nm)

@@ -7,18 +7,27 @@ on the device.  The oracle's answers on the same matrices (SuperLU +
 oracle Krylov-Schur, written by tests/golden/make_config_goldens.py) are
 stored as fixtures:
 
-* eigenvalues: relative difference <= 1e-10 for every one of the nev
-  oracle values (the bar of bin/solve.py's FP64 path);
 * residuals ||(A - lambda B) x|| / (||A||_F ||x||) <= 1e-10 per pair;
+* eigenvalues: relative difference <= 1e-10 for every one of the nev
+  oracle values (the bar of bin/solve.py's FP64 path), except where the
+  eigenvalue's own conditioning puts two backward-stable answers further
+  apart: with kappa = ||x|| ||y|| / |y^H B x| (stored with the fixture) and
+  the backward errors eta of the two pairs, first-order perturbation theory
+  bounds |lambda_dev - lambda_oracle| by kappa (eta_dev + eta_oracle) ||A||_F;
+  the test allows ten times that.  These pencils are far from normal at low
+  Ekman number (kappa up to ~1e7), so a few of their eigenvalues are
+  determined only to ~1e-9 in FP64 by any solver;
 * eigenvectors, phase-invariantly: the ratios (w_k^H x) / (w_0^H x) for
-  seeded probe vectors w_k agree with the oracle's to 1e-7 relative (an
-  eigenvector is determined to about residual / gap, so this bound is looser
-  than the eigenvalue's; a wrong vector misses it by orders of magnitude);
+  seeded probe vectors w_k agree with the oracle's to 1e-7 relative, or to
+  ten times the eigenvalue bound divided by the gap to the nearest other
+  computed eigenvalue where that is larger (a wrong vector misses either by
+  orders of magnitude);
 * the 200 000-unknown forced solve x = (A0 - 0.5i B0)^{-1} b (seeded
-  unit-norm complex b): ||x|| and the probe projections w_k^H x agree with
-  SuperLU's to 1e-8 relative (forward error ~ cond * eps at E = 1e-7), the
-  FP64 backward error is <= 1e-15 and the complex64 + GMRES path reaches the
-  same backward error bound.
+  unit-norm complex b): the backward error is <= 1e-15 on both paths, and
+  ||x|| and the probe projections w_k^H x agree with SuperLU's to 1e-8, or
+  to ten times cond_1(A0 - 0.5i B0) times the two backward errors where that
+  is larger (the complex64 + GMRES path stops at a backward error near
+  1e-15, the FP64 factor reaches ~1e-19).
 """
 import json
 import os
@@ -78,9 +87,9 @@ def _eigen_case(ctx, name, precision):
     assert r.nconv >= nev
     lam_o = cz(g["lam"])
     W = probes(A.shape[0], g["probe_seed"], g["nprobe"])
-    res = eigs.residuals(A, B, r.lam[:nev], r.X[:, :nev])
-    say("  max residual %.2e" % res.max())
-    assert res.max() <= 1e-10
+    res_all = eigs.residuals(A, B, r.lam, r.X)
+    say("  max residual %.2e" % res_all[:nev].max())
+    assert res_all[:nev].max() <= 1e-10
     for i, lo in enumerate(lam_o):
         j = int(np.argmin(np.abs(r.lam - lo)))
         d = abs(r.lam[j] - lo) / abs(lo)
@@ -89,9 +98,15 @@ def _eigen_case(ctx, name, precision):
         got = P[1:] / P[0]
         want = cz(g["vec_ratios"][i])
         dv = np.max(np.abs(got - want) / np.abs(want))
-        say("  lambda %s rel diff %.2e, vector ratios rel diff %.2e" % (lo, d, dv))
-        assert d <= 1e-10
-        assert dv <= 1e-7
+        # first-order bound of two backward-stable answers (module docstring)
+        bound_abs = 10 * g["kappa_abs"][i] * (res_all[j] + g["residual"][i]) * g["normA_fro"]
+        gap = np.min(np.abs(np.delete(lam_o, i) - lo))
+        tol_l = max(1e-10, bound_abs / abs(lo))
+        tol_v = max(1e-7, bound_abs / gap)
+        say("  lambda %s rel diff %.2e (tol %.1e, kappa %.1e), vector ratios rel diff %.2e (tol %.1e)" % (
+            lo, d, tol_l, g["kappa_abs"][i], dv, tol_v))
+        assert d <= tol_l
+        assert dv <= tol_v
     S.close()
     return r
 
@@ -142,7 +157,9 @@ def test_hydro_E1e7_N400_forced(ctx, path):
     want = cz(f["proj"])
     dp = np.max(np.abs(P - want) / np.abs(want))
     dn = abs(np.linalg.norm(x) - f["xnorm"]) / f["xnorm"]
-    say("  backward error %.2e, |x| rel diff %.2e, projections rel diff %.2e" % (berr, dn, dp))
+    tol = max(1e-8, 10 * f["cond1"] * (berr + f["backward_error"]))
+    say("  backward error %.2e, |x| rel diff %.2e, projections rel diff %.2e (tol %.1e, cond_1 %.1e)" % (
+        berr, dn, dp, tol, f["cond1"]))
     assert berr <= 1e-15
-    assert dn <= 1e-8 and dp <= 1e-8
+    assert dn <= tol and dp <= tol
     S.close()

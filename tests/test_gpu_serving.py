@@ -85,3 +85,26 @@ def test_device_residual_matches_host():
         assert abs(dev - host) <= 1e-10 * host, (kind, dev, host)
         S.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [backend.PREC_FP64, backend.PREC_C64])
+def test_gmres_backward_error_is_the_check(precision):
+    """The serving loop takes the backward error that kore_solve_gmres reports
+    as the check of a forced result: it agrees with the host formula
+    ||A x - b|| / (||A||_F ||x|| + ||b||) on the returned solution."""
+    r = next(q for q in small_trace() if q.kind == "forced")
+    print("case: gmres backward error as check, precision", precision, r.N, r.m, flush=True)
+    par, A0, B0 = serving.assemble_request(r)
+    ctx = backend.Context(0)
+    S = backend.System(ctx, backend.layout_from_params(par), A0, B0)
+    S.factor(r.sigma, precision)
+    b = serving.rhs_for(r, S.n)
+    x, info = S.solve_gmres(b, tol=1e-15, maxit=60)
+    host = serving.residual(A0, B0, r, x=x, b=b)
+    print("  iterations", info.iterations, "device", info.rel_correction, "host", host, flush=True)
+    # both are at the rounding level of a residual evaluation (~eps): they
+    # agree to eps, not relatively
+    assert info.converged and host <= 1e-15 and info.rel_correction <= 1e-15
+    assert abs(info.rel_correction - host) <= 2.3e-16
+    S.close()
+    ctx.close()
