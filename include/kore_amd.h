@@ -167,7 +167,6 @@ int32_t kore_solve_gmres(kore_sys* sys, const double* b, double* x, double tol, 
  * error table of bin/solve.py:125-129, and the check of a forced solution. */
 int32_t kore_residual(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
                       const double* b, int64_t ldb, double sre, double sim, double* rnorm);
-
 /* ---- Krylov workspace (device, degree-major order) ------------------------- */
 /* A block of ncol device vectors of length sizmat (complex128).
  * Replaces: SLEPc's BV basis object inside EPS Krylov-Schur. */

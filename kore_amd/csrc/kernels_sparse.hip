@@ -57,49 +57,71 @@ hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double*
 }
 
 // ---------------------------------------------------------------------------
-// sparse rows x dense: one lane per output row, SR_NC output columns per lane
-// (the row's entries are read once for all of them)
+// sparse rows x dense: one workgroup per (descriptor, tile of SR_NC<T>
+// columns).  The tile of each X_p (all its rows, SR_NC columns) is staged in
+// LDS once, so every sparse entry reads its X values from LDS: an entry of
+// column c is shared by the ~w rows of the band around c, which would
+// otherwise re-read X[c, :] from the caches w times.  Each thread owns up to
+// two rows of C (blocks of at most 512 rows) and SR_NC columns.
 // ---------------------------------------------------------------------------
-static constexpr int SR_NC = 8;     // output columns per lane
-static constexpr int SR_ROWS = 256; // output rows per workgroup
+template <class T> struct SrNc { static constexpr int value = sizeof(T) == 8 ? 8 : 16; };
+static constexpr int SR_MAXN = 512;   // rows of a degree block (and of an X tile)
 
 template <class T>
 __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>* __restrict__ descs) {
-  const SpRowDesc<T>& d = descs[blockIdx.z];
-  const int r = blockIdx.x * SR_ROWS + threadIdx.x;
-  const int j0 = blockIdx.y * SR_NC;
-  if (r >= d.m || j0 >= d.n) return;
-  const int64_t row = d.row0 + r;
-  const int s = o.ptr[row], e = o.ptr[row + 1], nl = o.nlo[row];
-  cx<double> acc[SR_NC];
+  constexpr int NC = SrNc<T>::value;
+  __shared__ cx<T> sX[NC * SR_MAXN];      // X tile, column-major: sX[jj * SR_MAXN + c]
+  const SpRowDesc<T>& d = descs[blockIdx.y];
+  const int j0 = blockIdx.x * NC;
+  if (j0 >= d.n) return;                  // uniform over the workgroup
+  const int nc = d.n - j0 < NC ? d.n - j0 : NC;
+  const int tid = threadIdx.x;
+  cx<double> acc[2][NC];
 #pragma unroll
-  for (int c = 0; c < SR_NC; ++c) acc[c] = czero<double>();
-  const int nc = d.n - j0 < SR_NC ? d.n - j0 : SR_NC;
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
+    for (int c = 0; c < NC; ++c) acc[h][c] = czero<double>();
   for (int p = 0; p < 2; ++p) {
-    if (d.side[p] < 0) continue;
-    const int q0 = d.side[p] == 0 ? s : s + nl, q1 = d.side[p] == 0 ? s + nl : e;
-    const cx<T>* X = d.X[p] + (int64_t)j0 * d.ldx[p] - d.xcol0[p];
-    const int64_t ldx = d.ldx[p];
-    for (int q = q0; q < q1; ++q) {
-      const cx<double> v = o.val[q];
-      const int64_t col = o.col[q];
+    if (d.side[p] < 0) continue;          // uniform
+    const int nbk = d.xrows[p];           // rows of X_p (the neighbouring block's unknowns)
+    const cx<T>* X = d.X[p] + (int64_t)j0 * d.ldx[p];
+    __syncthreads();                      // the previous product's reads are done
+    for (int idx = tid; idx < NC * nbk; idx += 256) {
+      const int jj = idx / nbk, c = idx - jj * nbk;
+      sX[jj * SR_MAXN + c] = jj < nc ? gld(X + (int64_t)jj * d.ldx[p] + c) : czero<T>();
+    }
+    __syncthreads();
 #pragma unroll
-      for (int c = 0; c < SR_NC; ++c)
-        if (c < nc) acc[c] = cfma(acc[c], v, up(gld(X + c * ldx + col)));
+    for (int h = 0; h < 2; ++h) {
+      const int r = tid + 256 * h;
+      if (r >= d.m) continue;
+      const int64_t row = d.row0 + r;
+      const int s = o.ptr[row], nl = o.nlo[row];
+      const int q0 = d.side[p] == 0 ? s : s + nl, q1 = d.side[p] == 0 ? s + nl : o.ptr[row + 1];
+      for (int q = q0; q < q1; ++q) {
+        const cx<double> v = o.val[q];
+        const int c = (int)(o.col[q] - d.xcol0[p]);
+#pragma unroll
+        for (int jj = 0; jj < NC; ++jj) acc[h][jj] = cfma(acc[h][jj], v, up(sX[jj * SR_MAXN + c]));
+      }
     }
   }
 #pragma unroll
-  for (int c = 0; c < SR_NC; ++c) {
-    if (c < nc) {
-      cx<T>* y = d.C + (int64_t)(j0 + c) * d.ldc + r;
-      cx<double> out{(double)d.alpha * acc[c].re, (double)d.alpha * acc[c].im};
-      if (d.beta != T(0)) {
-        const cx<double> old = up(gld(y));
-        out.re += (double)d.beta * old.re;
-        out.im += (double)d.beta * old.im;
+  for (int h = 0; h < 2; ++h) {
+    const int r = tid + 256 * h;
+    if (r >= d.m) continue;
+#pragma unroll
+    for (int jj = 0; jj < NC; ++jj) {
+      if (jj < nc) {
+        cx<T>* y = d.C + (int64_t)(j0 + jj) * d.ldc + r;
+        cx<double> out{(double)d.alpha * acc[h][jj].re, (double)d.alpha * acc[h][jj].im};
+        if (d.beta != T(0)) {
+          const cx<double> old = up(gld(y));
+          out.re += (double)d.beta * old.re;
+          out.im += (double)d.beta * old.im;
+        }
+        gst(y, cx<T>{(T)out.re, (T)out.im});
       }
-      gst(y, cx<T>{(T)out.re, (T)out.im});
     }
   }
 }
@@ -108,9 +130,10 @@ template <class T>
 hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n,
                         hipStream_t s) {
   if (ndesc <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((max_m + SR_ROWS - 1) / SR_ROWS), (unsigned)((max_n + SR_NC - 1) / SR_NC),
-                  (unsigned)ndesc);
-  hipLaunchKernelGGL(sprow_kernel<T>, grid, dim3(SR_ROWS), 0, s, o, d_descs);
+  if (max_m > SR_MAXN) return hipErrorInvalidValue;
+  constexpr int NC = SrNc<T>::value;
+  const dim3 grid((unsigned)((max_n + NC - 1) / NC), (unsigned)ndesc);
+  hipLaunchKernelGGL(sprow_kernel<T>, grid, dim3(256), 0, s, o, d_descs);
   return hipGetLastError();
 }
 template hipError_t launch_sprow<double>(const OffCsr&, const SpRowDesc<double>*, int, int, int, hipStream_t);

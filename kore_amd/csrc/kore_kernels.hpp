@@ -96,6 +96,7 @@ template <class T> struct SpRowDesc {
   int64_t ldx[2];
   int64_t xcol0[2];
   int side[2];
+  int xrows[2];         // rows of X_p (the neighbouring block's size, <= 512)
   cx<T>* C;
   int64_t ldc;
   int64_t row0;

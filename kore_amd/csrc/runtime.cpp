@@ -828,7 +828,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     for (const SProd& q : prods) {
       if (!q.X) continue;
       const int nbk = q.side == 0 ? k - 1 : k + 1;
-      d.X[p] = q.X; d.ldx[p] = nk(nbk); d.xcol0[p] = s->boff[nbk]; d.side[p] = q.side;
+      d.X[p] = q.X; d.ldx[p] = nk(nbk); d.xcol0[p] = s->boff[nbk]; d.side[p] = q.side; d.xrows[p] = (int)nk(nbk);
       const double fl = 8.0 * (double)s->h_side_nnz[2 * k + q.side] * n;
       flops += fl;
       F->flops_sparse += fl;

@@ -8,20 +8,24 @@ oracle Krylov-Schur, written by tests/golden/make_config_goldens.py) are
 stored as fixtures:
 
 * residuals ||(A - lambda B) x|| / (||A||_F ||x||) <= 1e-10 per pair;
-* eigenvalues: relative difference <= 1e-10 for every one of the nev
-  oracle values (the bar of bin/solve.py's FP64 path), except where the
-  eigenvalue's own conditioning puts two backward-stable answers further
-  apart: with kappa = ||x|| ||y|| / |y^H B x| (stored with the fixture) and
-  the backward errors eta of the two pairs, first-order perturbation theory
-  bounds |lambda_dev - lambda_oracle| by kappa (eta_dev + eta_oracle) ||A||_F;
-  the test allows ten times that.  These pencils are far from normal at low
-  Ekman number (kappa up to ~1e7), so a few of their eigenvalues are
-  determined only to ~1e-9 in FP64 by any solver;
+* eigenvalues: relative difference <= 1e-10 (the bar of bin/solve.py's
+  FP64 path) wherever FP64 determines the eigenvalue that well, i.e. where
+  its relative condition number kappa_rel = ||x|| ||y|| ||A||_F /
+  (|y^H B x| |lambda|) (x, y right and left eigenvectors; stored with the
+  fixture) times the unit roundoff is <= 1e-10.  These pencils are far from
+  normal at low Ekman number: kappa_rel is 1e13 - 1e15 for every wanted
+  eigenvalue of the three workloads, so a normwise backward error of 1e-17
+  (what both solvers reach) bounds the eigenvalue only to kappa_rel * 1e-17
+  (1e-3 and worse).  Two backward-stable FP64 answers (the oracle's and the
+  device's) are measured to agree far better than that bound, to 1e-12 -
+  3.2e-8, growing with kappa_rel (3.7e15: 9e-10; 9.1e15: 2.4e-8; 4.2e16:
+  3.2e-8).  Agreement to 1e-10 for these would need residuals in extended
+  precision on both sides; neither the reference (SLEPc + MUMPS in FP64) nor
+  this path uses them.  For those eigenvalues the bar is 1e-7;
 * eigenvectors, phase-invariantly: the ratios (w_k^H x) / (w_0^H x) for
-  seeded probe vectors w_k agree with the oracle's to 1e-7 relative, or to
-  ten times the eigenvalue bound divided by the gap to the nearest other
-  computed eigenvalue where that is larger (a wrong vector misses either by
-  orders of magnitude);
+  seeded probe vectors w_k agree with the oracle's to 1e-7 relative (1e-5
+  for the ill-conditioned eigenvalues above, measured up to 6e-6; a wrong
+  vector misses either by orders of magnitude);
 * the 200 000-unknown forced solve x = (A0 - 0.5i B0)^{-1} b (seeded
   unit-norm complex b): the backward error is <= 1e-15 on both paths, and
   ||x|| and the probe projections w_k^H x agree with SuperLU's to 1e-8, or
@@ -98,13 +102,12 @@ def _eigen_case(ctx, name, precision):
         got = P[1:] / P[0]
         want = cz(g["vec_ratios"][i])
         dv = np.max(np.abs(got - want) / np.abs(want))
-        # first-order bound of two backward-stable answers (module docstring)
-        bound_abs = 10 * g["kappa_abs"][i] * (res_all[j] + g["residual"][i]) * g["normA_fro"]
-        gap = np.min(np.abs(np.delete(lam_o, i) - lo))
-        tol_l = max(1e-10, bound_abs / abs(lo))
-        tol_v = max(1e-7, bound_abs / gap)
-        say("  lambda %s rel diff %.2e (tol %.1e, kappa %.1e), vector ratios rel diff %.2e (tol %.1e)" % (
-            lo, d, tol_l, g["kappa_abs"][i], dv, tol_v))
+        # two-tier bar by the eigenvalue's relative condition number (module docstring)
+        kappa_rel = g["kappa_abs"][i] * g["normA_fro"] / abs(lo)
+        ill = kappa_rel * 1.1e-16 > 1e-10
+        tol_l, tol_v = (1e-7, 1e-5) if ill else (1e-10, 1e-7)
+        say("  lambda %s rel diff %.2e (tol %.0e, kappa_rel %.1e), vector ratios rel diff %.2e (tol %.0e)" % (
+            lo, d, tol_l, kappa_rel, dv, tol_v))
         assert d <= tol_l
         assert dv <= tol_v
     S.close()
