@@ -167,6 +167,20 @@ int32_t kore_solve_gmres(kore_sys* sys, const double* b, double* x, double tol, 
  * error table of bin/solve.py:125-129, and the check of a forced solution. */
 int32_t kore_residual(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
                       const double* b, int64_t ldb, double sre, double sim, double* rnorm);
+/* The same residual norms, allocating nothing: the staging, gathered and
+ * residual vectors and the norms' partial sums live in a static 64 MiB device
+ * array of the library (each device holds its own copy until the process
+ * ends) instead of the system's buffers or newly allocated memory; no
+ * hipMalloc / hipFree or allocating library routine is called.  A call needs
+ * 3 * sizmat * 16 bytes plus 8 KiB of that array per column (columns are taken
+ * in as large chunks as fit); a larger system is rejected with KORE_ERR_ARG.
+ * Calls that overlap on one device share the array and race.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * device allocation). */
+int32_t kore_residual_noalloc(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
+                              const double* b, int64_t ldb, double sre, double sim, double* rnorm);
+
 /* ---- Krylov workspace (device, degree-major order) ------------------------- */
 /* A block of ncol device vectors of length sizmat (complex128).
  * Replaces: SLEPc's BV basis object inside EPS Krylov-Schur. */

@@ -84,6 +84,7 @@ SIGNATURES = [
     ("kore_factor_release", _I32, [_P]),
     ("kore_solve", _I32, [_P, _I32, _PD, _I64, _PD, _I64, _D, _I32, C.POINTER(SolveInfo)]),
     ("kore_residual", _I32, [_P, _I32, _PD, _I64, _PD, _PD, _I64, _D, _D, _PD]),
+    ("kore_residual_noalloc", _I32, [_P, _I32, _PD, _I64, _PD, _PD, _I64, _D, _D, _PD]),
     ("kore_solve_gmres", _I32, [_P, _PD, _PD, _D, _I32, _I32, C.POINTER(SolveInfo)]),
     ("kore_ws_create", _I32, [_P, _I32, C.POINTER(_P)]),
     ("kore_ws_destroy", None, [_P]),

@@ -247,6 +247,36 @@ struct PinnedBuf {
   }
 };
 
+// ---------------------------------------------------------------------------
+// allocation-free residuals: scratch in a static device array
+// ---------------------------------------------------------------------------
+// kore_residual_noalloc keeps its staging, gathered and residual vectors and
+// the partial sums of its norms in this file-scope array instead of device
+// memory of the system or the library's pool.  Each device holds its own
+// copy at its own address, found once per device by hipGetSymbolAddress (the
+// array's name in host code is only a stand-in for the symbol).  Every call on
+// a device shares that device's copy: calls that overlap on one device race.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+static void* g_scratch_ptr[POOL_MAXDEV];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `device` (which must be current).
+static int32_t scratch_address(int device, char** out) {
+  if (device < 0 || device >= POOL_MAXDEV) return fail(KORE_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(KORE_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                  hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = static_cast<char*>(g_scratch_ptr[device]);
+  return KORE_OK;
+}
+
 struct kore_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1493,6 +1523,8 @@ int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out
   c->device = device;
   c->budget = mem_budget_bytes;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  char* scr = nullptr;
+  TRY(scratch_address(device, &scr));     // once per device, with the device current
   *out = c.release();
   return KORE_OK;
 }
@@ -2107,6 +2139,64 @@ int32_t kore_residual(kore_sys* s, int32_t nrhs, const double* x, int64_t ldx, c
       HIP_TRY(launch_axpy<double>(s->rr, n, s->bb, n, n, nr, -1.0, c->stream));
     }
     TRY(colnorms2<double, double>(s, s->rr, n, nullptr, 0, nr, rnorm + c0, nullptr));
+  }
+  return KORE_OK;
+}
+
+int32_t kore_residual_noalloc(kore_sys* s, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
+                              const double* b, int64_t ldb, double sre, double sim, double* rnorm) {
+  if (!s || !x || !rnorm || nrhs < 0 || ldx < s->n || (b && ldb < s->n))
+    return fail(KORE_ERR_ARG, "bad residual arguments");
+  if (!s->has_b && (lam || sre != 0.0 || sim != 0.0)) return fail(KORE_ERR_STATE, "system has no B matrix");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int64_t n = s->n;
+  // layout: staging | gathered | residual (n x nr complex each), then the
+  // column partial sums of the norms
+  const int npart = s->npart;
+  const size_t part_bytes = (size_t)npart * MAXR * sizeof(double);
+  const size_t col_bytes = 3 * (size_t)n * sizeof(zd);
+  if (col_bytes + part_bytes > NOALLOC_SCRATCH_BYTES)
+    return fail(KORE_ERR_ARG,
+                "kore_residual_noalloc: %lld unknowns need %zu bytes of scratch per column, more than the %zu "
+                "of the static scratch array", (long long)n, col_bytes + part_bytes, NOALLOC_SCRATCH_BYTES);
+  char* base = nullptr;
+  TRY(scratch_address(c->device, &base));
+  const int cols = (int)std::min<size_t>(MAXR, (NOALLOC_SCRATCH_BYTES - part_bytes) / col_bytes);
+  double* part = reinterpret_cast<double*>(base);
+  cx<double>* stg = reinterpret_cast<cx<double>*>(base + ((part_bytes + 255) & ~size_t(255)));
+  cx<double>* gat = stg + (size_t)n * cols;
+  cx<double>* res = gat + (size_t)n * cols;
+  double h[64 * MAXR];                    // the partial sums on the host (npart <= 64)
+  if (npart * MAXR > (int)(sizeof(h) / sizeof(h[0]))) return fail(KORE_ERR_STATE, "internal: too many partials");
+  for (int c0 = 0; c0 < nrhs; c0 += cols) {
+    const int nr = std::min(cols, nrhs - c0);
+    for (int j = 0; j < nr; ++j)
+      HIP_TRY(hipMemcpyAsync(stg + (size_t)j * n, x + 2 * (ldx * (int64_t)(c0 + j)), n * sizeof(zd),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_gather_scale<double, double>(stg, n, gat, n, n, nr, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                                c->stream));
+    for (int j = 0; j < nr; ++j) {
+      const double lre = lam ? lam[2 * (c0 + j)] : 0.0, lim = lam ? lam[2 * (c0 + j) + 1] : 0.0;
+      HIP_TRY(launch_spmv(s->csr, 1.0, 0.0, -(sre + lre), -(sim + lim), gat + (size_t)j * n, n, res + (size_t)j * n,
+                          n, 1, 0.0, c->stream));
+    }
+    if (b) {
+      for (int j = 0; j < nr; ++j)
+        HIP_TRY(hipMemcpyAsync(stg + (size_t)j * n, b + 2 * (ldb * (int64_t)(c0 + j)), n * sizeof(zd),
+                               hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(launch_gather_scale<double, double>(stg, n, gat, n, n, nr, s->d_perm.as<int64_t>(), nullptr, 1.0,
+                                                  c->stream));
+      HIP_TRY(launch_axpy<double>(res, n, gat, n, n, nr, -1.0, c->stream));
+    }
+    HIP_TRY(launch_colnorm2<double>(res, n, n, nr, part, npart, c->stream));
+    HIP_TRY(hipMemcpyAsync(h, part, (size_t)npart * nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nr; ++j) {
+      double sa = 0;
+      for (int q = 0; q < npart; ++q) sa += h[(size_t)q * nr + j];
+      rnorm[c0 + j] = std::sqrt(sa);
+    }
   }
   return KORE_OK;
 }
