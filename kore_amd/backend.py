@@ -71,6 +71,7 @@ SIGNATURES = [
     ("kore_ctx_bytes_in_use", _I64, [_P]),
     ("kore_mem_trim", _I32, [_I32]),
     ("kore_mem_stats", _I32, [_I32, _PI64, _PI64]),
+    ("kore_ctx_set_graphs", _I32, [_P, _I32]),
     ("kore_ctx_set_timing", _I32, [_P, _I32]),
     ("kore_ctx_timing", _I32, [_P, _I32, _PI64, _PD]),
     ("kore_ctx_timing_reset", _I32, [_P]),
@@ -213,6 +214,10 @@ class Context:
 
     def bytes_in_use(self) -> int:
         return int(lib().kore_ctx_bytes_in_use(self._p))
+
+    def set_graphs(self, on: bool):
+        """Replay solve sweeps from captured HIP graphs (default) or launch them kernel by kernel."""
+        _check(lib().kore_ctx_set_graphs(self._p, 1 if on else 0))
 
     def set_timing(self, on: bool):
         _check(lib().kore_ctx_set_timing(self._p, 1 if on else 0))

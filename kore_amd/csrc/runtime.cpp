@@ -283,6 +283,7 @@ struct kore_ctx {
   int64_t budget = 0;
   std::atomic<int64_t> in_use{0};
   bool timing = false;
+  bool graphs = true;        // replay solve sweeps from captured HIP graphs
   std::vector<TimerRec> pending;
   std::vector<hipEvent_t> pool;
   double tot_ms[KORE_TIMER_COUNT] = {0};
@@ -561,6 +562,14 @@ template <class T> struct Level {
 };
 
 template <class T> struct Factor : FactorBase {
+  // solve sweeps captured as HIP graphs, one per column count (1..16): a
+  // sweep's launches read only device-resident descriptors and this factor's
+  // fixed vectors, so one capture serves every later sweep of the factor
+  hipGraphExec_t sweep_graph[17] = {};
+  ~Factor() override {
+    for (hipGraphExec_t g : sweep_graph)
+      if (g) hipGraphExecDestroy(g);
+  }
   DevBuf mem, aux, desc, vecs;
   std::vector<cx<T>*> D, L, U, G, H;   // G = Dinv L, H = Dinv U of eliminated blocks
   std::vector<Level<T>> lev;
@@ -1182,7 +1191,38 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
   return KORE_OK;
 }
 
+template <class T> static int32_t bcr_sweep_launches(kore_sys* s, Factor<T>* F, int nrhs);
+
+// One solve sweep of the factor on F->Bv -> F->Xv.  Without per-kernel
+// timing, the sweep for a column count is captured into a HIP graph the
+// first time and replayed afterwards (2 levels + 1 launches become one graph
+// launch); the sweep records no memset and no event, so the graph holds
+// kernel nodes only.
 template <class T> static int32_t bcr_solve(kore_sys* s, Factor<T>* F, int nrhs) {
+  kore_ctx* c = s->ctx;
+  if (!c->graphs || c->timing || nrhs < 1 || nrhs > MAXR) return bcr_sweep_launches<T>(s, F, nrhs);
+  hipStream_t st = c->stream;
+  if (!F->sweep_graph[nrhs]) {
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int32_t rc = bcr_sweep_launches<T>(s, F, nrhs);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc < 0) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess || !g) return fail(KORE_ERR_HIP, "sweep capture failed: %s", hipGetErrorString(ec));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(KORE_ERR_HIP, "sweep graph instantiation failed: %s", hipGetErrorString(ei));
+    F->sweep_graph[nrhs] = ex;
+  }
+  HIP_TRY(hipGraphLaunch(F->sweep_graph[nrhs], st));
+  return KORE_OK;
+}
+
+template <class T> static int32_t bcr_sweep_launches(kore_sys* s, Factor<T>* F, int nrhs) {
   kore_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int ltop = F->tail ? F->nlev - 1 : F->nlev;   // levels swept one by one
@@ -1571,6 +1611,13 @@ int32_t kore_ctx_set_timing(kore_ctx* ctx, int32_t enable) {
   if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
   resolve_timers(ctx);
   ctx->timing = enable != 0;
+  return KORE_OK;
+}
+
+int32_t kore_ctx_set_graphs(kore_ctx* ctx, int32_t enable) {
+  if (!ctx) return fail(KORE_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->graphs = enable != 0;
   return KORE_OK;
 }
 
