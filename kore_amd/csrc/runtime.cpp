@@ -1563,6 +1563,7 @@ int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out
   c->device = device;
   c->budget = mem_budget_bytes;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (const char* g = std::getenv("KORE_AMD_GRAPHS")) c->graphs = std::atoi(g) != 0;   // default on
   char* scr = nullptr;
   TRY(scratch_address(device, &scr));     // once per device, with the device current
   *out = c.release();
