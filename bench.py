@@ -225,11 +225,14 @@ def cpu_threads():
     return len(os.sched_getaffinity(0))
 
 
-def cpu_block_thomas(par, sigma, budget_s=60.0, forced=True, nops=None):
+def cpu_block_thomas(par, sigma, nops=None, max_blocks=None):
     """Stand-in (ii) of the CPU baseline: the oracle's block-Thomas LU
     (numpy/LAPACK zgetrf/zgetrs/zgemm through OpenBLAS, threads = cores) on the
     request's own matrices: factor and one solve timed; an eigensolve is
-    estimated as factor + nops solves."""
+    estimated as factor + nops solves.  With max_blocks, only the first
+    max_blocks degree blocks of the chain are factored and solved (a bounded
+    sample) and the times are scaled to all blocks: every step of the chain
+    costs the same (one LU, one solve and one product of blocks)."""
     from kore_amd import assembly, backend
     from oracle import blocktri, layout
     try:
@@ -244,17 +247,23 @@ def cpu_block_thomas(par, sigma, budget_s=60.0, forced=True, nops=None):
     M = layout.permute(A0 - sigma * B0, perm)
     D, L, U, off = layout.block_tridiag(M, sizes)
     t1 = time.perf_counter()
-    bt = blocktri.BlockThomas(D, L, U)
+    nb = len(sizes)
+    kb = nb if not max_blocks else min(nb, int(max_blocks))
+    bt = blocktri.BlockThomas(D[:kb], L[:kb], U[:kb])
     t2 = time.perf_counter()
     rng = np.random.default_rng(0)
-    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    x = bt.solve(b, off)
+    nk = int(off[kb])
+    b = rng.standard_normal(nk) + 1j * rng.standard_normal(nk)
+    x = bt.solve(b, off[:kb + 1])
     t3 = time.perf_counter()
-    res = float(np.linalg.norm(M @ x - b) / (np.sqrt(np.sum(np.abs(M.data) ** 2)) * np.linalg.norm(x) + np.linalg.norm(b)))
+    Mk = M[:nk, :nk]
+    res = float(np.linalg.norm(Mk @ x - b) / (np.sqrt(np.sum(np.abs(Mk.data) ** 2)) * np.linalg.norm(x)
+                                              + np.linalg.norm(b)))
     if lim is not None and hasattr(lim, "unregister"):
         lim.unregister()
-    out = dict(factor_s=t2 - t1, solve_s=t3 - t2, extract_s=t1 - t0, backward_error=res, n=int(n),
-               blocks=int(len(sizes)))
+    scale = nb / kb
+    out = dict(factor_s=(t2 - t1) * scale, solve_s=(t3 - t2) * scale, extract_s=t1 - t0, backward_error=res,
+               n=int(n), blocks=int(nb), sample_blocks=int(kb), sample_factor_s=t2 - t1, sample_solve_s=t3 - t2)
     out["forced_s"] = out["factor_s"] + out["solve_s"]
     if nops:
         out["eigen_s_est"] = out["factor_s"] + nops * out["solve_s"]
@@ -277,8 +286,22 @@ def classes_from(results):
     return ms, flops, prec
 
 
+def summarize_subset(results, first):
+    """Latency and throughput of the requests rid < first (a prefix of the trace)."""
+    sub = [r for r in results if r.rid < first]
+    lat = np.array([r.latency for r in sub])
+    return dict(requests=len(sub), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_service_s=float(np.mean([r.service for r in sub])), ok=int(sum(r.ok for r in sub)))
+
+
 def run_serving(args):
-    """Serving workload (module docstring)."""
+    """Serving workload (module docstring).  The trace of --requests requests
+    (1024, the serving configuration) is the timed region of the headline
+    (mixed) path; a "step" is one window of its arrivals, --steps windows of
+    about requests/steps requests each.  The FP64 path serves the first
+    --compare-requests requests of the same trace at the same arrival times
+    (both full traces would not fit a ten-minute run), and the mixed path's
+    figures over the same prefix are reported beside it."""
     import dataclasses
     from kore_amd import backend, serving
     e_lo, e_hi = (float(v) for v in args.e_range.split(","))
@@ -287,17 +310,18 @@ def run_serving(args):
         len(trace), min(r.N for r in trace), max(r.N for r in trace), sum(r.kind == "forced" for r in trace),
         sum(r.kind == "eigen" for r in trace)))
     at0 = lambda rs: [dataclasses.replace(r, arrival=0.0) for r in rs]
-    # warm-up (kernels loaded, memory pool filled): two requests per path
+    paths_req = args.paths.split(",")
+    # warm-up: W requests per path served untimed (kernels loaded, memory pool filled)
     for prec in (backend.PREC_FP64, backend.PREC_C64):
         srv = serving.Server(slots=args.slots, precision=prec, block=args.block)
-        srv.run(at0(trace[:2]))
+        srv.run(at0(trace[:max(1, args.warmup)]))
         srv.close()
     # capacity: a saturated pass (all requests queued at once) over the first
     # requests of the trace, per path
     cal = at0(trace[:min(args.calibrate, len(trace))])
     capacity = {}
     for name, prec in (("fp64", backend.PREC_FP64), ("mixed", backend.PREC_C64)):
-        if name not in args.paths.split(","):
+        if name not in paths_req:
             continue
         srv = serving.Server(slots=args.slots, precision=prec, block=args.block)
         t0 = time.perf_counter()
@@ -316,31 +340,43 @@ def run_serving(args):
         t += float(rng.exponential(1.0 / rate))
         timed.append(dataclasses.replace(r, arrival=t))
     log("arrivals: Poisson at %.2f /s (%.0f %% of FP64 capacity), trace spans %.1f s" % (rate, 100 * args.load, t))
-    paths = {}
-    for name in args.paths.split(","):
+    head_name = "mixed" if "mixed" in paths_req else paths_req[0]
+    order = [head_name] + [p for p in paths_req if p != head_name]
+    paths, outs = {}, {}
+    prefix = min(args.compare_requests, len(timed))
+    for name in order:
         prec = backend.PREC_C64 if name == "mixed" else backend.PREC_FP64
+        tr = timed if name == head_name else timed[:prefix]
         done = [0]
 
-        def progress(msg, _done=done, _name=name):
+        def progress(msg, _done=done, _name=name, _n=len(tr)):
             _done[0] += 1
             if _done[0] % 64 == 0:
-                log("  %s: %d / %d served" % (_name, _done[0], len(timed)))
+                log("  %s: %d / %d served" % (_name, _done[0], _n))
         srv = serving.Server(slots=args.slots, precision=prec, block=args.block, log=progress)
         t0 = time.perf_counter()
-        out = srv.run(timed)
+        out = srv.run(tr)
         wall = time.perf_counter() - t0
         summ = serving.summarize(out, wall)
         summ.update(max_inflight=srv.max_inflight, peak_predicted_bytes=srv.mem_peak,
                     failed=[r.rid for r in out if not r.ok][:10],
-                    capacity_solves_per_s=capacity.get(name, {}).get("solves_per_s"))
+                    capacity_solves_per_s=capacity.get(name, {}).get("solves_per_s"),
+                    trace_requests=len(tr))
         srv.close()
         paths[name] = summ
-        log("  %s: %.3f solves/s, p50 %.3f s, p99 %.3f s, ok %d/%d, worst residual %.2e, max in flight %d, "
-            "fallbacks %d" % (name, summ["solves_per_s"], summ["p50_s"], summ["p99_s"], summ["ok"], summ["requests"],
-                              summ["worst_residual"], srv.max_inflight, summ["fallbacks"]))
+        outs[name] = out
+        for r in out:
+            if not r.ok:
+                log("  FAILED request %d (%s): residual %.3e, precision %d, fallback %s, error %s" % (
+                    r.rid, r.kind, r.residual, r.precision, r.fallback, r.error or "-"))
+        log("  %s: %d requests, %.3f solves/s, p50 %.3f s, p99 %.3f s, ok %d/%d, worst residual %.2e, "
+            "max in flight %d, fallbacks %d" % (name, len(tr), summ["solves_per_s"], summ["p50_s"], summ["p99_s"],
+                                                summ["ok"], summ["requests"], summ["worst_residual"],
+                                                srv.max_inflight, summ["fallbacks"]))
+    # like for like: every path over the common prefix of the trace
+    common = {name: summarize_subset(outs[name], prefix) for name in outs}
     # roofline: an instrumented pass (HIP events around every launch) over a
     # sample of the trace on the headline path; the dominant class by device time
-    head_name = "mixed" if "mixed" in paths else next(iter(paths))
     hp = backend.PREC_C64 if head_name == "mixed" else backend.PREC_FP64
     sample = at0(trace[:args.roofline_sample])
     srv = serving.Server(slots=1, precision=hp, block=args.block, instrument=True)
@@ -363,39 +399,47 @@ def run_serving(args):
     rl.update(dominant_class=dom, class_device_ms={k: round(v, 3) for k, v in ms.items()},
               class_share={k: round(v / tot, 3) for k, v in ms.items()} if tot > 0 else {},
               sample="%d requests of the trace on the %s path, one slot" % (len(sample), head_name),
-              factor_precisions=sorted(precs))
+              factor_precisions=sorted(precs),
+              peak_note="FP64 matrix peak: the sample mixes complex64 (forced) and FP64 (eigen) factors"
+              if len(precs) > 1 else "")
     log("roofline: dominant %s, %.2f %s of %.1f (%.3f)" % (dom, rl["achieved"], rl["unit"], rl["peak"], rl["frac"]))
-    # CPU baseline: the first forced request's matrices through the oracle's
-    # block-Thomas LU on this host's cores (stand-in (ii); not MUMPS)
     cpu = None
     if not args.no_cpu_baseline:
         req = next(r for r in trace if r.kind == "forced")
         mean_ops = paths[head_name].get("mean_eigen_op_columns") or 60.0
         log("cpu baseline: request %d (N %d, m %d) ..." % (req.rid, req.N, req.m))
-        c = cpu_block_thomas(req.params(), req.sigma, nops=int(round(mean_ops)))
+        c = cpu_block_thomas(req.params(), req.sigma, nops=int(round(mean_ops)), max_blocks=args.cpu_blocks)
         nf = sum(r.kind == "forced" for r in trace)
         per_req = (nf * c["forced_s"] + (len(trace) - nf) * c["eigen_s_est"]) / len(trace)
         cpu = dict(value=1.0 / per_req, unit="solves/s", cores=cpu_threads(), kind="port",
                    sample="block-Thomas LU (oracle, OpenBLAS zgetrf/zgetrs/zgemm, %d threads) of request %d "
-                          "(N %d, %d unknowns): factor %.2f s, one solve %.3f s; mix of 70 %% forced (factor + "
-                          "solve) and 30 %% eigen (factor + %d solves, estimated)"
-                          % (cpu_threads(), req.rid, req.N, c["n"], c["factor_s"], c["solve_s"], int(round(mean_ops))),
+                          "(N %d, %d unknowns in %d degree blocks): the first %d blocks factored (%.2f s) and "
+                          "solved (%.3f s), scaled to all blocks (the chain costs the same per block): factor "
+                          "%.1f s, one solve %.2f s; the trace's 70 %% forced (factor + solve) and 30 %% eigen "
+                          "(factor + %d solves, estimated) averaged"
+                          % (cpu_threads(), req.rid, req.N, c["n"], c["blocks"], c["sample_blocks"],
+                             c["sample_factor_s"], c["sample_solve_s"], c["factor_s"], c["solve_s"],
+                             int(round(mean_ops))),
                    stand_in="oracle block-Thomas (not MUMPS)", detail=c)
         log("  cpu: %.4f solves/s (%s)" % (cpu["value"], cpu["sample"]))
     head = paths[head_name]
     ok = all(p["ok"] == p["requests"] for p in paths.values())
+    wall = head["wall_s"]
     out = {
         "metric": METRIC, "value": head["solves_per_s"] if ok else 0.0, "unit": "solves/s", "n_gpus": 1,
-        "steps": len(trace), "warmup": 2, "ms_per_step": 1e3 / max(head["solves_per_s"], 1e-12),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / max(1, args.steps),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 factor + f64 GMRES (forced), f64 (eigen)" if head_name == "mixed" else "f64",
         "data": "synthetic: seeded request trace, Kore hydrodynamic operators assembled in-process",
         "config": {"workload": "serving %d requests, E log-uniform in [%g, %g], N = min(ceil(8 E^-1/3), %d), "
                                "lmax = ceil(1.2 N), m in 0..4, 70%% forced / 30%% eigen nev 6, Poisson arrivals "
                                "at %.0f%% of FP64 capacity" % (len(trace), e_lo, e_hi, args.n_cap, 100 * args.load),
-                   "slots": args.slots, "headline_path": head_name, "arrival_rate_per_s": rate},
+                   "slots": args.slots, "headline_path": head_name, "arrival_rate_per_s": rate,
+                   "step": "one window of the arrival trace (%d requests / %d steps); warm-up steps are single "
+                           "requests per path" % (len(trace), args.steps),
+                   "fp64_path_requests": prefix},
         "p50_ms": 1e3 * head["p50_s"], "p99_ms": 1e3 * head["p99_s"], "residual_ok": ok,
-        "capacity": capacity, "paths": paths, "roofline": rl, "cpu_baseline": cpu,
+        "capacity": capacity, "paths": paths, "common_prefix": common, "roofline": rl, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -421,8 +465,8 @@ def dataclasses_replace(obj, **kw):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="serving", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--slots", type=int, default=2)
@@ -431,6 +475,9 @@ def main():
     ap.add_argument("--load", type=float, default=0.8, help="arrival rate as a fraction of the FP64 capacity")
     ap.add_argument("--calibrate", type=int, default=24, help="requests of the saturated capacity passes")
     ap.add_argument("--roofline-sample", type=int, default=8)
+    ap.add_argument("--compare-requests", type=int, default=256,
+                    help="requests of the trace (from its start) the FP64 path serves")
+    ap.add_argument("--cpu-blocks", type=int, default=96, help="degree blocks of the bounded CPU baseline sample")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--block", type=int, default=1)
     ap.add_argument("--ncv", type=int, default=None)

@@ -92,18 +92,16 @@ struct TimerRec {
 // factor, system and workspace buffers of each request (gigabytes at the
 // serving sizes), and hipMalloc / hipFree of such blocks cost milliseconds to
 // a second each, with hipFree waiting for the whole device, so that requests
-// on other streams stall.  A released block goes back to its pool with an
-// event recorded on the releasing stream; the next taker (any context) makes
-// its own stream wait for that event before it touches the block (a host
-// taker waits on the host), so reuse never races with work still queued.
-// Best fit: a request takes the smallest cached block that is at least as
-// large and at most twice as large (plus a little).  When the device runs
-// out of memory the cached blocks are returned to the runtime and the
+// on other streams stall.  A block goes back to its pool only after the
+// releasing stream has drained (objects are released after their stream is
+// synchronised, so this wait is normally free), so any context may take it
+// at once.  Best fit: a request takes the smallest cached block that is at
+// least as large and at most twice as large (plus a little).  When the device
+// runs out of memory the cached blocks are returned to the runtime and the
 // allocation is tried once more.
 struct PoolBlock {
   void* p;
   size_t bytes;
-  hipEvent_t ev;   // recorded on the releasing stream (may be null)
 };
 
 struct MemPool {
@@ -133,16 +131,9 @@ static bool pool_pick(MemPool& mp, size_t need, PoolBlock* out) {
 }
 
 static void pool_put(MemPool& mp, void* p, size_t bytes, hipStream_t st) {
-  hipEvent_t ev = nullptr;
-  if (st && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-    if (hipEventRecord(ev, st) != hipSuccess) {
-      hipEventDestroy(ev);
-      ev = nullptr;
-      hipStreamSynchronize(st);
-    }
-  }
+  if (st) hipStreamSynchronize(st);      // no work of the releasing stream may still use the block
   std::lock_guard<std::mutex> lk(mp.mu);
-  mp.blocks.emplace(bytes, PoolBlock{p, bytes, ev});
+  mp.blocks.emplace(bytes, PoolBlock{p, bytes});
   mp.cached += bytes;
 }
 
@@ -155,10 +146,6 @@ static void pool_trim(MemPool& mp, bool host) {
     mp.cached = 0;
   }
   for (auto& kv : all) {
-    if (kv.second.ev) {
-      hipEventSynchronize(kv.second.ev);
-      hipEventDestroy(kv.second.ev);
-    }
     if (host)
       hipHostFree(kv.second.p);
     else
@@ -171,10 +158,6 @@ static int32_t dev_take(int device, hipStream_t st, size_t need, void** p, size_
   MemPool& mp = g_dev_pool[device % POOL_MAXDEV];
   PoolBlock b{};
   if (pool_pick(mp, need, &b)) {
-    if (b.ev) {
-      hipStreamWaitEvent(st, b.ev, 0);
-      hipEventDestroy(b.ev);
-    }
     *p = b.p;
     *got = b.bytes;
     return KORE_OK;
@@ -196,14 +179,10 @@ static int32_t dev_take(int device, hipStream_t st, size_t need, void** p, size_
   return KORE_OK;
 }
 
-// Pinned host block of at least `need` bytes; waits for the work that last used it.
+// Pinned host block of at least `need` bytes.
 static int32_t host_take(size_t need, char** p, size_t* got) {
   PoolBlock b{};
   if (pool_pick(g_host_pool, need, &b)) {
-    if (b.ev) {
-      hipEventSynchronize(b.ev);
-      hipEventDestroy(b.ev);
-    }
     *p = static_cast<char*>(b.p);
     *got = b.bytes;
     return KORE_OK;
@@ -224,8 +203,8 @@ static int32_t host_take(size_t need, char** p, size_t* got) {
   return KORE_OK;
 }
 
-// A pinned host buffer from the pool, returned to it (behind an event on
-// `st`) when released.
+// A pinned host buffer from the pool, returned to it (after `st` drains)
+// when released.
 struct PinnedBuf {
   char* p = nullptr;
   size_t bytes = 0;
