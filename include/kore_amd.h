@@ -127,6 +127,15 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat,
                         const int32_t* a_indptr, const int32_t* a_indices, const double* a_data,
                         const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
                         int32_t b_is_complex, kore_sys** out);
+/* New values of A (a_data: interleaved complex, one per entry of the CSR
+ * given to kore_sys_create, in that order; same pattern), uploaded into the
+ * resident system; B, the layout and the device buffers stay.  The system
+ * must be factored again.  An entry that was a zero outside the three block
+ * diagonals at creation cannot take a nonzero value (KORE_ERR_UNSUPPORTED).
+ * Replaces: re-running assemble.py and re-creating the PETSc matrices for
+ * every Brent step of the critical-Rayleigh search
+ * (tests/jones2000/find_Rac.py:36-52), where only A's buoyancy values change. */
+int32_t kore_sys_update_a(kore_sys* sys, const double* a_data);
 /* Also destroys the system's workspaces that are still alive. */
 void kore_sys_destroy(kore_sys* sys);
 /* ||A - sigma B||_F of the stored (unscaled) matrices. */

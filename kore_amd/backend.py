@@ -80,6 +80,7 @@ SIGNATURES = [
     ("kore_sys_create", _I32, [_P, C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PI32, _PI32, _PD, _I32,
                                C.POINTER(_P)]),
     ("kore_sys_destroy", None, [_P]),
+    ("kore_sys_update_a", _I32, [_P, _PD]),
     ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
     ("kore_factor", _I32, [_P, _D, _D, _I32, C.POINTER(FactorInfo)]),
     ("kore_factor_release", _I32, [_P]),
@@ -273,6 +274,7 @@ class System:
                                  self._a[1].ctypes.data_as(_PI32), _dptr(self._a[2]),
                                  args_b[0], args_b[1], args_b[2], args_b[3], C.byref(p)))
         self._p = p
+        self.nnz_a = int(A.nnz)
         self._a = self._b = None           # the library keeps its own copies
         self._children = weakref.WeakSet()
         ctx._children.add(self)
@@ -283,6 +285,15 @@ class System:
     @property
     def handle(self):
         return self._p
+
+    def update_a(self, data):
+        """New values of A on the pattern given at creation (kore_sys_update_a);
+        the system must be factored again."""
+        data = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        if data.shape != (self.nnz_a,):
+            raise ValueError("expected %d values of A, got %s" % (self.nnz_a, data.shape))
+        _check(lib().kore_sys_update_a(self._p, _dptr(data)))
+        self.sigma = None
 
     def norm(self, sigma=0j) -> float:
         out = C.c_double(0)

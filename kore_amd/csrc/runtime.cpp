@@ -593,6 +593,7 @@ struct kore_sys {
   std::vector<int64_t> boff;
   std::vector<int64_t> perm;
   std::vector<int32_t> h_ip, h_blk;
+  std::vector<int32_t> h_amap;     // union-pattern position of each entry of the A given at creation (-1: dropped)
   int64_t nnz = 0;                 // entries of the union pattern (device arrays only)
   int64_t off_nnz[2] = {0, 0};     // entries outside the diagonal block, rows of even / odd blocks
   int64_t off_rows[2] = {0, 0};    // rows of even / odd blocks
@@ -1688,8 +1689,11 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   // (each range into its own arrays, then concatenated in row order)
   struct Ent {
     int32_t col;
+    int32_t src;   // index of the A entry it came from (-1: a B entry)
     zd a, b;
   };
+  const int64_t nnz_a = a_indptr[n];
+  s->h_amap.assign((size_t)nnz_a, -1);
   const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
                                                                std::max<int64_t>(1, n / 4096)}));
   // per-range upper bounds (entries of A plus B on the range's rows) place
@@ -1733,11 +1737,11 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
       const int64_t r = s->perm[i];
       row.clear();
       for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
-        row.push_back({(int32_t)inv[a_indices[k]], zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)});
+        row.push_back({(int32_t)inv[a_indices[k]], k, zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)});
       if (s->has_b)
         for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) {
           const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
-          row.push_back({(int32_t)inv[b_indices[k]], zd(0, 0), bv});
+          row.push_back({(int32_t)inv[b_indices[k]], -1, zd(0, 0), bv});
         }
       // rows hold a few dozen entries in a few sorted runs: a stable insertion
       // sort, no allocation per row
@@ -1755,15 +1759,21 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
       int32_t cnt = 0, cntb = 0;
       for (size_t q = 0; q < row.size();) {
         const int32_t col = row[q].col;
+        const size_t q0 = q;
         zd a = 0, b = 0;
         while (q < row.size() && row[q].col == col) {
           a += row[q].a;
           b += row[q].b;
           ++q;
         }
-        if (a == zd(0, 0) && b == zd(0, 0)) continue;
         const int kc = s->h_blk[col];
-        if ((kc < kb - 1 || kc > kb + 1) && bad_row[t] < 0) {
+        const bool inband = kc >= kb - 1 && kc <= kb + 1;
+        // explicit zeros inside the three block diagonals stay in the pattern
+        // (kore_sys_update_a may give them values); outside it they are dropped
+        if (a == zd(0, 0) && b == zd(0, 0) && !inband) continue;
+        for (size_t qq = q0; qq < q; ++qq)
+          if (row[qq].src >= 0) s->h_amap[row[qq].src] = (int32_t)cntt;   // range-local position, fixed below
+        if (!inband && bad_row[t] < 0) {
           bad_row[t] = i;
           bad_kb[t] = kb;
           bad_kc[t] = kc;
@@ -1799,6 +1809,23 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     for (auto& th : pool) th.join();
   }
   trace.mark("sys_create union pattern");
+  // positions of the A entries: range-local -> global
+  {
+    std::vector<int64_t> rbase(nth + 1, 0);
+    for (int t = 0; t < nth; ++t) rbase[t + 1] = rbase[t] + (int64_t)tcnt[t];
+    auto fix = [&](int t) {
+      const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
+      for (int64_t i = r0; i < r1; ++i) {
+        const int64_t r = s->perm[i];
+        for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
+          if (s->h_amap[k] >= 0) s->h_amap[k] = (int32_t)(s->h_amap[k] + rbase[t]);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(fix, t);
+    fix(0);
+    for (auto& th : pool) th.join();
+  }
   for (int t = 0; t < nth; ++t)
     if (bad_row[t] >= 0)
       return fail(KORE_ERR_UNSUPPORTED,
@@ -1968,6 +1995,32 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   TRY(s->up.init(ctx, 1 << 20));
   trace.mark("sys_create upload");
   *out = s.release();
+  return KORE_OK;
+}
+
+int32_t kore_sys_update_a(kore_sys* s, const double* a_data) {
+  if (!s || !a_data) return fail(KORE_ERR_ARG, "NULL argument");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  char* arena = nullptr;
+  TRY(ctx_pinned(c, (size_t)std::max<int64_t>(s->nnz, 1) * sizeof(zd), &arena));
+  zd* va = reinterpret_cast<zd*>(arena);
+  std::fill(va, va + s->nnz, zd(0, 0));
+  for (size_t k = 0; k < s->h_amap.size(); ++k) {
+    const zd v(a_data[2 * k], a_data[2 * k + 1]);
+    const int32_t pos = s->h_amap[k];
+    if (pos < 0) {
+      if (v != zd(0, 0))
+        return fail(KORE_ERR_UNSUPPORTED, "A entry %zu was a zero outside the block-tridiagonal pattern at "
+                    "creation and cannot take a value now", k);
+      continue;
+    }
+    va[pos] += v;
+  }
+  if (s->nnz > 0)
+    HIP_TRY(hipMemcpyAsync(s->d_va.p, va, (size_t)s->nnz * sizeof(zd), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  s->factored = false;            // the factorisation (if any) belongs to the old values
   return KORE_OK;
 }
 

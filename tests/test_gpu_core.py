@@ -526,3 +526,53 @@ def test_sweep_graph_replays_match_eager(ctx, hydro_small, precision):
             assert np.array_equal(x3, x3_ref), (sigma, rep)
     ws.close()
     S.close()
+
+
+def test_sys_update_a_matches_fresh_system(ctx):
+    """A system whose A values are replaced in place (kore_sys_update_a)
+    factors and solves exactly as a system created from the new values."""
+    import dataclasses
+    from conftest import params_dormy
+    from kore_amd import assembly, backend
+    from kore_amd.onset import _on_pattern
+    import scipy.sparse as ss
+    par = dataclasses.replace(params_dormy(1.6e6), N=24, lmax=32)
+    A1, B, _ = assembly.assemble(par)
+    A2, _, _ = assembly.assemble(dataclasses.replace(par, Ra_gap=2.4e6, Beyonce=None))
+    pat = (abs(A1) + abs(A2)).tocsr()
+    pat.sort_indices()
+    Ap1 = ss.csr_matrix((_on_pattern(pat, A1), pat.indices, pat.indptr), shape=pat.shape)
+    Ap2 = ss.csr_matrix((_on_pattern(pat, A2), pat.indices, pat.indptr), shape=pat.shape)
+    say("case: sys_update_a, sizmat", pat.shape[0])
+    S = _system(ctx, par, Ap1, B)
+    S.update_a(Ap2.data)
+    F = _system(ctx, par, Ap2, B)
+    b = np.random.default_rng(2).standard_normal(S.n) + 0j
+    for X in (S, F):
+        X.factor(0.3j, 0)
+    xs, _ = S.solve(b, ir_maxit=0)
+    xf, _ = F.solve(b, ir_maxit=0)
+    assert np.array_equal(xs, xf)
+    assert S.norm(0.3j) == F.norm(0.3j)
+    S.close()
+    F.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_onset_driver_resident_system(ctx, precision):
+    """The product onset driver (one resident system per m, A's buoyancy
+    values updated per Brent step, warm-started eigensolves) reproduces the
+    reference's Dormy et al. (2004) onset golden (tests/dormy2004/
+    reference.dormy04:1; printed values at rtol 1e-8) with two assemblies in
+    total instead of one per step."""
+    from conftest import DORMY_GOLDEN, params_dormy
+    from kore_amd import onset
+    say("case: onset driver, Dormy 2004, precision", precision)
+    ra, lam, solver = onset.critical_rayleigh(params_dormy(1.6e6), 1.6e6, ctx=ctx, precision=precision,
+                                              lo=1.6e6, hi=1.7e6)
+    say("  Ra_c %.5e omega_c %.5e, %d eigensolves, %d assemblies, %.2f s of eigensolves" % (
+        ra, lam.imag, len(solver.points), solver.assemblies, sum(p.seconds for p in solver.points)))
+    solver.close()
+    got = np.array([float("%.5e" % ra), float("%.5e" % lam.imag)])
+    np.testing.assert_allclose(got, [DORMY_GOLDEN[2], DORMY_GOLDEN[4]], rtol=1e-8, atol=1e-20)
+    assert solver.assemblies == 2
