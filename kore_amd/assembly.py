@@ -822,27 +822,9 @@ def assemble(par: Params, which: str = "eigen"):
 
 
 def forcing_vector(par: Params) -> ss.csr_matrix:
-    """B_forced column for the boundary-flow forcings (bin/assemble.py:280-330).
-
-    Implemented: forcing = 7 (longitudinal libration, no-slip, symm = 1).
-    """
-    if par.forcing != 7:
-        raise NotImplementedError("forcing %d is not implemented" % par.forcing)
-    if not (par.symm == 1 and par.bci == 1 and par.bco == 1 and par.m in (0, 2)):
-        raise ValueError("longitudinal libration requires m = 0 or 2, symm = 1 and no-slip boundaries")
-    top, bot, _ = ell(par.m, par.lmax, par.symm)
-    w = par.forcing_frequency
-    if par.m == 0:
-        pos = par.n + par.N1 * int(np.where(bot == 1)[0][0])
-        data = np.array([1j * w * par.forcing_amplitude_cmb / 2,
-                         1j * w * par.forcing_amplitude_icb * par.ricb / 2])
-    else:
-        L = 6.0
-        pos = par.N1 * int(np.where(top == 2)[0][0])
-        data = np.array([-w / L * par.forcing_amplitude_cmb,
-                         -w / L * par.forcing_amplitude_icb * par.ricb ** 2], dtype=complex)
-    rows = np.array([pos, pos + 1])
-    return ss.csr_matrix((data, (rows, np.zeros(2, dtype=int))), shape=(par.sizmat, 1))
+    """B_forced column of the forced problem (kore_amd.forcing.forcing_vector)."""
+    from .forcing import forcing_vector as fv
+    return fv(par)
 
 
 def save_npz_csr(path: str, M) -> None:

@@ -45,6 +45,7 @@ class Params:
     forcing_frequency: float = 1.0
     forcing_amplitude_cmb: float = 1.0
     forcing_amplitude_icb: float = 0.0
+    forcing_amplitude: Optional[float] = None    # body / tide forcings 2, 3, 6 (not in bin/parameters.py)
     # magnetic field (bin/parameters.py:103-141)
     magnetic: int = 0
     B0: str = "axial"
