@@ -97,9 +97,10 @@ int64_t kore_ctx_bytes_in_use(const kore_ctx* ctx);
  * MatDestroy / PetscFinalize (bin/solve.py:314-319). */
 int32_t kore_mem_trim(int32_t device);
 int32_t kore_mem_stats(int32_t device, int64_t* cached_bytes, int64_t* device_allocs);
-/* Solve sweeps are captured as HIP graphs at the first sweep of each
- * factorisation and column count, and replayed afterwards (default on);
- * enable = 0 launches every sweep kernel by kernel.  Per-kernel timing
+/* enable = 1: solve sweeps are captured as HIP graphs at the first sweep of
+ * each factorisation and column count, and replayed afterwards.  Default
+ * off (or KORE_AMD_GRAPHS=1 at context creation): kernel-by-kernel launches
+ * measured faster on MI355X.  Per-kernel timing
  * (below) always launches kernel by kernel.  No counterpart in the
  * reference (MUMPS solves inside KSPSolve / STApply, bin/solve.py:123, 227). */
 int32_t kore_ctx_set_graphs(kore_ctx* ctx, int32_t enable);

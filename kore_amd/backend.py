@@ -217,7 +217,7 @@ class Context:
         return int(lib().kore_ctx_bytes_in_use(self._p))
 
     def set_graphs(self, on: bool):
-        """Replay solve sweeps from captured HIP graphs (default) or launch them kernel by kernel."""
+        """Replay solve sweeps from captured HIP graphs, or launch them kernel by kernel (default)."""
         _check(lib().kore_ctx_set_graphs(self._p, 1 if on else 0))
 
     def set_timing(self, on: bool):

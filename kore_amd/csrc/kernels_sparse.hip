@@ -64,12 +64,10 @@ hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double*
 // otherwise re-read X[c, :] from the caches w times.  Each thread owns up to
 // two rows of C (blocks of at most 512 rows) and SR_NC columns.
 // ---------------------------------------------------------------------------
-template <class T> struct SrNc { static constexpr int value = sizeof(T) == 8 ? 8 : 16; };
 static constexpr int SR_MAXN = 512;   // rows of a degree block (and of an X tile)
 
-template <class T>
+template <class T, int NC>
 __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>* __restrict__ descs) {
-  constexpr int NC = SrNc<T>::value;
   __shared__ cx<T> sX[NC * SR_MAXN];      // X tile, column-major: sX[jj * SR_MAXN + c]
   const SpRowDesc<T>& d = descs[blockIdx.y];
   const int j0 = blockIdx.x * NC;
@@ -126,18 +124,27 @@ __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>
   }
 }
 
+// nc: columns per workgroup tile, 4 or 8 (complex128; 64 KiB of LDS at 8)
+// or 4, 8 or 16 (complex64); chosen per block-size class by the library's
+// tuner.
 template <class T>
-hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n,
+hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n, int nc,
                         hipStream_t s) {
   if (ndesc <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
   if (max_m > SR_MAXN) return hipErrorInvalidValue;
-  constexpr int NC = SrNc<T>::value;
-  const dim3 grid((unsigned)((max_n + NC - 1) / NC), (unsigned)ndesc);
-  hipLaunchKernelGGL(sprow_kernel<T>, grid, dim3(256), 0, s, o, d_descs);
+  const dim3 grid((unsigned)((max_n + nc - 1) / nc), (unsigned)ndesc);
+  if (nc == 4)
+    hipLaunchKernelGGL((sprow_kernel<T, 4>), grid, dim3(256), 0, s, o, d_descs);
+  else if (nc == 8)
+    hipLaunchKernelGGL((sprow_kernel<T, 8>), grid, dim3(256), 0, s, o, d_descs);
+  else if (nc == 16 && sizeof(T) == 4)
+    hipLaunchKernelGGL((sprow_kernel<T, sizeof(T) == 4 ? 16 : 8>), grid, dim3(256), 0, s, o, d_descs);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
-template hipError_t launch_sprow<double>(const OffCsr&, const SpRowDesc<double>*, int, int, int, hipStream_t);
-template hipError_t launch_sprow<float>(const OffCsr&, const SpRowDesc<float>*, int, int, int, hipStream_t);
+template hipError_t launch_sprow<double>(const OffCsr&, const SpRowDesc<double>*, int, int, int, int, hipStream_t);
+template hipError_t launch_sprow<float>(const OffCsr&, const SpRowDesc<float>*, int, int, int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // dense x sparse columns: one wave per output column, lanes over the rows;

@@ -116,8 +116,9 @@ template <class T> struct SpColDesc {
 };
 hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double* scale, const int32_t* split,
                           const OffCsr& o, hipStream_t s);
+// nc: columns per workgroup tile (4, 8; 16 for complex64)
 template <class T>
-hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n,
+hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n, int nc,
                         hipStream_t s);
 template <class T>
 hipError_t launch_spcol(const OffCsc& cc, const cx<double>* oval, const SpColDesc<T>* d_descs, int ndesc, int max_n,

@@ -262,7 +262,10 @@ struct kore_ctx {
   int64_t budget = 0;
   std::atomic<int64_t> in_use{0};
   bool timing = false;
-  bool graphs = true;        // replay solve sweeps from captured HIP graphs
+  // replay solve sweeps from captured HIP graphs: off by default, since the
+  // replay of one sweep measured slower than its eager launches on MI355X
+  // (43.2 against 39.7 ms median per forced request at N = 500)
+  bool graphs = false;
   std::vector<TimerRec> pending;
   std::vector<hipEvent_t> pool;
   double tot_ms[KORE_TIMER_COUNT] = {0};
@@ -545,6 +548,8 @@ template <class T> struct Factor : FactorBase {
   // sweep's launches read only device-resident descriptors and this factor's
   // fixed vectors, so one capture serves every later sweep of the factor
   hipGraphExec_t sweep_graph[17] = {};
+  int sprow_nc = 8;                     // column tile of the level-0 sparse-row products (tuned)
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};   // set while the tuner times the products
   ~Factor() override {
     for (hipGraphExec_t g : sweep_graph)
       if (g) hipGraphExecDestroy(g);
@@ -1147,7 +1152,9 @@ template <class T> static int32_t factor_run(kore_sys* s, Factor<T>* F) {
     if (l == 0 && F->lev0_sparse) {
       Timed tm(c, KORE_TIMER_GEMM);
       HIP_TRY(launch_spcol<T>(s->offc, s->off.val, L.sc, L.nsc, L.maxn_sc, st));
-      HIP_TRY(launch_sprow<T>(s->off, L.sr, L.nsr, L.maxm_sr, L.maxn_sr, st));
+      if (F->tune_ev[0]) HIP_TRY(hipEventRecord(F->tune_ev[0], st));
+      HIP_TRY(launch_sprow<T>(s->off, L.sr, L.nsr, L.maxm_sr, L.maxn_sr, F->sprow_nc, st));
+      if (F->tune_ev[1]) HIP_TRY(hipEventRecord(F->tune_ev[1], st));
       continue;
     }
     {
@@ -1525,6 +1532,109 @@ static int32_t gmres_dev(kore_sys* s, const cx<double>* b, cx<double>* x, double
 }
 
 // ---------------------------------------------------------------------------
+// tuner: tile shapes chosen by measurement
+// ---------------------------------------------------------------------------
+// The level-0 sparse-row products take one of a few column tiles per
+// workgroup: a wider tile reads each row's entries fewer times but needs
+// more LDS and registers per workgroup.  Which is fastest depends on the
+// block size and the precision, so the first factorisation of each
+// (precision, block-size class of 64) runs the whole factorisation once per
+// candidate tile with the products timed by HIP events and keeps the
+// fastest.  The choice is cached for the process and, with
+// KORE_AMD_TUNE_FILE set, read from and appended to that file (lines
+// "sprow <precision> <class> <tile>") so that later processes skip the
+// measurement.  KORE_AMD_SPROW_NC forces a tile (no measurement).
+struct Tuner {
+  std::mutex mu;
+  std::map<std::pair<int, int>, int> sprow;
+  bool loaded = false;
+};
+static Tuner g_tuner;
+
+static std::vector<int> sprow_tiles(int prec) {
+  return prec == KORE_PREC_C64 ? std::vector<int>{4, 8, 16} : std::vector<int>{4, 8};
+}
+
+static int tuner_lookup(int prec, int nmax) {
+  std::lock_guard<std::mutex> lk(g_tuner.mu);
+  if (!g_tuner.loaded) {
+    g_tuner.loaded = true;
+    if (const char* path = std::getenv("KORE_AMD_TUNE_FILE")) {
+      if (FILE* f = fopen(path, "r")) {
+        char what[32];
+        int pr, cl, nc;
+        while (fscanf(f, "%31s %d %d %d", what, &pr, &cl, &nc) == 4)
+          if (!strcmp(what, "sprow")) g_tuner.sprow[{pr, cl}] = nc;
+        fclose(f);
+      }
+    }
+  }
+  auto it = g_tuner.sprow.find({prec, (nmax + 63) / 64});
+  return it == g_tuner.sprow.end() ? -1 : it->second;
+}
+
+static void tuner_store(int prec, int nmax, int nc) {
+  std::lock_guard<std::mutex> lk(g_tuner.mu);
+  const int cl = (nmax + 63) / 64;
+  g_tuner.sprow[{prec, cl}] = nc;
+  if (const char* path = std::getenv("KORE_AMD_TUNE_FILE")) {
+    if (FILE* f = fopen(path, "a")) {
+      fprintf(f, "sprow %d %d %d\n", prec, cl, nc);
+      fclose(f);
+    }
+  }
+}
+
+// Chooses the factor's level-0 product tile: forced, cached, or measured
+// (one full factorisation per candidate; the caller's factorisation follows).
+template <class T> static int32_t tune_factor(kore_sys* s, Factor<T>* F, int prec) {
+  if (!F->lev0_sparse) return KORE_OK;
+  const std::vector<int> cands = sprow_tiles(prec);
+  if (const char* e = std::getenv("KORE_AMD_SPROW_NC")) {
+    const int nc = std::atoi(e);
+    if (std::find(cands.begin(), cands.end(), nc) == cands.end())
+      return fail(KORE_ERR_ARG, "KORE_AMD_SPROW_NC=%d is not a tile of this precision", nc);
+    F->sprow_nc = nc;
+    return KORE_OK;
+  }
+  const int known = tuner_lookup(prec, s->nmax);
+  if (known > 0) {
+    F->sprow_nc = known;
+    return KORE_OK;
+  }
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipEventCreate(&F->tune_ev[0]));
+  HIP_TRY(hipEventCreate(&F->tune_ev[1]));
+  float best = 1e30f;
+  int best_nc = cands[0];
+  int32_t rc = KORE_OK;
+  for (int nc : cands) {
+    F->sprow_nc = nc;
+    rc = factor_run<T>(s, F);
+    if (rc < 0) break;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = fail(KORE_ERR_HIP, "factorisation failed while tuning");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, F->tune_ev[0], F->tune_ev[1]);
+    if (getenv("KORE_AMD_TRACE")) fprintf(stderr, "kore_amd: tune sprow prec %d nmax %d tile %d: %.3f ms\n", prec,
+                                          s->nmax, nc, ms);
+    if (ms < best) {
+      best = ms;
+      best_nc = nc;
+    }
+  }
+  hipEventDestroy(F->tune_ev[0]);
+  hipEventDestroy(F->tune_ev[1]);
+  F->tune_ev[0] = F->tune_ev[1] = nullptr;
+  if (rc < 0) return rc;
+  F->sprow_nc = best_nc;
+  tuner_store(prec, s->nmax, best_nc);
+  return KORE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1543,7 +1653,7 @@ int32_t kore_ctx_create(int32_t device, int64_t mem_budget_bytes, kore_ctx** out
   c->device = device;
   c->budget = mem_budget_bytes;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  if (const char* g = std::getenv("KORE_AMD_GRAPHS")) c->graphs = std::atoi(g) != 0;   // default on
+  if (const char* g = std::getenv("KORE_AMD_GRAPHS")) c->graphs = std::atoi(g) != 0;   // default off
   char* scr = nullptr;
   TRY(scratch_address(device, &scr));     // once per device, with the device current
   *out = c.release();
@@ -2089,8 +2199,9 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     }
     if (rc >= 0) {
       Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
+      rc = tune_factor<double>(s, F, precision);
       hipEventRecord(e0, c->stream);
-      rc = factor_run<double>(s, F);
+      if (rc >= 0) rc = factor_run<double>(s, F);
       hipEventRecord(e1, c->stream);
       dinfo = F->info;
     }
@@ -2103,8 +2214,9 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     }
     if (rc >= 0) {
       Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
+      rc = tune_factor<float>(s, F, precision);
       hipEventRecord(e0, c->stream);
-      rc = factor_run<float>(s, F);
+      if (rc >= 0) rc = factor_run<float>(s, F);
       hipEventRecord(e1, c->stream);
       dinfo = F->info;
     }

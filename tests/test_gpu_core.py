@@ -524,8 +524,35 @@ def test_sweep_graph_replays_match_eager(ctx, hydro_small, precision):
             x3, _ = S.solve(b3, ir_maxit=0 if precision == 0 else 3)
             assert np.array_equal(x1, x1_ref), (sigma, rep)
             assert np.array_equal(x3, x3_ref), (sigma, rep)
+    ctx.set_graphs(False)
     ws.close()
     S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sprow_tiles_bit_identical(ctx, hydro_small, precision, monkeypatch):
+    """Every column tile the tuner may choose for the level-0 sparse-row
+    products (4 or 8 columns for complex128, 4, 8 or 16 for complex64) gives
+    the same factor bit for bit: each output entry sums the row's entries in
+    the same order whatever the tile.  Forced tiles (KORE_AMD_SPROW_NC) and
+    the measured choice (no variable) are compared by their solves."""
+    par, A, B = hydro_small
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal((A.shape[0], 2)) + 1j * rng.standard_normal((A.shape[0], 2))
+    tiles = [4, 8] if precision == 0 else [4, 8, 16]
+    out = {}
+    for nc in tiles + [None]:
+        say("case: sprow tile", nc, "precision", precision)
+        if nc is None:
+            monkeypatch.delenv("KORE_AMD_SPROW_NC", raising=False)
+        else:
+            monkeypatch.setenv("KORE_AMD_SPROW_NC", str(nc))
+        S = _system(ctx, par, A, B)
+        S.factor(0.37 + 0.9j, precision)
+        out[nc], _ = S.solve(b, ir_maxit=0)
+        S.close()
+    for nc in tiles[1:] + [None]:
+        assert np.array_equal(out[nc], out[tiles[0]]), nc
 
 
 def test_sys_update_a_matches_fresh_system(ctx):
