@@ -24,6 +24,7 @@ extern "C" {
 #define KORE_OK 0
 #define KORE_WARN_IR_NOT_CONVERGED 1
 #define KORE_WARN_BREAKDOWN 2        /* Krylov block is numerically rank deficient */
+#define KORE_WARN_NOT_CONVERGED 3    /* fewer than nev eigenpairs converged within maxit restarts */
 #define KORE_ERR_ARG (-1)
 #define KORE_ERR_HIP (-2)
 #define KORE_ERR_OOM (-3)
@@ -41,6 +42,20 @@ extern "C" {
 #define KORE_TIMER_ORTHO 4    /* Krylov workspace dot / gemm                  */
 #define KORE_TIMER_OTHER 5    /* block scatter, gathers, conversions          */
 #define KORE_TIMER_COUNT 6
+
+/* Which eigenvalues are wanted, in the order of bin/solve.py:99-117
+ * (which_eigenpairs of bin/parameters.py): nearest the target in modulus,
+ * real part or imaginary part; largest / smallest modulus, real part,
+ * imaginary part. */
+#define KORE_WHICH_TM 0
+#define KORE_WHICH_TR 1
+#define KORE_WHICH_TI 2
+#define KORE_WHICH_LM 3
+#define KORE_WHICH_SM 4
+#define KORE_WHICH_LR 5
+#define KORE_WHICH_SR 6
+#define KORE_WHICH_LI 7
+#define KORE_WHICH_SI 8
 
 typedef struct kore_ctx kore_ctx;
 typedef struct kore_sys kore_sys;
@@ -74,6 +89,52 @@ typedef struct {
   int64_t bytes;           /* device bytes held by the factor */
   double ms;               /* device time, HIP events on the library stream */
 } kore_factor_info;
+
+/* State of a system (kore_sys_query). */
+typedef struct {
+  int64_t sizmat;          /* unknowns */
+  int32_t nblk;            /* degree blocks */
+  int32_t has_b;           /* 1 if B was given */
+  int32_t factored;        /* 1 if a factorisation of A - sigma B is held */
+  int32_t precision;       /* its precision (KORE_PREC_*), -1 if none */
+  double sre, sim;         /* its shift sigma */
+} kore_sys_state;
+
+/* Options of kore_eigs; kore_eigs_default_opts fills the defaults given here. */
+typedef struct {
+  int32_t nev;             /* wanted eigenpairs (-eps_nev; default 6) */
+  int32_t ncv;             /* basis columns; <= 0: max(2 nev, nev + 15, 96) (-eps_ncv) */
+  double tol;              /* relative tolerance on ||OP y - theta y|| / |theta| (-eps_tol; 1e-12) */
+  int32_t maxit;           /* restarts (-eps_max_it; 100) */
+  int32_t which;           /* KORE_WHICH_* (default TM) */
+  int32_t block;           /* Krylov block size 1..16 (default 1) */
+  int32_t precision;       /* factor precision (default FP64); with KORE_PREC_C64 every OP application is
+                              refined in FP64, and the system is refactored in FP64 (and the iteration
+                              restarted) as soon as one application needs more than fallback_sweeps */
+  int32_t fallback_sweeps; /* default 4 */
+  double ir_tol;           /* refinement tolerance of complex64 OP applications (1e-13) */
+  int32_t ir_maxit;        /* refinement sweeps at most (12) */
+  int32_t check_every;     /* early convergence checks while the basis grows: < 0 adaptive (default),
+                              0 none, > 0 fixed spacing in columns */
+  uint64_t seed;           /* start block (device generator) when no v0 is given */
+} kore_eigs_opts;
+
+typedef struct {
+  int32_t nconv;           /* converged pairs (may exceed nev) */
+  int32_t returned;        /* pairs written: min(nconv, max_pairs) */
+  int32_t restarts;
+  int32_t op_columns;      /* OP applications (vectors) of the run that produced the pairs */
+  int32_t op_calls;        /* block applications */
+  int32_t ir_max_iterations;
+  int32_t precision;       /* precision of the factor the pairs were computed with */
+  int32_t fallback;        /* 1 if a complex64 factor was replaced by an FP64 one */
+  int32_t breakdowns;      /* invariant subspaces met (replaced by random blocks) */
+  int32_t c64_op_columns;  /* OP applications made with the complex64 factor */
+  int32_t c64_ir_max;      /* most refinement sweeps of one complex64 application */
+  int32_t basis_columns;   /* ncv used */
+  double seconds;          /* wall time of the call */
+  double host_seconds;     /* of which serial host work on the projected matrix */
+} kore_eigs_info;
 
 /* ---- library / context ---------------------------------------------------- */
 int32_t kore_version(void);
@@ -139,6 +200,8 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat,
 int32_t kore_sys_update_a(kore_sys* sys, const double* a_data);
 /* Also destroys the system's workspaces that are still alive. */
 void kore_sys_destroy(kore_sys* sys);
+/* Size and factorisation state of a system. */
+int32_t kore_sys_query(const kore_sys* sys, kore_sys_state* out);
 /* ||A - sigma B||_F of the stored (unscaled) matrices. */
 int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
 
@@ -196,6 +259,53 @@ int32_t kore_residual(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx,
  * device allocation). */
 int32_t kore_residual_noalloc(kore_sys* sys, int32_t nrhs, const double* x, int64_t ldx, const double* lam,
                               const double* b, int64_t ldb, double sre, double sim, double* rnorm);
+
+/* ---- stream-ordered (asynchronous) variants ------------------------------- */
+/* For a serving loop that keeps its vectors in device memory and orders its
+ * work on its own HIP streams.  stream is a hipStream_t (NULL: the legacy
+ * default stream).  The work runs on the context's stream, ordered after
+ * everything queued on `stream` before the call, and `stream` waits for it
+ * (HIP events both ways): the call returns without waiting for the device.
+ * Calls on one system must not overlap with calls on it from other threads.
+ * kore_factor_async: as kore_factor; the first factorisation of a system in
+ * a precision (plan set-up, and the tile tuner's measurement for a new
+ * block-size class) blocks until it is built.  Zero pivots are reported by
+ * kore_factor_check, which waits for the context's stream.
+ * kore_solve_async: d_b, d_x are device arrays in Kore order (column-major,
+ * ldb, ldx >= sizmat); nrhs columns; after the factor solve, ir_sweeps FP64
+ * refinement sweeps with no convergence test (use 2-4 with a complex64 factor,
+ * 0 with an FP64 one).
+ * Replaces: the numeric factorisation and KSPSolve of bin/solve.py:220-227 as
+ * queued work of a serving loop (the reference runs one blocking solve per
+ * process). */
+int32_t kore_factor_async(kore_sys* sys, double sre, double sim, int32_t precision, void* stream);
+int32_t kore_factor_check(kore_sys* sys);
+int32_t kore_solve_async(kore_sys* sys, int32_t nrhs, const double* d_b, int64_t ldb, double* d_x, int64_t ldx,
+                         int32_t ir_sweeps, void* stream);
+
+/* ---- eigensolver ------------------------------------------------------------- */
+/* The nev eigenpairs of A x = lambda B x nearest the target tau (in the sense
+ * of opts->which) by shift-and-invert block Krylov-Schur on the device.  The
+ * system is factored at tau (opts->precision) unless it already holds that
+ * factorisation.  Writes min(nconv, max_pairs) pairs in wanted order: lam
+ * (interleaved complex), X (sizmat x max_pairs complex, column-major, ldx,
+ * Kore order, unit 2-norm columns; may be NULL), errest (relative error
+ * estimates; may be NULL).  v0: optional start vector (sizmat complex, Kore
+ * order).  Returns KORE_WARN_NOT_CONVERGED if fewer than nev converged.
+ * Replaces: E.solve() and the extraction loop of bin/solve.py:91-158 (EPS
+ * GNHEP, Krylov-Schur, ST sinvert, EPSGetEigenpair). */
+void kore_eigs_default_opts(kore_eigs_opts* opts);
+int32_t kore_eigs(kore_sys* sys, double tau_re, double tau_im, const kore_eigs_opts* opts, const double* v0,
+                  int32_t max_pairs, double* lam, double* X, int64_t ldx, double* errest, kore_eigs_info* info);
+/* Host only: the projected-problem step of Krylov-Schur (SLEPc's DS object,
+ * DSSolve / DSSort inside E.solve(), bin/solve.py:123).  kore_schur: Schur
+ * form of the complex n x n matrix T in place (upper triangular) with its
+ * unitary Schur vectors in Q, T_in = Q T Q^H.  kore_schur_reorder: moves the
+ * diagonal entries with select[k] != 0 to the top of (S, Q), keeping their
+ * relative order; *m = number moved. */
+int32_t kore_schur(int32_t n, double* T, int64_t ldt, double* Q, int64_t ldq);
+int32_t kore_schur_reorder(int32_t n, double* S, int64_t lds, double* Q, int64_t ldq, const int32_t* select,
+                           int32_t* m);
 
 /* ---- Krylov workspace (device, degree-major order) ------------------------- */
 /* A block of ncol device vectors of length sizmat (complex128).

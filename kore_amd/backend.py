@@ -21,6 +21,8 @@ LIBPATH = os.environ.get("KORE_AMD_LIB", os.path.join(HERE, "libkore_amd.so"))
 KORE_OK = 0
 KORE_WARN_IR_NOT_CONVERGED = 1
 KORE_WARN_BREAKDOWN = 2
+KORE_WARN_NOT_CONVERGED = 3
+WHICH = {w: i for i, w in enumerate(["TM", "TR", "TI", "LM", "SM", "LR", "SR", "LI", "SI"])}
 PREC_FP64 = 0
 PREC_C64 = 1
 TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER = range(6)
@@ -53,6 +55,25 @@ class FactorInfo(C.Structure):
                 ("solve_launches", C.c_int32), ("bytes", C.c_int64), ("ms", C.c_double)]
 
 
+class SysState(C.Structure):
+    _fields_ = [("sizmat", C.c_int64), ("nblk", C.c_int32), ("has_b", C.c_int32), ("factored", C.c_int32),
+                ("precision", C.c_int32), ("sre", C.c_double), ("sim", C.c_double)]
+
+
+class EigsOpts(C.Structure):
+    _fields_ = [("nev", C.c_int32), ("ncv", C.c_int32), ("tol", C.c_double), ("maxit", C.c_int32),
+                ("which", C.c_int32), ("block", C.c_int32), ("precision", C.c_int32),
+                ("fallback_sweeps", C.c_int32), ("ir_tol", C.c_double), ("ir_maxit", C.c_int32),
+                ("check_every", C.c_int32), ("seed", C.c_uint64)]
+
+
+class EigsInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nconv", "returned", "restarts", "op_columns", "op_calls",
+                                         "ir_max_iterations", "precision", "fallback", "breakdowns",
+                                         "c64_op_columns", "c64_ir_max", "basis_columns")] + \
+               [("seconds", C.c_double), ("host_seconds", C.c_double)]
+
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -82,6 +103,14 @@ SIGNATURES = [
     ("kore_sys_destroy", None, [_P]),
     ("kore_sys_update_a", _I32, [_P, _PD]),
     ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
+    ("kore_sys_query", _I32, [_P, C.POINTER(SysState)]),
+    ("kore_factor_async", _I32, [_P, _D, _D, _I32, _P]),
+    ("kore_factor_check", _I32, [_P]),
+    ("kore_solve_async", _I32, [_P, _I32, _P, _I64, _P, _I64, _I32, _P]),
+    ("kore_eigs_default_opts", None, [C.POINTER(EigsOpts)]),
+    ("kore_eigs", _I32, [_P, _D, _D, C.POINTER(EigsOpts), _PD, _I32, _PD, _PD, _I64, _PD, C.POINTER(EigsInfo)]),
+    ("kore_schur", _I32, [_I32, _PD, _I64, _PD, _I64]),
+    ("kore_schur_reorder", _I32, [_I32, _PD, _I64, _PD, _I64, _PI32, _PI32]),
     ("kore_factor", _I32, [_P, _D, _D, _I32, C.POINTER(FactorInfo)]),
     ("kore_factor_release", _I32, [_P]),
     ("kore_solve", _I32, [_P, _I32, _PD, _I64, _PD, _I64, _D, _I32, C.POINTER(SolveInfo)]),
@@ -249,6 +278,27 @@ class Context:
             pass
 
 
+def schur(T):
+    """(S, Q) with T = Q S Q^H, S upper triangular (kore_schur, host only)."""
+    S = np.asfortranarray(np.array(T, dtype=np.complex128))
+    n = S.shape[0]
+    Q = np.zeros((n, n), dtype=np.complex128, order="F")
+    _check(lib().kore_schur(n, _dptr(S), max(n, 1), _dptr(Q), max(n, 1)))
+    return S, Q
+
+
+def schur_reorder(S, Q, select):
+    """(S', Q', m): the selected diagonal entries moved to the top (kore_schur_reorder, host only)."""
+    S = np.asfortranarray(np.array(S, dtype=np.complex128))
+    Q = np.asfortranarray(np.array(Q, dtype=np.complex128))
+    n = S.shape[0]
+    sel = np.ascontiguousarray(np.asarray(select, dtype=np.int32))
+    m = C.c_int32(0)
+    _check(lib().kore_schur_reorder(n, _dptr(S), max(n, 1), _dptr(Q), max(n, 1), sel.ctypes.data_as(_PI32),
+                                    C.byref(m)))
+    return S, Q, m.value
+
+
 class System:
     """A + sigma B system resident on the device (Kore ordering on the host)."""
 
@@ -311,6 +361,61 @@ class System:
 
     def release_factor(self):
         _check(lib().kore_factor_release(self._p))
+
+    def query(self) -> SysState:
+        out = SysState()
+        _check(lib().kore_sys_query(self._p, C.byref(out)))
+        return out
+
+    def factor_async(self, sigma, precision=PREC_FP64, stream=0):
+        """Queues the factorisation behind the work on the HIP stream
+        ``stream`` (an integer hipStream_t of the HIP runtime the library is
+        linked against; 0: the default stream); ``stream`` waits for it
+        (kore_factor_async)."""
+        sigma = complex(sigma)
+        _check(lib().kore_factor_async(self._p, sigma.real, sigma.imag, int(precision), _P(int(stream) or None)))
+        self.sigma = sigma
+        self.precision = precision
+
+    def factor_check(self):
+        """Waits for the context's stream; raises on a zero pivot of a queued factorisation."""
+        _check(lib().kore_factor_check(self._p))
+
+    def solve_async(self, d_b: int, d_x: int, nrhs=1, ldb=None, ldx=None, ir_sweeps=0, stream=0):
+        """Queues x = (A - sigma B)^{-1} b on device arrays (addresses d_b,
+        d_x of complex128 columns in Kore order) behind ``stream`` (kore_solve_async)."""
+        _check(lib().kore_solve_async(self._p, int(nrhs), _P(int(d_b)), int(ldb or self.n), _P(int(d_x)),
+                                      int(ldx or self.n), int(ir_sweeps), _P(int(stream) or None)))
+
+    def eigs_native(self, tau, nev, ncv=0, tol=1e-12, maxit=100, which="TM", block=1, precision=PREC_FP64,
+                    seed=0, v0=None, max_pairs=None, vectors=True, check_every=-1, fallback_sweeps=4,
+                    ir_tol=1e-13, ir_maxit=12):
+        """Eigenpairs nearest tau through the library's own Krylov-Schur
+        (kore_eigs): returns (lam, X, errest, EigsInfo, status)."""
+        L = lib()
+        o = EigsOpts()
+        L.kore_eigs_default_opts(C.byref(o))
+        o.nev, o.ncv, o.tol, o.maxit = int(nev), int(ncv or 0), float(tol), int(maxit)
+        o.which, o.block, o.precision, o.seed = WHICH[which.upper()], int(block), int(precision), int(seed)
+        o.check_every, o.fallback_sweeps, o.ir_tol, o.ir_maxit = (int(check_every), int(fallback_sweeps),
+                                                                  float(ir_tol), int(ir_maxit))
+        kmax = int(max_pairs if max_pairs is not None else max(nev, 1) + 16)
+        lam = np.zeros(kmax, dtype=np.complex128)
+        err = np.zeros(kmax)
+        X = np.zeros((self.n, kmax), dtype=np.complex128, order="F") if vectors else None
+        v = None
+        if v0 is not None:
+            v = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(self.n))
+        info = EigsInfo()
+        tau = complex(tau)
+        rc = _check(L.kore_eigs(self._p, tau.real, tau.imag, C.byref(o), _dptr(v) if v is not None else None,
+                                kmax, _dptr(lam), _dptr(X) if X is not None else None, self.n, _dptr(err),
+                                C.byref(info)))
+        st = self.query()
+        self.sigma = complex(st.sre, st.sim)
+        self.precision = st.precision
+        k = info.returned
+        return lam[:k], (X[:, :k] if X is not None else None), err[:k], info, rc
 
     def get_workspace(self, ncol: int) -> "Workspace":
         """A device workspace of at least ncol columns, kept with the system

@@ -18,7 +18,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkore_amd.so")
 
-SOURCES = ["kernels_dense.hip", "kernels_vec.hip", "kernels_sparse.hip", "runtime.cpp"]
+SOURCES = ["kernels_dense.hip", "kernels_vec.hip", "kernels_sparse.hip", "runtime.cpp", "eigs.cpp", "dense_host.cpp"]
+
+# per-source extra flags: the host-only dense kernels multiply complex numbers
+# without the NaN/Inf recovery calls of full-range complex arithmetic
+EXTRA = {"dense_host.cpp": ["-fcx-limited-range"]}
 
 ARCH = os.environ.get("KORE_OFFLOAD_ARCH", "gfx950")
 
@@ -40,7 +44,7 @@ def _hipcc():
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc()] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [_hipcc()] + COMMON + EXTRA.get(src, []) + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
@@ -49,7 +53,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), "-shared", "-fPIC", "--offload-arch=%s" % ARCH, "-o", tmp] + objs

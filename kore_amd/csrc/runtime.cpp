@@ -48,6 +48,12 @@ static int32_t fail(int32_t code, const char* fmt, ...) {
   return code;
 }
 
+// the error text of this host thread, set from the library's other sources
+int32_t kore_internal_set_error(int32_t code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
 #define HIP_TRY(...)                                                                             \
   do {                                                                                           \
     hipError_t _e = (__VA_ARGS__);                                                               \
@@ -638,6 +644,15 @@ struct kore_sys {
   int gm_cols = 0;
   std::mutex ws_mu;
   std::vector<kore_ws*> wss; // live workspaces: destroyed with the system at the latest
+  // stream-ordered entry points: the caller's stream and the context's stream
+  // wait for each other through these events
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  PinnedBuf info_pin;        // pivot flags of a queued factorisation (read by kore_factor_check)
+  bool info_pending = false;
+  ~kore_sys() {
+    if (ev_in) hipEventDestroy(ev_in);
+    if (ev_out) hipEventDestroy(ev_out);
+  }
 };
 
 struct kore_ws {
@@ -2147,6 +2162,18 @@ void kore_sys_destroy(kore_sys* sys) {
   delete sys;
 }
 
+int32_t kore_sys_query(const kore_sys* s, kore_sys_state* out) {
+  if (!s || !out) return fail(KORE_ERR_ARG, "NULL argument");
+  out->sizmat = s->n;
+  out->nblk = s->nblk;
+  out->has_b = s->has_b ? 1 : 0;
+  out->factored = (s->f && s->factored) ? 1 : 0;
+  out->precision = s->f ? s->f->prec : -1;
+  out->sre = s->sre;
+  out->sim = s->sim;
+  return KORE_OK;
+}
+
 int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
   if (!s || !out) return fail(KORE_ERR_ARG, "NULL argument");
   kore_ctx* c = s->ctx;
@@ -2164,13 +2191,14 @@ int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
   return KORE_OK;
 }
 
-int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore_factor_info* info) {
-  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
-  if (precision != KORE_PREC_FP64 && precision != KORE_PREC_C64) return fail(KORE_ERR_ARG, "bad precision");
+}  // extern "C"
+
+// Queues the factorisation of A - sigma B on the context's stream (plan and
+// tile choice first, which may wait for the device); e0/e1 bracket the
+// factorisation proper when given.  *dinfo: the device pivot flags.
+static int32_t factor_enqueue(kore_sys* s, double sre, double sim, int32_t precision, hipEvent_t e0, hipEvent_t e1,
+                              const int** dinfo) {
   kore_ctx* c = s->ctx;
-  HIP_TRY(hipSetDevice(c->device));
-  // the reduction plan and its buffers depend only on the layout: keep them
-  // across shifts of the same system, rebuild only when the precision changes
   // keep the other precision's plan (and its device memory) when the
   // precision changes, so that alternating precisions does not rebuild plans
   if (s->f && s->f->prec != precision) {
@@ -2179,17 +2207,14 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
   }
   s->sre = sre;
   s->sim = sim;
+  s->factored = false;
+  s->info_pending = false;
   // power-of-two row equilibration of A - sigma B (device), then the
   // level-0 couplings' values R (A - sigma B) for the compact views
   HIP_TRY(launch_row_scale(s->csr, sre, sim, s->d_scale.as<double>(), c->stream));
   if (s->nnz_off > 0)
     HIP_TRY(launch_offvals(s->csr, sre, sim, s->d_scale.as<double>(), s->d_split.as<int32_t>(), s->off, c->stream));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
   int32_t rc = KORE_OK;
-  std::vector<int> hinfo(s->nblk, 0);
-  const int* dinfo = nullptr;
   if (precision == KORE_PREC_FP64) {
     if (!s->f) {
       auto F = std::make_unique<Factor<double>>();
@@ -2200,10 +2225,10 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     if (rc >= 0) {
       Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
       rc = tune_factor<double>(s, F, precision);
-      hipEventRecord(e0, c->stream);
+      if (e0) hipEventRecord(e0, c->stream);
       if (rc >= 0) rc = factor_run<double>(s, F);
-      hipEventRecord(e1, c->stream);
-      dinfo = F->info;
+      if (e1) hipEventRecord(e1, c->stream);
+      *dinfo = F->info;
     }
   } else {
     if (!s->f) {
@@ -2215,32 +2240,72 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     if (rc >= 0) {
       Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
       rc = tune_factor<float>(s, F, precision);
-      hipEventRecord(e0, c->stream);
+      if (e0) hipEventRecord(e0, c->stream);
       if (rc >= 0) rc = factor_run<float>(s, F);
-      hipEventRecord(e1, c->stream);
-      dinfo = F->info;
+      if (e1) hipEventRecord(e1, c->stream);
+      *dinfo = F->info;
     }
   }
-  if (rc >= 0 && dinfo) hipMemcpyAsync(hinfo.data(), dinfo, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-  hipError_t se = hipStreamSynchronize(c->stream);
-  float ms = 0.f;
-  if (rc >= 0 && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (rc < 0) {
-    s->f.reset();
-    return rc;
-  }
-  if (se != hipSuccess) {
-    s->f.reset();
-    return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
-  }
+  if (rc < 0) s->f.reset();
+  return rc;
+}
+
+// Pivot flags (host copy) -> status; a zero pivot leaves the system unfactored.
+static int32_t factor_verdict(kore_sys* s, const int* hinfo) {
   s->factored = true;
   for (int k = 0; k < s->nblk; ++k)
     if (hinfo[k] != 0) {
       s->factored = false;
       return fail(KORE_ERR_SINGULAR, "zero pivot in degree block %d (column %d) during cyclic reduction", k, hinfo[k]);
     }
+  return KORE_OK;
+}
+
+// The context's stream waits for the work queued so far on `user`.
+static int32_t stream_join_in(kore_sys* s, hipStream_t user) {
+  kore_ctx* c = s->ctx;
+  if (user == c->stream) return KORE_OK;
+  if (!s->ev_in) HIP_TRY(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(s->ev_in, user));
+  HIP_TRY(hipStreamWaitEvent(c->stream, s->ev_in, 0));
+  return KORE_OK;
+}
+
+// `user` waits for the work queued so far on the context's stream.
+static int32_t stream_join_out(kore_sys* s, hipStream_t user) {
+  kore_ctx* c = s->ctx;
+  if (user == c->stream) return KORE_OK;
+  if (!s->ev_out) HIP_TRY(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(s->ev_out, c->stream));
+  HIP_TRY(hipStreamWaitEvent(user, s->ev_out, 0));
+  return KORE_OK;
+}
+
+extern "C" {
+
+int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore_factor_info* info) {
+  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
+  if (precision != KORE_PREC_FP64 && precision != KORE_PREC_C64) return fail(KORE_ERR_ARG, "bad precision");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  std::vector<int> hinfo(s->nblk, 0);
+  const int* dinfo = nullptr;
+  int32_t rc = factor_enqueue(s, sre, sim, precision, e0, e1, &dinfo);
+  if (rc >= 0 && dinfo) hipMemcpyAsync(hinfo.data(), dinfo, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  hipError_t se = hipStreamSynchronize(c->stream);
+  float ms = 0.f;
+  if (rc >= 0 && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc < 0) return rc;
+  if (se != hipSuccess) {
+    s->f.reset();
+    return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
+  }
+  TRY(factor_verdict(s, hinfo.data()));
   if (info) {
     info->levels = s->f->nlev;
     info->nblk = s->nblk;
@@ -2254,6 +2319,39 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     info->ms = ms;
   }
   return KORE_OK;
+}
+
+int32_t kore_factor_async(kore_sys* s, double sre, double sim, int32_t precision, void* stream) {
+  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
+  if (precision != KORE_PREC_FP64 && precision != KORE_PREC_C64) return fail(KORE_ERR_ARG, "bad precision");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const hipStream_t user = static_cast<hipStream_t>(stream);
+  TRY(stream_join_in(s, user));
+  TRY(s->info_pin.reserve(s->nblk * sizeof(int), c->stream));
+  const int* dinfo = nullptr;
+  TRY(factor_enqueue(s, sre, sim, precision, nullptr, nullptr, &dinfo));
+  HIP_TRY(hipMemcpyAsync(s->info_pin.p, dinfo, s->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  TRY(stream_join_out(s, user));
+  // solves queued behind this call use the factor; its pivots are checked
+  // by kore_factor_check
+  s->factored = true;
+  s->info_pending = true;
+  return KORE_OK;
+}
+
+int32_t kore_factor_check(kore_sys* s) {
+  if (!s) return fail(KORE_ERR_ARG, "sys is NULL");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const hipError_t se = hipStreamSynchronize(c->stream);
+  if (se != hipSuccess) {
+    s->factored = false;
+    return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
+  }
+  if (!s->info_pending) return s->factored ? KORE_OK : fail(KORE_ERR_STATE, "system is not factored");
+  s->info_pending = false;
+  return factor_verdict(s, reinterpret_cast<const int*>(s->info_pin.p));
 }
 
 int32_t kore_factor_release(kore_sys* s) {
@@ -2297,6 +2395,47 @@ int32_t kore_solve(kore_sys* s, int32_t nrhs, const double* b, int64_t ldb, doub
     }
   }
   return (info && !info->converged) ? KORE_WARN_IR_NOT_CONVERGED : KORE_OK;
+}
+
+int32_t kore_solve_async(kore_sys* s, int32_t nrhs, const double* d_b, int64_t ldb, double* d_x, int64_t ldx,
+                         int32_t ir_sweeps, void* stream) {
+  if (!s || !d_b || !d_x || nrhs < 0 || ldb < s->n || ldx < s->n || ir_sweeps < 0)
+    return fail(KORE_ERR_ARG, "bad solve arguments");
+  if (!s->f || !s->factored) return fail(KORE_ERR_STATE, "no factorisation: call kore_factor first");
+  kore_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const hipStream_t user = static_cast<hipStream_t>(stream);
+  const hipStream_t st = c->stream;
+  const int64_t n = s->n;
+  const bool c64 = s->f->prec == KORE_PREC_C64;
+  TRY(stream_join_in(s, user));
+  const cx<double>* b = reinterpret_cast<const cx<double>*>(d_b);
+  cx<double>* x = reinterpret_cast<cx<double>*>(d_x);
+  for (int c0 = 0; c0 < nrhs; c0 += MAXR) {
+    const int nr = std::min(MAXR, nrhs - c0);
+    // Kore order -> degree order, the factor solve, then fixed refinement
+    // sweeps (r = b - (A - sigma B) x by FP64 SpMV, x += factor^{-1} R r)
+    HIP_TRY(launch_gather_scale<double, double>(b + (size_t)c0 * ldb, ldb, s->bb, n, n, nr, s->d_perm.as<int64_t>(),
+                                                nullptr, 1.0, st));
+    if (c64) TRY(refine_sweep<float>(s, static_cast<Factor<float>*>(s->f.get()), s->bb, n, s->xr, n, nr, true));
+    else TRY(refine_sweep<double>(s, static_cast<Factor<double>*>(s->f.get()), s->bb, n, s->xr, n, nr, true));
+    for (int it = 0; it < ir_sweeps; ++it) {
+      HIP_TRY(launch_copy_cols(s->bb, n, s->rr, n, n, nr, st));
+      HIP_TRY(launch_spmv(s->csr, -1.0, 0.0, s->sre, s->sim, s->xr, n, s->rr, n, nr, 1.0, st));
+      if (c64) {
+        Factor<float>* F = static_cast<Factor<float>*>(s->f.get());
+        TRY(refine_sweep<float>(s, F, s->rr, n, s->xr, n, nr, false));
+        HIP_TRY(launch_axpy<float>(s->xr, n, F->Xv, n, n, nr, 1.0, st));
+      } else {
+        Factor<double>* F = static_cast<Factor<double>*>(s->f.get());
+        TRY(refine_sweep<double>(s, F, s->rr, n, s->xr, n, nr, false));
+        HIP_TRY(launch_axpy<double>(s->xr, n, F->Xv, n, n, nr, 1.0, st));
+      }
+    }
+    HIP_TRY(launch_scatter(s->xr, n, x + (size_t)c0 * ldx, ldx, n, nr, s->d_perm.as<int64_t>(), st));
+  }
+  TRY(stream_join_out(s, user));
+  return KORE_OK;
 }
 
 int32_t kore_residual(kore_sys* s, int32_t nrhs, const double* x, int64_t ldx, const double* lam, const double* b,

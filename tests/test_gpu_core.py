@@ -195,14 +195,15 @@ def test_solve_many_rhs(ctx, hydro_small):
 
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("block", [1, 4])
-def test_eigs_hydro_small(ctx, hydro_small, precision, block):
+@pytest.mark.parametrize("driver", ["native", "python"])
+def test_eigs_hydro_small(ctx, hydro_small, precision, block, driver):
     from kore_amd import eigs
     from oracle import krylov_schur
     par, A, B = hydro_small
-    say("case: hydro_small eigs precision", precision, "block", block)
+    say("case: hydro_small eigs precision", precision, "block", block, "driver", driver)
     S = _system(ctx, par, A, B)
     S.factor(par.tau, precision)
-    r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=block)
+    r = eigs.eigs(S, par.tau, nev=5, tol=1e-12, maxit=100, block=block, driver=driver)
     say("  nconv", r.nconv, "restarts", r.restarts, "ops", r.op_columns, "fallback", r.fallback,
         "c64 ops", r.c64_op_columns, "ir max", r.c64_ir_max)
     if precision == 1:     # the pairs come from the complex64 factor with FP64 refinement
