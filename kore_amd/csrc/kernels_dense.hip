@@ -21,23 +21,18 @@ namespace kore {
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
 constexpr int GT_PADB = 1;
 
+// One 64 x 64 tile of C = alpha sum_p A_p B_p + beta C for the workgroup
+// (256 threads); sA / sB are the workgroup's LDS staging.
 template <class T>
-__global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>* __restrict__ descs) {
+__device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int tile, cx<T> (&sA)[GT_K][GT_M],
+                                               cx<T> (&sB)[GT_N][GT_K + GT_PADB]) {
   using A4 = typename acc4<T>::type;
-  const GemmDesc<T> d = descs[blockIdx.y];
-  const int ntiles = d.tiles_m * d.tiles_n;
-  if ((int)blockIdx.x >= ntiles) return;
-
-  __shared__ cx<T> sA[GT_K][GT_M];
-  __shared__ cx<T> sB[GT_N][GT_K + GT_PADB];
-
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, lq = lane >> 4;
-
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  {
     const int i0 = (tile % d.tiles_m) * GT_M;
     const int j0 = (tile / d.tiles_m) * GT_N;
     // 16 x 16 sub-tiles wholly outside C (edge tiles of sizes that are not
@@ -134,27 +129,49 @@ __global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>*
       k0 = kn;
     }
     __syncthreads();                        // LDS is reused by the next tile
-    // epilogue: lane holds C[i0+wr+16rs+l16][j0+wc+16cs+mfma_mrow(lq,r)]
+    // epilogue: lane holds C[i0+wr+16rs+l16][j0+wc+16cs+mfma_mrow(lq,r)].
+    // Per row sub-tile, the eight old values of C are loaded together before
+    // any store, so that their latencies overlap (a load after a store to a
+    // possibly aliasing address would wait for the previous one).
+    const bool acc = d.beta != T(0);
 #pragma unroll
-    for (int rs = 0; rs < 2; ++rs)
+    for (int rs = 0; rs < 2; ++rs) {
+      const int gi = i0 + wr + 16 * rs + l16;
+      cx<T> o[2][4];
 #pragma unroll
       for (int cs = 0; cs < 2; ++cs)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int gi = i0 + wr + 16 * rs + l16;
+          const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, r);
+          o[cs][r] = czero<T>();
+          if (acc && gi < d.m && gj < d.n) o[cs][r] = gld(d.C + (int64_t)gj * d.ldc + gi);
+        }
+#pragma unroll
+      for (int cs = 0; cs < 2; ++cs)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
           const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, r);
           if (gi < d.m && gj < d.n) {
-            cx<T>* c = d.C + (int64_t)gj * d.ldc + gi;
             cx<T> v{d.alpha * cre[rs][cs][r], d.alpha * cim[rs][cs][r]};
-            if (d.beta != T(0)) {
-              cx<T> o = gld(c);
-              v.re += d.beta * o.re;
-              v.im += d.beta * o.im;
+            if (acc) {
+              v.re += d.beta * o[cs][r].re;
+              v.im += d.beta * o[cs][r].im;
             }
-            gst(c, v);
+            gst(d.C + (int64_t)gj * d.ldc + gi, v);
           }
         }
+    }
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256, 3) void gemm_batched_kernel(const GemmDesc<T>* __restrict__ descs) {
+  const GemmDesc<T> d = descs[blockIdx.y];
+  const int ntiles = d.tiles_m * d.tiles_n;
+  if ((int)blockIdx.x >= ntiles) return;
+  __shared__ cx<T> sA[GT_K][GT_M];
+  __shared__ cx<T> sB[GT_N][GT_K + GT_PADB];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) gemm_tile_body<T>(d, tile, sA, sB);
 }
 
 template <class T> __device__ __forceinline__ T tiny_pivot() { return sizeof(T) == 8 ? T(1e-150) : T(1e-18); }
@@ -447,8 +464,8 @@ __global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __res
 //                     for j outside J, on matrix cores;
 // then gj_perm_kernel twice: A^{-1}[k, l] = M[r_k, r^{-1}(l)] through the
 // scratch block and back.  Scratch layout (per block, n x n complex): R
-// (32 x n, R[k, j] at R[j 32 + k]), then n ints of row roles (-1 unused, -2
-// pivot of an earlier panel, c >= 0: pivot c of the current panel).  The
+// (32 x n, R[k, j] at R[j 32 + k]), then n ints of row roles (-1 unused,
+// c >= 0: the pivot row of column c).  The
 // multi-panel composition: with T the product of the panel's elementary
 // transforms, T differs from the identity only in the pivot columns P, so for
 // a column j outside J,  T M[:, j] = [i not in P] M[i, j] + T[:, P] M[P, j],
@@ -465,6 +482,30 @@ __device__ __forceinline__ V pick(const V (&v)[N], int idx) {
 
 constexpr int GJM_WP = 32;
 constexpr int GJ_MULTI_MIN = 96;          // blocks at least this large take the multi-workgroup form
+// Two-level panels for blocks of at least GJ_OUTER_MIN rows: the panels of
+// 32 columns are grouped into outer panels O of GJ_WO columns.  A panel's
+// rank-32 update is applied to the columns of its outer panel only; once the
+// outer panel is complete, the columns outside it receive the composite
+// transform of its GJ_WO eliminations in one rank-GJ_WO product (the same
+// composition argument as for a single panel, with P = the outer panel's
+// pivot rows).  That halves the passes over the block and turns most of the
+// update into a product with K = GJ_WO on the matrix cores.  Scratch layout
+// then: R (32 x n), n ints of row roles, R_O (GJ_WO x n): 161 n entries, so
+// the block needs n >= 161.  Below 256 rows the two extra launches per outer
+// panel cost more than the passes they save (measured at 200 rows).
+constexpr int GJ_WO = 128;
+constexpr int GJ_OUTER_MIN = 256;
+
+// Outer panel [q0, q0 + wo) holding column p0 of an n x n block.
+__device__ __forceinline__ void gj_outer_range(int n, int p0, int& q0, int& wo) {
+  if (n >= GJ_OUTER_MIN) {
+    q0 = (p0 / GJ_WO) * GJ_WO;
+    wo = min(GJ_WO, n - q0);
+  } else {
+    q0 = 0;
+    wo = n;
+  }
+}
 
 template <class T, int NMAX>
 __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
@@ -495,9 +536,11 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   for (int q = tid; q < WP; q += NT) sPiv[q] = 0;
   __syncthreads();
   // every row is unused at the first panel (the scratch holds an earlier call's
-  // roles); afterwards any pivot of an earlier panel is "used"
+  // roles); afterwards a row's role is the column it pivots, or -1
   int role = -2;
-  if (live) role = (p0 == 0 || roles[r] == -1) ? -1 : -2;
+  if (live) role = p0 == 0 ? -1 : roles[r];
+  int q0, wo;
+  gj_outer_range(n, p0, q0, wo);
   cx<T> myipv = czero<T>();               // this row's inverse pivot, if it becomes a pivot row
   cx<T> v[CPT];
 #pragma unroll
@@ -532,7 +575,7 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
       }
       const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
       if (tid == 0) sPiv[c] = p;
-      if (r == p) role = c;
+      if (r == p) role = p0 + c;
       const cx<T>* __restrict__ prow = sCand[par][q];
       cx<T> pv = prow[(c / CPT) * SL + c % CPT];
       T den = pv.re * pv.re + pv.im * pv.im;
@@ -576,13 +619,14 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   for (int c = 0; c < WP / 2; ++c) step(c);
 #pragma unroll
   for (int c = WP / 2; c < WP; ++c) step(c);
-  if (role >= 0) {                        // pivot rows of this panel: the deferred scale
+  if (role >= p0) {                       // pivot rows of this panel: the deferred scale
 #pragma unroll
     for (int jj = 0; jj < CPT; ++jj) v[jj] = cmul(v[jj], myipv);
   }
   __syncthreads();
-  // C -> M[:, J]; R = M[P, :] outside J (disjoint columns, so the order of the
-  // two does not matter; the update kernel has not run yet)
+  // C -> M[:, J]; R = M[P, :] on the outer panel's columns outside J
+  // (disjoint columns, so the order of the two does not matter; the update
+  // kernel has not run yet)
   if (live) {
 #pragma unroll
     for (int jj = 0; jj < CPT; ++jj) {
@@ -593,26 +637,27 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   }
   if (tid < w) rowof[p0 + tid] = sPiv[tid];
   constexpr int RB = 8;                   // loads in flight per thread
-  for (int base = 0; base < WP * n; base += RB * NT) {
+  for (int base = 0; base < WP * wo; base += RB * NT) {
     cx<T> x[RB];
 #pragma unroll
     for (int t = 0; t < RB; ++t) {
       const int idx = base + t * NT + tid;
-      const int k = idx % WP, j = idx / WP;
+      const int k = idx % WP, j = q0 + idx / WP;
       x[t] = czero<T>();
-      if (idx < WP * n && k < w && (j < p0 || j >= p0 + w)) x[t] = M[(int64_t)j * ld + sPiv[k]];
+      if (idx < WP * wo && k < w && (j < p0 || j >= p0 + w)) x[t] = M[(int64_t)j * ld + sPiv[k]];
     }
 #pragma unroll
     for (int t = 0; t < RB; ++t) {
       const int idx = base + t * NT + tid;
-      if (idx < WP * n) R[idx] = x[t];
+      if (idx < WP * wo) R[(int64_t)q0 * WP + idx] = x[t];
     }
   }
   // info: reset by the first panel, then the first singular column wins
   if (tid == 0 && d.info && (p0 == 0 || (sSing && *d.info == 0))) *d.info = sSing;
 }
 
-// One 64 x 64 tile of M per workgroup, 4 waves of 2 x 2 MFMA 16 x 16 tiles.
+// One 64 x 64 tile of M per workgroup, 4 waves of 2 x 2 MFMA 16 x 16 tiles,
+// over the columns of the panel's outer panel outside the panel.
 template <class T>
 __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
   using A4 = typename acc4<T>::type;
@@ -623,9 +668,12 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
   const InvDesc<T> d = descs[blockIdx.y];
   const int n = d.n;
   if (p0 >= n) return;
+  int q0, wo;
+  gj_outer_range(n, p0, q0, wo);
+  const int ne = q0 + wo;                 // end of the outer panel's columns
   const int nt = (n + 63) / 64;
-  if ((int)blockIdx.x >= nt * nt) return;
-  const int i0 = (blockIdx.x % nt) * 64, j0 = (blockIdx.x / nt) * 64;
+  if ((int)blockIdx.x >= nt * ((wo + 63) / 64)) return;
+  const int i0 = (blockIdx.x % nt) * 64, j0 = q0 + (blockIdx.x / nt) * 64;
   const int w = min(WP, n - p0);
   const int64_t ld = d.ld;
   cx<T>* __restrict__ M = d.M;
@@ -635,7 +683,7 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, lq = lane >> 4;
   const bool rv0 = i0 + wr < n, rv1 = i0 + wr + 16 < n;    // sub-tiles inside the block
-  const bool cv0 = j0 + wc < n, cv1 = j0 + wc + 16 < n;
+  const bool cv0 = j0 + wc < ne, cv1 = j0 + wc + 16 < ne;
   for (int idx = tid; idx < WP * 64; idx += 256) {
     const int k = idx / 64, i = idx % 64;           // consecutive threads: consecutive rows
     cx<T> x = czero<T>();
@@ -645,7 +693,7 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
   for (int idx = tid; idx < WP * 64; idx += 256) {
     const int k = idx % WP, j = idx / WP;           // consecutive threads: consecutive k of R
     cx<T> x = czero<T>();
-    if (j0 + j < n) x = R[(int64_t)(j0 + j) * WP + k];   // zero beyond w (panel kernel)
+    if (j0 + j < ne) x = R[(int64_t)(j0 + j) * WP + k];  // zero beyond w (panel kernel)
     sB[j][k] = x;
   }
   if (tid < 64) sRole[tid] = (i0 + tid < n) ? roles[i0 + tid] : -1;
@@ -672,33 +720,45 @@ __global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __rest
       for (int cs = 0; cs < 2; ++cs)
         if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
   }
+  // epilogue: per row sub-tile, the old values are loaded together first
+  // (rows that are pivots of this panel are replaced, the others accumulate)
 #pragma unroll
-  for (int rs = 0; rs < 2; ++rs)
+  for (int rs = 0; rs < 2; ++rs) {
+    const int li = wr + 16 * rs + l16;
+    const int gi = i0 + li;
+    const bool acc = !(sRole[li] >= p0 && sRole[li] < p0 + w);
+    cx<T> o[2][4];
 #pragma unroll
     for (int cs = 0; cs < 2; ++cs)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int li = wr + 16 * rs + l16;
-        const int gi = i0 + li;
         const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, rr);
-        if (gi < n && gj < n && (gj < p0 || gj >= p0 + w)) {
-          cx<T>* cp = M + (int64_t)gj * ld + gi;
+        o[cs][rr] = czero<T>();
+        if (acc && gi < n && gj < ne && (gj < p0 || gj >= p0 + w)) o[cs][rr] = M[(int64_t)gj * ld + gi];
+      }
+#pragma unroll
+    for (int cs = 0; cs < 2; ++cs)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, rr);
+        if (gi < n && gj < ne && (gj < p0 || gj >= p0 + w)) {
           cx<T> x{cre[rs][cs][rr], cim[rs][cs][rr]};
-          if (!(sRole[li] >= 0)) {
-            const cx<T> o = *cp;
-            x.re += o.re;
-            x.im += o.im;
-          }
-          *cp = x;
+          x.re += o[cs][rr].re;
+          x.im += o[cs][rr].im;
+          M[(int64_t)gj * ld + gi] = x;
         }
       }
+  }
 }
 
 // phase 0: S[l n + k] = M[r_k, r^{-1}(l)];  phase 1: M <- S.   Grid: (column chunks, blocks).
+// Phase 0 reads each source column whole (coalesced) into LDS and gathers
+// its rows from there.
 template <class T>
 __global__ __launch_bounds__(256) void gj_perm_kernel(const InvDesc<T>* __restrict__ descs, int phase) {
   __shared__ int sRinv[512];
   __shared__ int sRow[512];
+  __shared__ cx<T> sCol[512];
   const InvDesc<T> d = descs[blockIdx.y];
   const int n = d.n;
   constexpr int CC = 16;                                   // columns per workgroup
@@ -716,12 +776,83 @@ __global__ __launch_bounds__(256) void gj_perm_kernel(const InvDesc<T>* __restri
     __syncthreads();
     for (int l = l0; l < min(n, l0 + CC); ++l) {
       const cx<T>* src = M + (int64_t)sRinv[l] * ld;
-      for (int k = threadIdx.x; k < n; k += 256) S[(int64_t)l * n + k] = src[sRow[k]];
+      for (int k = threadIdx.x; k < n; k += 256) sCol[k] = src[k];
+      __syncthreads();
+      for (int k = threadIdx.x; k < n; k += 256) S[(int64_t)l * n + k] = sCol[sRow[k]];
+      __syncthreads();                                     // sCol is refilled for the next column
     }
   } else {
     for (int l = l0; l < min(n, l0 + CC); ++l)
       for (int k = threadIdx.x; k < n; k += 256) M[(int64_t)l * ld + k] = S[(int64_t)l * n + k];
   }
+}
+
+// End of outer panel O = [q0, q0 + wo): R_O[k, j] = M[P_O[k], j] for the
+// columns j outside O (at R_O[j GJ_WO + k]), and those entries of M are
+// cleared (the outer product then adds T_O[:, P_O] R_O to every row, which
+// replaces the pivot rows and accumulates into the others).
+template <class T>
+__global__ __launch_bounds__(256) void gj_outer_gather_kernel(const InvDesc<T>* __restrict__ descs, int q0) {
+  const InvDesc<T> d = descs[blockIdx.y];
+  const int n = d.n;
+  if (n < GJ_OUTER_MIN || q0 >= n) return;
+  const int wo = min(GJ_WO, n - q0);
+  const int64_t ld = d.ld;
+  cx<T>* __restrict__ M = d.M;
+  cx<T>* __restrict__ RO = d.scratch + (size_t)(GJM_WP + 1) * n;
+  const int* __restrict__ piv = d.ipiv + q0;
+  const int total = (n - wo) * GJ_WO;
+  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < total; idx += gridDim.x * 256) {
+    const int k = idx % GJ_WO, jc = idx / GJ_WO;
+    const int j = jc < q0 ? jc : jc + wo;
+    cx<T> x = czero<T>();
+    if (k < wo) {
+      cx<T>* p = M + (int64_t)j * ld + piv[k];
+      x = *p;
+      *p = czero<T>();
+    }
+    RO[(int64_t)j * GJ_WO + k] = x;
+  }
+}
+
+// M[:, j] += M[:, O] R_O[:, j] for the columns outside O: the batched GEMM's
+// tile on descriptors made here (columns [0, q0), then [q0 + wo, n)).
+template <class T>
+__global__ __launch_bounds__(256, 3) void gj_outer_update_kernel(const InvDesc<T>* __restrict__ descs, int q0) {
+  __shared__ cx<T> sA[GT_K][GT_M];
+  __shared__ cx<T> sB[GT_N][GT_K + GT_PADB];
+  const InvDesc<T> v = descs[blockIdx.y];
+  const int n = v.n;
+  if (n < GJ_OUTER_MIN || q0 >= n) return;
+  const int wo = min(GJ_WO, n - q0);
+  const int tm = (n + 63) / 64, tn0 = (q0 + 63) / 64, n1 = n - q0 - wo, tn1 = (n1 + 63) / 64;
+  int t = blockIdx.x;
+  if (t >= tm * (tn0 + tn1)) return;      // uniform over the workgroup
+  int jstart = 0, ncols = q0;
+  if (t >= tm * tn0) {
+    t -= tm * tn0;
+    jstart = q0 + wo;
+    ncols = n1;
+  }
+  GemmDesc<T> d{};
+  d.A[0] = v.M + (int64_t)q0 * v.ld;
+  d.lda[0] = v.ld;
+  d.k[0] = wo;
+  d.B[0] = v.scratch + (size_t)(GJM_WP + 1) * n + (size_t)jstart * GJ_WO;
+  d.ldb[0] = GJ_WO;
+  d.A[1] = nullptr;
+  d.B[1] = nullptr;
+  d.k[1] = 0;
+  d.C = v.M + (int64_t)jstart * v.ld;
+  d.ldc = v.ld;
+  d.m = n;
+  d.n = ncols;
+  d.opA = 0;
+  d.tiles_m = tm;
+  d.tiles_n = (ncols + 63) / 64;
+  d.alpha = T(1);
+  d.beta = T(1);
+  gemm_tile_body<T>(d, t, sA, sB);
 }
 
 template <class T>
@@ -732,7 +863,18 @@ static hipError_t launch_gj_inverse_multi(const InvDesc<T>* d_descs, int ndesc, 
       hipLaunchKernelGGL((gj_panel_kernel<T, 256>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
     else
       hipLaunchKernelGGL((gj_panel_kernel<T, 512>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
-    hipLaunchKernelGGL((gj_update_kernel<T>), dim3(nt * nt, ndesc), dim3(256), 0, s, d_descs, p0);
+    // column tiles of an outer panel: 2 (blocks of GJ_OUTER_MIN rows or
+    // more), else the whole block, at most 4 (blocks below 256 rows)
+    const int ntc = nmax >= GJ_OUTER_MIN ? (nt < 4 ? nt : 4) : nt;
+    hipLaunchKernelGGL((gj_update_kernel<T>), dim3(nt * ntc, ndesc), dim3(256), 0, s, d_descs, p0);
+    // the end of an outer panel (of every block whose outer panel holds p0)
+    const int pe = p0 + GJM_WP;
+    if (nmax >= GJ_OUTER_MIN && (pe % GJ_WO == 0 || pe >= nmax)) {
+      const int q0 = (p0 / GJ_WO) * GJ_WO;
+      const int chunks = (nmax * GJ_WO + 255) / 256 < 256 ? (nmax * GJ_WO + 255) / 256 : 256;
+      hipLaunchKernelGGL((gj_outer_gather_kernel<T>), dim3(chunks, ndesc), dim3(256), 0, s, d_descs, q0);
+      hipLaunchKernelGGL((gj_outer_update_kernel<T>), dim3(nt * nt, ndesc), dim3(256), 0, s, d_descs, q0);
+    }
   }
   const int nc = (nmax + 15) / 16;
   hipLaunchKernelGGL((gj_perm_kernel<T>), dim3(nc, ndesc), dim3(256), 0, s, d_descs, 0);
