@@ -124,8 +124,8 @@ def run_path(S, ctx, wl, precision, steps, warmup, block, ncv, timing_steps=2):
 
 def roofline(run, precision, bsz, n):
     """Roofline of the dominant kernel class of an eigensolve step.  Solve
-    sweeps are priced at the block-Thomas LU's bytes (the
-    algorithmic bytes of one single-column solve); the bytes the cyclic
+    sweeps are priced at the block-Thomas LU's bytes (the algorithmic bytes
+    of one single-column solve); the bytes the cyclic
     reduction's sweep actually streams are reported beside them
     (traffic_ratio).  Factor classes at the flops actually run."""
     t = run["timing"]
@@ -144,13 +144,15 @@ def roofline(run, precision, bsz, n):
                                    traffic_ratio=f.solve_bytes / alg, launches_per_sweep=f.solve_launches,
                                    sweeps=sweeps / steps, avg_launch_ms=ms / cnt))
     peak = PEAK_FP32 if precision else PEAK_FP64
-    for key, flops in (("inv", f.flops_inv), ("gemm", f.flops_gemm + f.flops_sparse)):
+    for key, flops in (("inv", f.flops_inv), ("gemm", f.flops_gemm), ("sparse", f.flops_sparse)):
         cnt, ms = t[key]
         if cnt and ms > 0:
             tf = steps * flops / (ms * 1e-3) / 1e12
             cand[key] = (ms, dict(bound="mfma", achieved=tf, peak=peak, unit="TFLOP/s",
-                                  kernel="gj_inverse (panel + MFMA update kernels)" if key == "inv" else
-                                  "gemm_batched + level-0 sparse products", avg_launch_ms=ms / cnt))
+                                  kernel={"inv": "gj_inverse (panel, update, outer-update, permutation kernels)",
+                                          "gemm": "gemm_batched_kernel",
+                                          "sparse": "spcol_kernel + sprow_kernel (level 0)"}[key],
+                                  avg_launch_ms=ms / cnt))
     if not cand:
         return None, {}
     dom = max(cand, key=lambda k: cand[k][0])
@@ -214,8 +216,8 @@ def cpu_baseline(A, B, wl, budget_s=45.0):
 
 
 def solve_bytes_8d(bsz, s=16):
-    """Bytes of one single-column solve by the block-Thomas LU:
-    sum_k s (n_k^2 + n_k n_{k+1}) + 3 s sizmat."""
+    """Bytes of one single-column solve by the block-Thomas LU (the
+    algorithmic bytes of the path): sum_k s (n_k^2 + n_k n_{k+1}) + 3 s sizmat."""
     bsz = [int(b) for b in bsz]
     t = sum(s * (bsz[k] ** 2 + (bsz[k] * bsz[k + 1] if k + 1 < len(bsz) else 0)) for k in range(len(bsz)))
     return float(t + 3 * s * sum(bsz))
@@ -271,19 +273,72 @@ def cpu_block_thomas(par, sigma, nops=None, max_blocks=None):
 
 
 def classes_from(results):
-    """Device ms, launches and algorithmic work per kernel class, summed over
+    """Device ms and algorithmic work per kernel class, summed over
     instrumented results."""
     ms = {}
-    flops = dict(inv=0.0, gemm=0.0)
+    flops = dict(inv=0.0, gemm=0.0, sparse=0.0)
     prec = set()
     for r in results:
         for k, v in r.kernel_ms.items():
             ms[k] = ms.get(k, 0.0) + v
         for key, f in r.factor_flops.items():
             prec.add(key)
-            flops["inv"] += f["inv"]
-            flops["gemm"] += f["gemm"] + f["sparse"]
+            for c in flops:
+                flops[c] += f[c]
     return ms, flops, prec
+
+
+def serving_roofline(inst, head_name):
+    """Roofline of the dominant kernel of an instrumented serving sample.
+    The factorisation's batched GEMM (one kernel, gemm_batched_kernel) is
+    priced per precision at the flops of the GEMMs run, over its launches
+    (HIP events around each launch); the solve sweeps at the block-Thomas
+    bytes.  The kernel reported is the one with the most device time among
+    the single-kernel classes (GEMM per precision, solve sweep); every class's
+    share is reported beside it."""
+    ms, flops, precs = classes_from(inst)
+    tot = sum(ms.values())
+    g = {}
+    for r in inst:
+        keys = list(r.factor_flops)
+        if len(keys) != 1:             # a fallback mixed two precisions: left out of the per-precision sums
+            continue
+        e = g.setdefault(keys[0], [0.0, 0, 0.0])
+        e[0] += r.kernel_ms.get("gemm", 0.0)
+        e[1] += r.kernel_launches.get("gemm", 0)
+        e[2] += r.factor_flops[keys[0]]["gemm"]
+    cands = {}
+    for key, (gms, cnt, fl) in g.items():
+        if gms > 0 and cnt:
+            peak = PEAK_FP32 if key == "c64" else PEAK_FP64
+            ach = fl / (gms * 1e-3) / 1e12
+            cands["gemm_" + key] = (gms, dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s",
+                                              kernel="gemm_batched_kernel (%s factorisations)" % key,
+                                              launches=cnt, avg_launch_ms=gms / cnt,
+                                              flops_per_launch=fl / cnt))
+    sk = ms.get("skinny", 0.0)
+    nsk = sum(r.kernel_launches.get("skinny", 0) for r in inst)
+    if sk > 0:
+        sb = sum(r.sweep_bytes_8d for r in inst)
+        cands["skinny"] = (sk, dict(bound="hbm", achieved=sb / (sk * 1e-3) / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
+                                    kernel="skinny + offsweep (solve sweep)", launches=nsk,
+                                    avg_launch_ms=sk / max(1, nsk)))
+    dom = max(cands, key=lambda k: cands[k][0])
+    rl = dict(cands[dom][1])
+    rl["frac"] = rl["achieved"] / rl["peak"]
+    rl["traffic"] = None           # HBM bytes per launch: rocprofv3 --pmc passes, committed under profiles/
+    rl["traffic_profile"] = "profiles/serving_gemm_c64_roofline.md"
+    rl["other_kernels"] = {k: dict(achieved=round(v[1]["achieved"], 3), unit=v[1]["unit"],
+                                   frac=round(v[1]["achieved"] / v[1]["peak"], 4), launches=v[1]["launches"],
+                                   avg_launch_ms=round(v[1]["avg_launch_ms"], 4))
+                           for k, v in cands.items() if k != dom}
+    rl.update(dominant=dom, class_device_ms={k: round(v, 3) for k, v in ms.items()},
+              class_share={k: round(v / tot, 3) for k, v in ms.items()} if tot > 0 else {},
+              class_flops={k: v for k, v in flops.items()},
+              sample="%d requests of the trace on the %s path, one slot, HIP events around every launch" % (
+                  len(inst), head_name),
+              factor_precisions=sorted(precs))
+    return rl
 
 
 def summarize_subset(results, first):
@@ -292,6 +347,36 @@ def summarize_subset(results, first):
     lat = np.array([r.latency for r in sub])
     return dict(requests=len(sub), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_service_s=float(np.mean([r.service for r in sub])), ok=int(sum(r.ok for r in sub)))
+
+
+def roofline_sample(args, trace, head_name):
+    """The instrumented pass behind the serving line's roofline: the first
+    --roofline-sample requests of the trace, all queued at once, one slot,
+    HIP events around every launch (outside the timed region).
+    `bench.py --roofline-only` runs this pass alone (the command profiled
+    with rocprofv3 for the committed kernel summaries)."""
+    import dataclasses
+    from kore_amd import backend, serving
+    hp = backend.PREC_C64 if head_name == "mixed" else backend.PREC_FP64
+    sample = [dataclasses.replace(r, arrival=0.0) for r in trace[:args.roofline_sample]]
+    # an untimed pass first: the tile tuner measures at the first
+    # factorisation of each block-size class, which is not the sample's work
+    srv = serving.Server(slots=1, precision=hp, block=args.block)
+    srv.run(sample)
+    srv.close()
+    srv = serving.Server(slots=1, precision=hp, block=args.block, instrument=True)
+    inst = srv.run(sample)
+    srv.close()
+    by_id = {q.rid: q for q in sample}
+    for r in inst:
+        # single-column solve sweeps at the block-Thomas bytes (one per OP column / GMRES step)
+        q = by_id[r.rid]
+        _, sizes, _ = backend.layout_blocks(backend.layout_from_params(q.params()))
+        r.sweep_bytes_8d = r.sweeps * solve_bytes_8d(sizes, 8 if r.precision else 16)
+    rl = serving_roofline(inst, head_name)
+    log("roofline: dominant %s, %.2f %s of %.1f (%.3f), %d launches of %.3f ms" % (
+        rl["dominant"], rl["achieved"], rl["unit"], rl["peak"], rl["frac"], rl["launches"], rl["avg_launch_ms"]))
+    return rl
 
 
 def run_serving(args):
@@ -311,10 +396,20 @@ def run_serving(args):
         sum(r.kind == "eigen" for r in trace)))
     at0 = lambda rs: [dataclasses.replace(r, arrival=0.0) for r in rs]
     paths_req = args.paths.split(",")
-    # warm-up: W requests per path served untimed (kernels loaded, memory pool filled)
+    # warm-up, untimed: W requests per path (kernels loaded, memory pool
+    # filled), then one request of each kind and block-size class of the
+    # trace per path, so that the tile tuner has measured every (precision,
+    # class) before the timed region
+    reps = {}
+    for r in trace:
+        _, sizes, _ = backend.layout_blocks(backend.layout_from_params(r.params()))
+        reps.setdefault(((int(max(sizes)) + 63) // 64, r.kind), r)
+    log("warm-up: %d requests, then %d of distinct kind and block-size class, per path" % (
+        max(1, args.warmup), len(reps)))
     for prec in (backend.PREC_FP64, backend.PREC_C64):
         srv = serving.Server(slots=args.slots, precision=prec, block=args.block)
         srv.run(at0(trace[:max(1, args.warmup)]))
+        srv.run(at0(list(reps.values())))
         srv.close()
     # capacity: a saturated pass (all requests queued at once) over the first
     # requests of the trace, per path
@@ -377,32 +472,7 @@ def run_serving(args):
     common = {name: summarize_subset(outs[name], prefix) for name in outs}
     # roofline: an instrumented pass (HIP events around every launch) over a
     # sample of the trace on the headline path; the dominant class by device time
-    hp = backend.PREC_C64 if head_name == "mixed" else backend.PREC_FP64
-    sample = at0(trace[:args.roofline_sample])
-    srv = serving.Server(slots=1, precision=hp, block=args.block, instrument=True)
-    inst = srv.run(sample)
-    srv.close()
-    ms, flops, precs = classes_from(inst)
-    tot = sum(ms.values())
-    dom = max(("inv", "gemm", "skinny"), key=lambda k: ms.get(k, 0.0))
-    peak = PEAK_FP32 if precs == {"c64"} else PEAK_FP64
-    if dom in ("inv", "gemm"):
-        ach = flops[dom] / (ms[dom] * 1e-3) / 1e12
-        rl = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, traffic=None,
-                  kernel="gj_inverse (panel + MFMA update)" if dom == "inv" else
-                  "gemm_batched + level-0 sparse products")
-    else:
-        sb = sum(r.solve_bytes * r.sweeps for r in inst)
-        ach = sb / (ms["skinny"] * 1e-3) / 1e9
-        rl = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=None,
-                  kernel="skinny (solve sweep)")
-    rl.update(dominant_class=dom, class_device_ms={k: round(v, 3) for k, v in ms.items()},
-              class_share={k: round(v / tot, 3) for k, v in ms.items()} if tot > 0 else {},
-              sample="%d requests of the trace on the %s path, one slot" % (len(sample), head_name),
-              factor_precisions=sorted(precs),
-              peak_note="FP64 matrix peak: the sample mixes complex64 (forced) and FP64 (eigen) factors"
-              if len(precs) > 1 else "")
-    log("roofline: dominant %s, %.2f %s of %.1f (%.3f)" % (dom, rl["achieved"], rl["unit"], rl["peak"], rl["frac"]))
+    rl = roofline_sample(args, trace, head_name)
     cpu = None
     if not args.no_cpu_baseline:
         req = next(r for r in trace if r.kind == "forced")
@@ -483,6 +553,8 @@ def main():
     ap.add_argument("--ncv", type=int, default=None)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the instrumented roofline pass of the serving line (the command profiled with rocprofv3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy, three times (scale for HBM counters)")
     args = ap.parse_args()
@@ -494,6 +566,14 @@ def main():
         sys.exit(2)
 
     if args.workload == "serving":
+        if args.roofline_only:
+            from kore_amd import serving
+            e_lo, e_hi = (float(v) for v in args.e_range.split(","))
+            trace = serving.serving_trace(nreq=args.requests, seed=args.seed, e_range=(e_lo, e_hi),
+                                          n_cap=args.n_cap)
+            rl = roofline_sample(args, trace, "mixed" if "mixed" in args.paths.split(",") else "fp64")
+            print(json.dumps({"roofline": rl}), flush=True)
+            return
         run_serving(args)
         return
     from kore_amd import assembly, backend, eigs

@@ -41,7 +41,8 @@ extern "C" {
 #define KORE_TIMER_SPMV 3     /* sparse products (B x, residuals)             */
 #define KORE_TIMER_ORTHO 4    /* Krylov workspace dot / gemm                  */
 #define KORE_TIMER_OTHER 5    /* block scatter, gathers, conversions          */
-#define KORE_TIMER_COUNT 6
+#define KORE_TIMER_SPARSE 6   /* level 0's products with the sparse couplings */
+#define KORE_TIMER_COUNT 7
 
 /* Which eigenvalues are wanted, in the order of bin/solve.py:99-117
  * (which_eigenpairs of bin/parameters.py): nearest the target in modulus,

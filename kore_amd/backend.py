@@ -25,8 +25,8 @@ KORE_WARN_NOT_CONVERGED = 3
 WHICH = {w: i for i, w in enumerate(["TM", "TR", "TI", "LM", "SM", "LR", "SR", "LI", "SI"])}
 PREC_FP64 = 0
 PREC_C64 = 1
-TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER = range(6)
-TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "ortho", "other"]
+TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER, TIMER_SPARSE = range(7)
+TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "ortho", "other", "sparse"]
 
 
 class KoreLibraryError(RuntimeError):

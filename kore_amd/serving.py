@@ -84,6 +84,7 @@ class Result:
     sweeps: int = 0            # single-column solve sweeps (OP applications, GMRES steps)
     solve_bytes: float = 0.0   # bytes one sweep reads (factor blocks + level-0 couplings)
     kernel_ms: dict = dataclasses.field(default_factory=dict)      # device ms per kernel class (instrumented runs)
+    kernel_launches: dict = dataclasses.field(default_factory=dict)  # launches per kernel class (instrumented runs)
 
 
 def serving_trace(nreq=1024, seed=1234, e_range=(1e-7, 1e-5), n_cap=500, forced_frac=0.7, rate=None, nev=6,
@@ -266,7 +267,9 @@ class Server:
                              sweeps=r.op_columns)
             out.solve_bytes = float(S.last_factor.solve_bytes)
             if self.instrument:
-                out.kernel_ms = {k: v[1] for k, v in ctx.timing().items()}
+                tm = ctx.timing()
+                out.kernel_ms = {k: v[1] for k, v in tm.items()}
+                out.kernel_launches = {k: v[0] for k, v in tm.items()}
         finally:
             S.close()
         now = time.perf_counter()
