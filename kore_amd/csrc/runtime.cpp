@@ -1822,8 +1822,7 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   // (each range into its own arrays, then concatenated in row order)
   struct Ent {
     int32_t col;
-    int32_t src;   // index of the A entry it came from (-1: a B entry)
-    zd a, b;
+    int32_t src;   // k >= 0: A entry k; k < 0: B entry -(k + 1)
   };
   const int64_t nnz_a = a_indptr[n];
   s->h_amap.assign((size_t)nnz_a, -1);
@@ -1857,9 +1856,45 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0), rowlo(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
+  // A row (or B row) in Kore order maps to degree order as a few increasing
+  // runs (one per section and degree), disjoint in their columns: ordering
+  // the runs by their first column sorts the row.  Rows whose runs overlap
+  // (unsorted input) fall back to an insertion sort.
+  auto sort_runs = [](std::vector<Ent>& in, std::vector<Ent>& out, std::vector<std::pair<int32_t, int32_t>>& runs) {
+    out.clear();
+    runs.clear();
+    const int32_t m = (int32_t)in.size();
+    for (int32_t q = 0; q < m;) {
+      int32_t e = q + 1;
+      while (e < m && in[e].col > in[e - 1].col) ++e;
+      runs.push_back({q, e});
+      q = e;
+    }
+    if (runs.size() > 1) {
+      std::sort(runs.begin(), runs.end(), [&](const std::pair<int32_t, int32_t>& x, const std::pair<int32_t, int32_t>& y) {
+        return in[x.first].col < in[y.first].col;
+      });
+      for (size_t k = 1; k < runs.size(); ++k)
+        if (in[runs[k - 1].second - 1].col >= in[runs[k].first].col) {   // overlapping runs
+          out = in;
+          for (size_t q = 1; q < out.size(); ++q) {
+            const Ent e = out[q];
+            size_t p = q;
+            while (p > 0 && out[p - 1].col > e.col) {
+              out[p] = out[p - 1];
+              --p;
+            }
+            out[p] = e;
+          }
+          return;
+        }
+    }
+    for (const auto& rg : runs) out.insert(out.end(), in.begin() + rg.first, in.begin() + rg.second);
+  };
   auto build_rows = [&](int t) {
     const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
-    std::vector<Ent> row;
+    std::vector<Ent> row, ra, rb, sa, sb;
+    std::vector<std::pair<int32_t, int32_t>> runs;
     int32_t* oix = pix + ub[t];
     zd* ova = pva + ub[t];
     zd* ovb = pvb + ub[t];
@@ -1868,25 +1903,22 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     size_t cntt = 0, cnttb = 0;
     for (int64_t i = r0; i < r1; ++i) {
       const int64_t r = s->perm[i];
-      row.clear();
-      for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k)
-        row.push_back({(int32_t)inv[a_indices[k]], k, zd(a_data[2 * (int64_t)k], a_data[2 * (int64_t)k + 1]), zd(0, 0)});
+      ra.clear();
+      rb.clear();
+      for (int32_t k = a_indptr[r]; k < a_indptr[r + 1]; ++k) ra.push_back({(int32_t)inv[a_indices[k]], k});
       if (s->has_b)
-        for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) {
-          const zd bv = b_is_complex ? zd(b_data[2 * (int64_t)k], b_data[2 * (int64_t)k + 1]) : zd(b_data[k], 0.0);
-          row.push_back({(int32_t)inv[b_indices[k]], -1, zd(0, 0), bv});
+        for (int32_t k = b_indptr[r]; k < b_indptr[r + 1]; ++k) rb.push_back({(int32_t)inv[b_indices[k]], -(k + 1)});
+      // each part sorted by its runs, then the two merged (A before B on equal
+      // columns); the buffers are reused from row to row
+      sort_runs(ra, sa, runs);
+      sort_runs(rb, sb, runs);
+      row.clear();
+      {
+        size_t p = 0, q = 0;
+        while (p < sa.size() || q < sb.size()) {
+          if (q == sb.size() || (p < sa.size() && sa[p].col <= sb[q].col)) row.push_back(sa[p++]);
+          else row.push_back(sb[q++]);
         }
-      // rows hold a few dozen entries in a few sorted runs: a stable insertion
-      // sort, no allocation per row
-      for (size_t q = 1; q < row.size(); ++q) {
-        if (row[q].col >= row[q - 1].col) continue;
-        const Ent e = row[q];
-        size_t p = q;
-        while (p > 0 && row[p - 1].col > e.col) {
-          row[p] = row[p - 1];
-          --p;
-        }
-        row[p] = e;
       }
       const int kb = s->h_blk[i];
       int32_t cnt = 0, cntb = 0;
@@ -1895,8 +1927,13 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
         const size_t q0 = q;
         zd a = 0, b = 0;
         while (q < row.size() && row[q].col == col) {
-          a += row[q].a;
-          b += row[q].b;
+          const int64_t k = row[q].src;
+          if (k >= 0) {
+            a += zd(a_data[2 * k], a_data[2 * k + 1]);
+          } else {
+            const int64_t kb = -(k + 1);
+            b += b_is_complex ? zd(b_data[2 * kb], b_data[2 * kb + 1]) : zd(b_data[kb], 0.0);
+          }
           ++q;
         }
         const int kc = s->h_blk[col];

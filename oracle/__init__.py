@@ -15,7 +15,6 @@ bcr           FP64 block cyclic reduction, the GPU factorisation's algorithm,
               used to check the device kernels level by level.
 krylov_schur  shift-and-invert Krylov-Schur (Stewart 2001), the SLEPc
               default EPS solver that bin/solve.py:91-123 configures.
-writers       the solve.py output files (bin/solve.py:160-199, 241-311).
 
 How each function is pinned is stated in its module header.
 """
