@@ -414,6 +414,12 @@ def run_serving(args):
     # capacity: a saturated pass (all requests queued at once) over the first
     # requests of the trace, per path
     cal = at0(trace[:min(args.calibrate, len(trace))])
+    # the operator cache (repeated structures: one axpy per request after two
+    # assemblies) is warmed on the calibration requests first, untimed, so
+    # that both paths' capacity passes see the same cache
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(6) as ex:
+        list(ex.map(serving.OPERATOR_CACHE.assemble, cal))
     capacity = {}
     for name, prec in (("fp64", backend.PREC_FP64), ("mixed", backend.PREC_C64)):
         if name not in paths_req:
@@ -449,10 +455,13 @@ def run_serving(args):
             if _done[0] % 64 == 0:
                 log("  %s: %d / %d served" % (_name, _done[0], _n))
         srv = serving.Server(slots=args.slots, precision=prec, block=args.block, log=progress)
+        h0, m0 = serving.OPERATOR_CACHE.hits, serving.OPERATOR_CACHE.misses
         t0 = time.perf_counter()
         out = srv.run(tr)
         wall = time.perf_counter() - t0
         summ = serving.summarize(out, wall)
+        summ.update(operator_cache=dict(hits=serving.OPERATOR_CACHE.hits - h0,
+                                        full_assemblies=serving.OPERATOR_CACHE.misses - m0))
         summ.update(max_inflight=srv.max_inflight, peak_predicted_bytes=srv.mem_peak,
                     failed=[r.rid for r in out if not r.ok][:10],
                     capacity_solves_per_s=capacity.get(name, {}).get("solves_per_s"),

@@ -624,9 +624,11 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
     for (int jj = 0; jj < CPT; ++jj) v[jj] = cmul(v[jj], myipv);
   }
   __syncthreads();
-  // C -> M[:, J]; R = M[P, :] on the outer panel's columns outside J
-  // (disjoint columns, so the order of the two does not matter; the update
-  // kernel has not run yet)
+  // C -> M[:, J]; R = M[P, :] on the outer panel's columns outside J, and
+  // those entries of M cleared (disjoint columns, so the order of the two
+  // does not matter; the update kernel has not run yet): the update adds
+  // C R to every row, which replaces the pivot rows and accumulates into the
+  // others
   if (live) {
 #pragma unroll
     for (int jj = 0; jj < CPT; ++jj) {
@@ -644,7 +646,11 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
       const int idx = base + t * NT + tid;
       const int k = idx % WP, j = q0 + idx / WP;
       x[t] = czero<T>();
-      if (idx < WP * wo && k < w && (j < p0 || j >= p0 + w)) x[t] = M[(int64_t)j * ld + sPiv[k]];
+      if (idx < WP * wo && k < w && (j < p0 || j >= p0 + w)) {
+        cx<T>* p = M + (int64_t)j * ld + sPiv[k];
+        x[t] = *p;
+        *p = czero<T>();                  // the update then replaces the pivot rows
+      }
     }
 #pragma unroll
     for (int t = 0; t < RB; ++t) {
@@ -656,99 +662,49 @@ __global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restr
   if (tid == 0 && d.info && (p0 == 0 || (sSing && *d.info == 0))) *d.info = sSing;
 }
 
-// One 64 x 64 tile of M per workgroup, 4 waves of 2 x 2 MFMA 16 x 16 tiles,
-// over the columns of the panel's outer panel outside the panel.
+// The panel's update: M[:, j] += M[:, J] R[:, j] for the columns j of its
+// outer panel outside J (pivot rows cleared by the panel kernel), on the
+// batched GEMM's tile with descriptors made here (columns [q0, p0), then
+// [p0 + w, q0 + wo)); K = w <= 32.
 template <class T>
-__global__ __launch_bounds__(256) void gj_update_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
-  using A4 = typename acc4<T>::type;
-  constexpr int WP = GJM_WP;
-  __shared__ cx<T> sA[WP][64];            // C tile, k-major
-  __shared__ cx<T> sB[64][WP + 1];        // R tile, j-major
-  __shared__ int sRole[64];
-  const InvDesc<T> d = descs[blockIdx.y];
-  const int n = d.n;
+__global__ __launch_bounds__(256, 3) void gj_update_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
+  __shared__ cx<T> sA[GT_K][GT_M];
+  __shared__ cx<T> sB[GT_N][GT_K + GT_PADB];
+  const InvDesc<T> v = descs[blockIdx.y];
+  const int n = v.n;
   if (p0 >= n) return;
   int q0, wo;
   gj_outer_range(n, p0, q0, wo);
-  const int ne = q0 + wo;                 // end of the outer panel's columns
-  const int nt = (n + 63) / 64;
-  if ((int)blockIdx.x >= nt * ((wo + 63) / 64)) return;
-  const int i0 = (blockIdx.x % nt) * 64, j0 = q0 + (blockIdx.x / nt) * 64;
-  const int w = min(WP, n - p0);
-  const int64_t ld = d.ld;
-  cx<T>* __restrict__ M = d.M;
-  const cx<T>* __restrict__ R = d.scratch;
-  const int* __restrict__ roles = reinterpret_cast<const int*>(d.scratch + (size_t)WP * n);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int l16 = lane & 15, lq = lane >> 4;
-  const bool rv0 = i0 + wr < n, rv1 = i0 + wr + 16 < n;    // sub-tiles inside the block
-  const bool cv0 = j0 + wc < ne, cv1 = j0 + wc + 16 < ne;
-  for (int idx = tid; idx < WP * 64; idx += 256) {
-    const int k = idx / 64, i = idx % 64;           // consecutive threads: consecutive rows
-    cx<T> x = czero<T>();
-    if (k < w && i0 + i < n) x = M[(int64_t)(p0 + k) * ld + i0 + i];
-    sA[k][i] = x;
+  const int w = min(GJM_WP, n - p0);
+  const int n0 = p0 - q0, n1 = q0 + wo - (p0 + w);
+  const int tm = (n + 63) / 64, tn0 = (n0 + 63) / 64, tn1 = (n1 + 63) / 64;
+  int t = blockIdx.x;
+  if (t >= tm * (tn0 + tn1)) return;      // uniform over the workgroup
+  int jstart = q0, ncols = n0;
+  if (t >= tm * tn0) {
+    t -= tm * tn0;
+    jstart = p0 + w;
+    ncols = n1;
   }
-  for (int idx = tid; idx < WP * 64; idx += 256) {
-    const int k = idx % WP, j = idx / WP;           // consecutive threads: consecutive k of R
-    cx<T> x = czero<T>();
-    if (j0 + j < ne) x = R[(int64_t)(j0 + j) * WP + k];  // zero beyond w (panel kernel)
-    sB[j][k] = x;
-  }
-  if (tid < 64) sRole[tid] = (i0 + tid < n) ? roles[i0 + tid] : -1;
-  __syncthreads();
-  A4 cre[2][2], cim[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      cre[a][b] = A4{0, 0, 0, 0};
-      cim[a][b] = A4{0, 0, 0, 0};
-    }
-#pragma unroll
-  for (int kk = 0; kk < WP; kk += 4) {
-    cx<T> a[2], b[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      a[q] = sA[kk + lq][wr + 16 * q + l16];
-      b[q] = sB[wc + 16 * q + l16][kk + lq];
-    }
-#pragma unroll
-    for (int rs = 0; rs < 2; ++rs)
-#pragma unroll
-      for (int cs = 0; cs < 2; ++cs)
-        if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
-  }
-  // epilogue: per row sub-tile, the old values are loaded together first
-  // (rows that are pivots of this panel are replaced, the others accumulate)
-#pragma unroll
-  for (int rs = 0; rs < 2; ++rs) {
-    const int li = wr + 16 * rs + l16;
-    const int gi = i0 + li;
-    const bool acc = !(sRole[li] >= p0 && sRole[li] < p0 + w);
-    cx<T> o[2][4];
-#pragma unroll
-    for (int cs = 0; cs < 2; ++cs)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, rr);
-        o[cs][rr] = czero<T>();
-        if (acc && gi < n && gj < ne && (gj < p0 || gj >= p0 + w)) o[cs][rr] = M[(int64_t)gj * ld + gi];
-      }
-#pragma unroll
-    for (int cs = 0; cs < 2; ++cs)
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, rr);
-        if (gi < n && gj < ne && (gj < p0 || gj >= p0 + w)) {
-          cx<T> x{cre[rs][cs][rr], cim[rs][cs][rr]};
-          x.re += o[cs][rr].re;
-          x.im += o[cs][rr].im;
-          M[(int64_t)gj * ld + gi] = x;
-        }
-      }
-  }
+  GemmDesc<T> d{};
+  d.A[0] = v.M + (int64_t)p0 * v.ld;
+  d.lda[0] = v.ld;
+  d.k[0] = w;
+  d.B[0] = v.scratch + (size_t)jstart * GJM_WP;
+  d.ldb[0] = GJM_WP;
+  d.A[1] = nullptr;
+  d.B[1] = nullptr;
+  d.k[1] = 0;
+  d.C = v.M + (int64_t)jstart * v.ld;
+  d.ldc = v.ld;
+  d.m = n;
+  d.n = ncols;
+  d.opA = 0;
+  d.tiles_m = tm;
+  d.tiles_n = (ncols + 63) / 64;
+  d.alpha = T(1);
+  d.beta = T(1);
+  gemm_tile_body<T>(d, t, sA, sB);
 }
 
 // phase 0: S[l n + k] = M[r_k, r^{-1}(l)];  phase 1: M <- S.   Grid: (column chunks, blocks).
@@ -863,9 +819,10 @@ static hipError_t launch_gj_inverse_multi(const InvDesc<T>* d_descs, int ndesc, 
       hipLaunchKernelGGL((gj_panel_kernel<T, 256>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
     else
       hipLaunchKernelGGL((gj_panel_kernel<T, 512>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
-    // column tiles of an outer panel: 2 (blocks of GJ_OUTER_MIN rows or
-    // more), else the whole block, at most 4 (blocks below 256 rows)
-    const int ntc = nmax >= GJ_OUTER_MIN ? (nt < 4 ? nt : 4) : nt;
+    // column tiles of the two ranges an update covers: at most 3 in an outer
+    // panel of 128 columns, at most nt + 1 in a block below GJ_OUTER_MIN rows
+    // (at most 5 when such a block shares a launch with larger ones)
+    const int ntc = nmax >= GJ_OUTER_MIN ? 5 : nt + 1;
     hipLaunchKernelGGL((gj_update_kernel<T>), dim3(nt * ntc, ndesc), dim3(256), 0, s, d_descs, p0);
     // the end of an outer panel (of every block whose outer panel holds p0)
     const int pe = p0 + GJM_WP;
