@@ -61,6 +61,7 @@ extern "C" {
 typedef struct kore_ctx kore_ctx;
 typedef struct kore_sys kore_sys;
 typedef struct kore_ws kore_ws;
+typedef struct kore_sys_plan kore_sys_plan;
 
 /* Field content and truncation of a Kore problem: the globals of
  * bin/parameters.py (m, lmax, symm, hydro, magnetic, thermal,
@@ -190,6 +191,19 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat,
                         const int32_t* a_indptr, const int32_t* a_indices, const double* a_data,
                         const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
                         int32_t b_is_complex, kore_sys** out);
+/* kore_sys_create in two steps: kore_sys_prepare does the host work (checks,
+ * the degree-order union pattern, the level-0 coupling views, values staged
+ * in pinned memory) and needs no context or device, so a serving loop can
+ * run it on its assembly threads ahead of a free device slot;
+ * kore_sys_create_prepared allocates the device buffers on ctx and uploads
+ * (the plan is used up; destroy it afterwards, or instead, with
+ * kore_sys_plan_destroy).  Replaces: the same set-up as kore_sys_create. */
+int32_t kore_sys_prepare(const kore_layout* lay, int64_t sizmat,
+                         const int32_t* a_indptr, const int32_t* a_indices, const double* a_data,
+                         const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
+                         int32_t b_is_complex, kore_sys_plan** out);
+int32_t kore_sys_create_prepared(kore_ctx* ctx, kore_sys_plan* plan, kore_sys** out);
+void kore_sys_plan_destroy(kore_sys_plan* plan);
 /* New values of A (a_data: interleaved complex, one per entry of the CSR
  * given to kore_sys_create, in that order; same pattern), uploaded into the
  * resident system; B, the layout and the device buffers stay.  The system

@@ -101,6 +101,10 @@ SIGNATURES = [
     ("kore_sys_create", _I32, [_P, C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PI32, _PI32, _PD, _I32,
                                C.POINTER(_P)]),
     ("kore_sys_destroy", None, [_P]),
+    ("kore_sys_prepare", _I32, [C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PI32, _PI32, _PD, _I32,
+                                C.POINTER(_P)]),
+    ("kore_sys_create_prepared", _I32, [_P, _P, C.POINTER(_P)]),
+    ("kore_sys_plan_destroy", None, [_P]),
     ("kore_sys_update_a", _I32, [_P, _PD]),
     ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
     ("kore_sys_query", _I32, [_P, C.POINTER(SysState)]),
@@ -299,33 +303,72 @@ def schur_reorder(S, Q, select):
     return S, Q, m.value
 
 
+def _csr_args(A, B):
+    """(n, nnz_a, keep-alive arrays, argument list) of the CSR pair for kore_sys_create / _prepare."""
+    A = ss.csr_matrix(A)
+    a = (A.indptr.astype(np.int32, copy=False), A.indices.astype(np.int32, copy=False),
+         np.ascontiguousarray(A.data.astype(np.complex128, copy=False)))
+    args = [a[0].ctypes.data_as(_PI32), a[1].ctypes.data_as(_PI32), _dptr(a[2])]
+    keep = [a]
+    if B is not None:
+        B = ss.csr_matrix(B)
+        bc = np.iscomplexobj(B.data)
+        b = (B.indptr.astype(np.int32, copy=False), B.indices.astype(np.int32, copy=False),
+             np.ascontiguousarray(B.data.astype(np.complex128 if bc else np.float64, copy=False)))
+        keep.append(b)
+        args += [b[0].ctypes.data_as(_PI32), b[1].ctypes.data_as(_PI32), b[2].ctypes.data_as(_PD), 1 if bc else 0]
+    else:
+        args += [None, None, None, 0]
+    return A.shape[0], int(A.nnz), keep, args
+
+
+class SystemPlan:
+    """The host half of a system's set-up (kore_sys_prepare): needs no
+    device, so it can run ahead of a free slot; System.from_plan uploads it."""
+
+    def __init__(self, lay: Layout, A, B=None):
+        n, nnz_a, keep, args = _csr_args(A, B)
+        p = _P()
+        _check(lib().kore_sys_prepare(C.byref(lay), n, *args, C.byref(p)))
+        self._p = p
+        self.n = n
+        self.nnz_a = nnz_a
+        self.lay = lay
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().kore_sys_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class System:
     """A + sigma B system resident on the device (Kore ordering on the host)."""
 
-    def __init__(self, ctx: Context, lay: Layout, A, B=None):
+    def __init__(self, ctx: Context, lay: Layout, A, B=None, plan: SystemPlan = None):
         L = lib()
-        A = ss.csr_matrix(A)
-        n = A.shape[0]
-        self.n = n
         self.ctx = ctx
         self.lay = lay
-        self._a = (A.indptr.astype(np.int32, copy=False), A.indices.astype(np.int32, copy=False),
-                   np.ascontiguousarray(A.data.astype(np.complex128, copy=False)))
-        args_b = [None, None, None, 0]
-        if B is not None:
-            B = ss.csr_matrix(B)
-            bc = np.iscomplexobj(B.data)
-            bdata = np.ascontiguousarray(B.data.astype(np.complex128 if bc else np.float64, copy=False))
-            self._b = (B.indptr.astype(np.int32, copy=False), B.indices.astype(np.int32, copy=False), bdata)
-            args_b = [self._b[0].ctypes.data_as(_PI32), self._b[1].ctypes.data_as(_PI32),
-                      self._b[2].ctypes.data_as(_PD), 1 if bc else 0]
         p = _P()
-        _check(L.kore_sys_create(ctx.handle, C.byref(lay), n, self._a[0].ctypes.data_as(_PI32),
-                                 self._a[1].ctypes.data_as(_PI32), _dptr(self._a[2]),
-                                 args_b[0], args_b[1], args_b[2], args_b[3], C.byref(p)))
+        if plan is not None:
+            if plan._p is None:
+                raise ValueError("the system plan was already used")
+            rc = L.kore_sys_create_prepared(ctx.handle, plan._p, C.byref(p))
+            plan.close()
+            _check(rc)
+            n, nnz_a = plan.n, plan.nnz_a
+        else:
+            n, nnz_a, keep, args = _csr_args(A, B)
+            _check(L.kore_sys_create(ctx.handle, C.byref(lay), n, *args, C.byref(p)))
+            del keep                           # the library keeps its own copies
+        self.n = n
         self._p = p
-        self.nnz_a = int(A.nnz)
-        self._a = self._b = None           # the library keeps its own copies
+        self.nnz_a = nnz_a
         self._children = weakref.WeakSet()
         ctx._children.add(self)
         self.sigma = None

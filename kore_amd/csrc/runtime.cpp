@@ -1776,14 +1776,28 @@ int64_t kore_factor_bytes(const kore_layout* lay, int32_t precision) {
   return precision == KORE_PREC_C64 ? factor_count_bytes<float>(sizes, n) : factor_count_bytes<double>(sizes, n);
 }
 
-int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr,
-                        const int32_t* a_indices, const double* a_data, const int32_t* b_indptr,
-                        const int32_t* b_indices, const double* b_data, int32_t b_is_complex, kore_sys** out) {
-  if (!ctx || !lay || !out || !a_indptr || !a_indices || !a_data) return fail(KORE_ERR_ARG, "NULL argument");
-  *out = nullptr;
-  HIP_TRY(hipSetDevice(ctx->device));
-  auto s = std::make_unique<kore_sys>();
-  s->ctx = ctx;
+}  // extern "C"
+
+// A system prepared on the host (kore_sys_prepare): the layout, the union
+// pattern and values staged in pinned memory, the level-0 coupling views;
+// kore_sys_create_prepared allocates the device buffers and uploads it.
+struct kore_sys_plan {
+  std::unique_ptr<kore_sys> s;
+  PinnedBuf arena;                        // staged pattern and values, by row range
+  int nth = 1;
+  std::vector<size_t> ub, ubb, tcnt, tcntb;
+  std::vector<int32_t> rowcntb, optr, onlo, ocol, cptr, cq, crow;
+  zd *pva = nullptr, *pvb = nullptr, *pvbb = nullptr;
+  int32_t *pix = nullptr, *pixb = nullptr;
+};
+
+static int32_t sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr, const int32_t* a_indices,
+                           const double* a_data, const int32_t* b_indptr, const int32_t* b_indices,
+                           const double* b_data, int32_t b_is_complex, std::unique_ptr<kore_sys_plan>& P) {
+  if (!lay || !a_indptr || !a_indices || !a_data) return fail(KORE_ERR_ARG, "NULL argument");
+  P = std::make_unique<kore_sys_plan>();
+  P->s = std::make_unique<kore_sys>();
+  auto& s = P->s;
   s->lay = *lay;
   int64_t n = 0;
   StageTrace trace;
@@ -1830,7 +1844,10 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
                                                                std::max<int64_t>(1, n / 4096)}));
   // per-range upper bounds (entries of A plus B on the range's rows) place
   // each range's output in the pinned arena before anything is built
-  std::vector<size_t> ub(nth + 1, 0), ubb(nth + 1, 0);
+  std::vector<size_t>& ub = P->ub;
+  std::vector<size_t>& ubb = P->ubb;
+  ub.assign(nth + 1, 0);
+  ubb.assign(nth + 1, 0);
   for (int t = 0; t < nth; ++t) {
     const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
     size_t u = 0, ubv = 0;
@@ -1843,17 +1860,28 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     ubb[t + 1] = ubb[t] + ubv;
   }
   const size_t cap = std::max<size_t>(ub[nth], 1), capb = std::max<size_t>(ubb[nth], 1);
-  char* arena = nullptr;
-  TRY(ctx_pinned(ctx, cap * (sizeof(int32_t) + 2 * sizeof(zd)) + capb * (sizeof(int32_t) + sizeof(zd)) + 512,
-                 &arena));
+  TRY(P->arena.reserve(cap * (sizeof(int32_t) + 2 * sizeof(zd)) + capb * (sizeof(int32_t) + sizeof(zd)) + 512,
+                       nullptr));
+  char* arena = P->arena.p;
   zd* pva = reinterpret_cast<zd*>(arena);
   zd* pvb = pva + cap;
   zd* pvbb = pvb + cap;                     // B alone
   int32_t* pix = reinterpret_cast<int32_t*>(pvbb + capb);
   int32_t* pixb = pix + cap;
-  std::vector<size_t> tcnt(nth, 0), tcntb(nth, 0);
+  P->pva = pva;
+  P->pvb = pvb;
+  P->pvbb = pvbb;
+  P->pix = pix;
+  P->pixb = pixb;
+  P->nth = nth;
+  std::vector<size_t>& tcnt = P->tcnt;
+  std::vector<size_t>& tcntb = P->tcntb;
+  tcnt.assign(nth, 0);
+  tcntb.assign(nth, 0);
   std::vector<std::vector<int32_t>> tocol(nth);   // off-block columns of each range, in row order
-  std::vector<int32_t> rowcnt(n, 0), rowcntb(n, 0), rowoff(n, 0), rowlo(n, 0);
+  std::vector<int32_t> rowcnt(n, 0), rowoff(n, 0), rowlo(n, 0);
+  std::vector<int32_t>& rowcntb = P->rowcntb;
+  rowcntb.assign(n, 0);
   std::vector<int64_t> bad_row(nth, -1);
   std::vector<int> bad_kb(nth, 0), bad_kc(nth, 0);
   // A row (or B row) in Kore order maps to degree order as a few increasing
@@ -2014,7 +2042,11 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   }
   // compact off-block CSR: rows in degree order, each row's previous-block
   // entries first (rowlo of them), then its next-block entries
-  std::vector<int32_t> optr(n + 1, 0), onlo(n, 0), ocol;
+  std::vector<int32_t>& optr = P->optr;
+  std::vector<int32_t>& onlo = P->onlo;
+  std::vector<int32_t>& ocol = P->ocol;
+  optr.assign(n + 1, 0);
+  onlo.assign(n, 0);
   for (int64_t i = 0; i < n; ++i) {
     optr[i + 1] = optr[i] + rowoff[i];
     onlo[i] = rowlo[i];
@@ -2043,8 +2075,12 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     }
   }
   if (ne != s->nnz_off) return fail(KORE_ERR_ARG, "internal: off-block side counts mismatch");
-  std::vector<int32_t> cptr((size_t)std::max<int64_t>(ncp, 1), 0), cq((size_t)std::max<int64_t>(ne, 1)),
-      crow((size_t)std::max<int64_t>(ne, 1));
+  std::vector<int32_t>& cptr = P->cptr;
+  std::vector<int32_t>& cq = P->cq;
+  std::vector<int32_t>& crow = P->crow;
+  cptr.assign((size_t)std::max<int64_t>(ncp, 1), 0);
+  cq.assign((size_t)std::max<int64_t>(ne, 1), 0);
+  crow.assign((size_t)std::max<int64_t>(ne, 1), 0);
   {
     auto fill_blocks = [&](int k0, int k1) {
       std::vector<int32_t> fillp;
@@ -2079,6 +2115,32 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
     for (auto& th : pool) th.join();
   }
   trace.mark("sys_create off-block views");
+  return KORE_OK;
+}
+
+// Device buffers and upload of a prepared system on ctx's stream.
+static int32_t sys_upload(kore_ctx* ctx, kore_sys_plan* P, kore_sys** out) {
+  StageTrace trace;
+  std::unique_ptr<kore_sys>& s = P->s;
+  s->ctx = ctx;
+  const int64_t n = s->n;
+  const int nth = P->nth;
+  const auto& tcnt = P->tcnt;
+  const auto& tcntb = P->tcntb;
+  const auto& ub = P->ub;
+  const auto& ubb = P->ubb;
+  const auto& rowcntb = P->rowcntb;
+  const auto& optr = P->optr;
+  const auto& onlo = P->onlo;
+  const auto& ocol = P->ocol;
+  const auto& cptr = P->cptr;
+  const auto& cq = P->cq;
+  const auto& crow = P->crow;
+  const zd* pva = P->pva;
+  const zd* pvb = P->pvb;
+  const zd* pvbb = P->pvbb;
+  const int32_t* pix = P->pix;
+  const int32_t* pixb = P->pixb;
   const size_t nnz = (size_t)s->h_ip[n];
   s->nnz = (int64_t)nnz;
   TRY(s->d_ip.alloc(ctx, (n + 1) * sizeof(int32_t)));
@@ -2166,6 +2228,45 @@ int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, c
   trace.mark("sys_create upload");
   *out = s.release();
   return KORE_OK;
+}
+
+extern "C" {
+
+int32_t kore_sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr, const int32_t* a_indices,
+                         const double* a_data, const int32_t* b_indptr, const int32_t* b_indices,
+                         const double* b_data, int32_t b_is_complex, kore_sys_plan** out) {
+  if (!out) return fail(KORE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  std::unique_ptr<kore_sys_plan> P;
+  TRY(sys_prepare(lay, sizmat, a_indptr, a_indices, a_data, b_indptr, b_indices, b_data, b_is_complex, P));
+  *out = P.release();
+  return KORE_OK;
+}
+
+int32_t kore_sys_create_prepared(kore_ctx* ctx, kore_sys_plan* plan, kore_sys** out) {
+  if (!ctx || !plan || !out || !plan->s) return fail(KORE_ERR_ARG, "NULL argument or a plan already used");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int32_t rc = sys_upload(ctx, plan, out);
+  if (rc < 0) hipStreamSynchronize(ctx->stream);   // no copy may still read the staging
+  plan->arena.release();
+  return rc;
+}
+
+void kore_sys_plan_destroy(kore_sys_plan* plan) { delete plan; }
+
+int32_t kore_sys_create(kore_ctx* ctx, const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr,
+                        const int32_t* a_indices, const double* a_data, const int32_t* b_indptr,
+                        const int32_t* b_indices, const double* b_data, int32_t b_is_complex, kore_sys** out) {
+  if (!ctx || !out) return fail(KORE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::unique_ptr<kore_sys_plan> P;
+  TRY(sys_prepare(lay, sizmat, a_indptr, a_indices, a_data, b_indptr, b_indices, b_data, b_is_complex, P));
+  const int32_t rc = sys_upload(ctx, P.get(), out);
+  if (rc < 0) hipStreamSynchronize(ctx->stream);   // no copy may still read the staging
+  P->arena.release();
+  return rc;
 }
 
 int32_t kore_sys_update_a(kore_sys* s, const double* a_data) {

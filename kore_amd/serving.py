@@ -13,7 +13,11 @@ which are bound by their many solve sweeps, factor in FP64 from the start.
 
 Pipeline:
 
-* CPU assembly workers build (A0, B0) for queued requests ahead of the GPU;
+* at its arrival, a request's host work runs on the CPU assembly threads:
+  its operators (A0, B0), from the operator cache when its structure repeats,
+  and the host half of its system's set-up (kore_sys_prepare), so that a slot
+  only uploads and computes; at most ``lookahead`` requests are prepared and
+  not yet dispatched;
 * admission: a request enters a device slot when a slot is free and its
   predicted device footprint (library estimate of the factor bytes plus the
   system and Krylov workspace) fits in the HBM budget;
@@ -234,7 +238,8 @@ class Server:
 
     def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
-                 gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE):
+                 gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE,
+                 prepare_systems=True, lookahead=None):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -261,8 +266,11 @@ class Server:
         self._cv = threading.Condition()
         self.asm_pool = ThreadPoolExecutor(max_workers=assembly_workers)
         # operators of repeated structures from the process-wide cache (None:
-        # every request assembled in full)
+        # every request assembled in full); the host half of each system's
+        # set-up on the assembly threads too (prepare_systems)
         self.operator_cache = operator_cache
+        self.prepare_systems = prepare_systems
+        self.lookahead = lookahead if lookahead is not None else assembly_workers + slots
 
     def close(self):
         self.asm_pool.shutdown(wait=True)
@@ -270,13 +278,22 @@ class Server:
             c.close()
 
     # -- one request on one slot ------------------------------------------------
-    def _execute(self, slot, req: Request, par, A0, B0, t0) -> Result:
+    def _prepare(self, req: Request):
+        """Host work of a request, on the assembly threads ahead of a free
+        slot: its operators (from the cache when the structure repeats) and
+        the host half of its system's set-up (kore_sys_prepare)."""
+        par, A0, B0 = (self.operator_cache.assemble(req) if self.operator_cache is not None
+                       else assemble_request(req))
+        plan = backend.SystemPlan(backend.layout_from_params(par), A0, B0) if self.prepare_systems else None
+        return par, A0, B0, plan
+
+    def _execute(self, slot, req: Request, par, A0, B0, t0, plan=None) -> Result:
         ctx = self.ctxs[slot]
         t_start = time.perf_counter()
         st = {}
         flops = {}
         lay = backend.layout_from_params(par)
-        S = backend.System(ctx, lay, A0, B0)
+        S = backend.System(ctx, lay, A0, B0, plan=plan)
         if self.instrument:
             ctx.timing_reset()
 
@@ -356,16 +373,31 @@ class Server:
         t0 = time.perf_counter()
         results = {}
         lock = threading.Lock()
-        # assembly runs ahead: submit in arrival order
+        order = sorted(trace, key=lambda r: r.arrival)
+        # a request's host work (operators, system plan) starts at its arrival,
+        # on the assembly threads; at most `lookahead` requests are prepared and
+        # not yet dispatched (each holds its operators and staged system)
         futs = {}
-        for req in sorted(trace, key=lambda r: r.arrival):
-            futs[req.rid] = self.asm_pool.submit(
-                self.operator_cache.assemble if self.operator_cache is not None else assemble_request, req)
+        ready = {r.rid: threading.Event() for r in order}
+        window = threading.Semaphore(self.lookahead)
+
+        def submitter():
+            for req in order:
+                dt = t0 + req.arrival - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                window.acquire()
+                with lock:
+                    futs[req.rid] = self.asm_pool.submit(self._prepare, req)
+                ready[req.rid].set()
+
+        sub = threading.Thread(target=submitter, daemon=True)
+        sub.start()
         threads = []
 
-        def worker(slot, req, par, A0, B0, need):
+        def worker(slot, req, par, A0, B0, need, plan):
             try:
-                res = self._execute(slot, req, par, A0, B0, t0)
+                res = self._execute(slot, req, par, A0, B0, t0, plan)
             except Exception as e:  # report, keep serving
                 res = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0, float("inf"), False,
                              error=repr(e))
@@ -380,18 +412,22 @@ class Server:
                 self.log("req %d %s done: latency %.3f s, service %.3f s, residual %.1e" % (
                     req.rid, req.kind, res.latency, res.service, res.residual))
 
-        for req in sorted(trace, key=lambda r: r.arrival):
-            # wait for the arrival time
-            dt = t0 + req.arrival - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            par, A0, B0 = futs[req.rid].result()
+        for req in order:
+            ready[req.rid].wait()               # arrived and submitted
+            with lock:
+                fut = futs.pop(req.rid)
+            try:
+                par, A0, B0, plan = fut.result()
+            finally:
+                window.release()
             # FP64 bytes even on the mixed path: eigensolves and refinement
             # fallbacks hold an FP64 factor
             need = footprint(par, backend.PREC_FP64, req.nev, self.block)
             if need > self.mem_budget:
                 results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
                                           error="request needs %d bytes, budget %d" % (need, self.mem_budget))
+                if plan is not None:
+                    plan.close()
                 continue
             with self._cv:
                 while self.free_slots.empty() or self.mem_in_use + need > self.mem_budget:
@@ -401,10 +437,10 @@ class Server:
                 self.mem_peak = max(self.mem_peak, self.mem_in_use)
                 self._inflight += 1
                 self.max_inflight = max(self.max_inflight, self._inflight)
-            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need), daemon=True)
+            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan), daemon=True)
             th.start()
             threads.append(th)
-            futs.pop(req.rid, None)
+        sub.join()
         for th in threads:
             th.join()
         return [results[r.rid] for r in trace]
