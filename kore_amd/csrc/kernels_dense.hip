@@ -39,13 +39,22 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
     // multiples of 64, e.g. 200 = 3*64 + 8) are skipped, wave-uniformly
     const bool rv0 = i0 + wr < d.m, rv1 = i0 + wr + 16 < d.m;
     const bool cv0 = j0 + wc < d.n, cv1 = j0 + wc + 16 < d.n;
-    A4 cre[2][2], cim[2][2];
+    // complex128: cre, cim hold the real and imaginary parts (four real
+    // products per complex step).  complex64: three real products per
+    // complex step (Karatsuba's 3M form): cre = sum ar br, cim = sum ai bi,
+    // csum = sum (ar + ai)(br + bi), and at the end re = cre - cim,
+    // im = csum - cre - cim; 3/4 of the matrix-core work, with an error bound
+    // of the same order for the complex64 factor (a preconditioner whose
+    // results FP64 refinement or GMRES corrects).
+    constexpr bool k3m = sizeof(T) == 4;
+    A4 cre[2][2], cim[2][2], csum[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         cre[a][b] = A4{0, 0, 0, 0};
         cim[a][b] = A4{0, 0, 0, 0};
+        csum[a][b] = A4{0, 0, 0, 0};
       }
 
     // Slabs of 16 k-columns over the (up to two) products, register
@@ -119,11 +128,29 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
           a[s] = sA[kk + lq][wr + 16 * s + l16];
           b[s] = sB[wc + 16 * s + l16][kk + lq];
         }
+        if constexpr (k3m) {
+          T as[2], bs[2];
 #pragma unroll
-        for (int rs = 0; rs < 2; ++rs)
+          for (int q = 0; q < 2; ++q) {
+            as[q] = a[q].re + a[q].im;
+            bs[q] = b[q].re + b[q].im;
+          }
 #pragma unroll
-          for (int cs = 0; cs < 2; ++cs)
-            if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
+          for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+            for (int cs = 0; cs < 2; ++cs)
+              if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) {
+                cre[rs][cs] = mfma16(b[cs].re, a[rs].re, cre[rs][cs]);
+                cim[rs][cs] = mfma16(b[cs].im, a[rs].im, cim[rs][cs]);
+                csum[rs][cs] = mfma16(bs[cs], as[rs], csum[rs][cs]);
+              }
+        } else {
+#pragma unroll
+          for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+            for (int cs = 0; cs < 2; ++cs)
+              if ((rs ? rv1 : rv0) && (cs ? cv1 : cv0)) cmma<T>(cre[rs][cs], cim[rs][cs], a[rs], b[cs]);
+        }
       }
       p = pn;
       k0 = kn;
@@ -152,7 +179,12 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
         for (int r = 0; r < 4; ++r) {
           const int gj = j0 + wc + 16 * cs + mfma_mrow<T>(lq, r);
           if (gi < d.m && gj < d.n) {
-            cx<T> v{d.alpha * cre[rs][cs][r], d.alpha * cim[rs][cs][r]};
+            T vre = cre[rs][cs][r], vim = cim[rs][cs][r];
+            if constexpr (k3m) {
+              vre = cre[rs][cs][r] - cim[rs][cs][r];
+              vim = csum[rs][cs][r] - cre[rs][cs][r] - cim[rs][cs][r];
+            }
+            cx<T> v{d.alpha * vre, d.alpha * vim};
             if (acc) {
               v.re += d.beta * o[cs][r].re;
               v.im += d.beta * o[cs][r].im;
