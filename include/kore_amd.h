@@ -35,6 +35,12 @@ extern "C" {
 #define KORE_PREC_FP64 0   /* complex128 factor */
 #define KORE_PREC_C64 1    /* complex64 factor (FP32 MFMA); FP64 refinement in kore_solve */
 
+/* Form of a factorisation (kore_sys_set_factor_form).  Both hold the same
+ * inverse blocks and give the same solutions to rounding. */
+#define KORE_FORM_AUTO 0    /* LEAN from degree blocks of 320 unknowns, SWEEP below */
+#define KORE_FORM_SWEEP 1   /* keeps E = L Dinv, F = U Dinv: two launches per level in a solve sweep */
+#define KORE_FORM_LEAN 2    /* no E, F: about a quarter fewer GEMM flops, three launches per level */
+
 #define KORE_TIMER_INV 0      /* block inversions of the factorisation        */
 #define KORE_TIMER_GEMM 1     /* Schur-complement GEMMs of the factorisation  */
 #define KORE_TIMER_SKINNY 2   /* solve sweeps (one event pair per kernel)     */
@@ -90,6 +96,7 @@ typedef struct {
   int32_t solve_launches;  /* kernel launches of one single-column solve sweep */
   int64_t bytes;           /* device bytes held by the factor */
   double ms;               /* device time, HIP events on the library stream */
+  int32_t form;            /* KORE_FORM_SWEEP or KORE_FORM_LEAN */
 } kore_factor_info;
 
 /* State of a system (kore_sys_query). */
@@ -217,6 +224,12 @@ int32_t kore_sys_update_a(kore_sys* sys, const double* a_data);
 void kore_sys_destroy(kore_sys* sys);
 /* Size and factorisation state of a system. */
 int32_t kore_sys_query(const kore_sys* sys, kore_sys_state* out);
+/* Form of the system's later factorisations (KORE_FORM_*; default AUTO); a
+ * held factorisation of another form is rebuilt by the next kore_factor.
+ * No counterpart in the reference (MUMPS chooses its own factor layout):
+ * for callers that know whether a factor serves few sweeps (a forced
+ * problem: LEAN) or many (an eigensolve of small blocks: SWEEP). */
+int32_t kore_sys_set_factor_form(kore_sys* sys, int32_t form);
 /* ||A - sigma B||_F of the stored (unscaled) matrices. */
 int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
 

@@ -25,6 +25,7 @@ KORE_WARN_NOT_CONVERGED = 3
 WHICH = {w: i for i, w in enumerate(["TM", "TR", "TI", "LM", "SM", "LR", "SR", "LI", "SI"])}
 PREC_FP64 = 0
 PREC_C64 = 1
+FORM_AUTO, FORM_SWEEP, FORM_LEAN = range(3)
 TIMER_INV, TIMER_GEMM, TIMER_SKINNY, TIMER_SPMV, TIMER_ORTHO, TIMER_OTHER, TIMER_SPARSE = range(7)
 TIMER_NAMES = ["inv", "gemm", "skinny", "spmv", "ortho", "other", "sparse"]
 
@@ -52,7 +53,7 @@ class FactorInfo(C.Structure):
     _fields_ = [("levels", C.c_int32), ("nblk", C.c_int32), ("flops", C.c_double),
                 ("flops_inv", C.c_double), ("flops_gemm", C.c_double), ("flops_sparse", C.c_double),
                 ("solve_bytes", C.c_double),
-                ("solve_launches", C.c_int32), ("bytes", C.c_int64), ("ms", C.c_double)]
+                ("solve_launches", C.c_int32), ("bytes", C.c_int64), ("ms", C.c_double), ("form", C.c_int32)]
 
 
 class SysState(C.Structure):
@@ -107,6 +108,7 @@ SIGNATURES = [
     ("kore_sys_plan_destroy", None, [_P]),
     ("kore_sys_update_a", _I32, [_P, _PD]),
     ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
+    ("kore_sys_set_factor_form", _I32, [_P, _I32]),
     ("kore_sys_query", _I32, [_P, C.POINTER(SysState)]),
     ("kore_factor_async", _I32, [_P, _D, _D, _I32, _P]),
     ("kore_factor_check", _I32, [_P]),
@@ -387,6 +389,11 @@ class System:
             raise ValueError("expected %d values of A, got %s" % (self.nnz_a, data.shape))
         _check(lib().kore_sys_update_a(self._p, _dptr(data)))
         self.sigma = None
+
+    def set_factor_form(self, form):
+        """Form of the later factorisations (FORM_AUTO, FORM_SWEEP, FORM_LEAN;
+        kore_sys_set_factor_form)."""
+        _check(lib().kore_sys_set_factor_form(self._p, int(form)))
 
     def norm(self, sigma=0j) -> float:
         out = C.c_double(0)

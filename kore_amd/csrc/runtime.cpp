@@ -527,6 +527,9 @@ struct FactorBase {
   int solve_launches = 0;
   int64_t bytes = 0;
   int nlev = 0;
+  // lean form: no E = L Dinv, F = U Dinv products; the kept blocks are
+  // updated through G, H and a sweep's forward level takes two launches
+  bool lean = false;
 };
 
 template <class T> struct Level {
@@ -538,6 +541,8 @@ template <class T> struct Level {
   int ng2 = 0, t2 = 0;
   SkinnyDesc<T>* fw = nullptr;
   int nfw = 0, mfw = 0;
+  SkinnyDesc<T>* fy = nullptr;          // lean form: y_odd = Dinv_odd b_odd before fw
+  int nfy = 0, mfy = 0;
   SkinnyDesc<T>* bw = nullptr;
   int nbw = 0, mbw = 0;
   // level 0 through the sparse couplings
@@ -633,6 +638,7 @@ struct kore_sys {
   cx<double>* st = nullptr;
   double* part = nullptr;
   int npart = 64;
+  int32_t form = KORE_FORM_AUTO;        // kore_sys_set_factor_form
   std::unique_ptr<FactorBase> f;
   std::unique_ptr<FactorBase> fspare;   // the plan of the other precision, kept for the next switch
   bool factored = false;     // f holds a valid factorisation of A - sigma B
@@ -672,6 +678,8 @@ struct kore_ws {
 };
 
 static constexpr int MAXR = 16;
+// degree-block size from which KORE_FORM_AUTO factors in the lean form
+static constexpr int LEAN_AUTO_MIN = 320;
 static constexpr int DOT_MAXCHUNK = 1024;   // split-K chunks of the workspace inner products
 static constexpr int DOT_ROWS = 64;        // minimum rows per chunk
 
@@ -688,7 +696,7 @@ struct FactorPlan {
   int64_t cntD = 0, cntDLU = 0, cnt = 0;
 };
 
-static void factor_plan(const std::vector<int>& bsz, FactorPlan& P, bool allow_sparse0) {
+static void factor_plan(const std::vector<int>& bsz, FactorPlan& P, bool allow_sparse0, bool lean) {
   const int nb = (int)bsz.size();
   int nmax = 0;
   for (int b : bsz) nmax = std::max(nmax, b);
@@ -744,7 +752,11 @@ static void factor_plan(const std::vector<int>& bsz, FactorPlan& P, bool allow_s
     P.offE[l].assign(T_, -1);
     P.offF[l].assign(T_, -1);
     if (l == 0 && P.sp0) continue;
-    for (int t = 0; t < T_; t += 2) {
+    // sweep form: E, F of the kept (even) positions.  Lean form: the
+    // couplings L, U of every position of the levels after the first one on
+    // dense couplings (that one reads them from the per-block L, U)
+    if (lean && l == (P.sp0 ? 1 : 0)) continue;
+    for (int t = 0; t < T_; t += lean ? 1 : 2) {
       if (t - 1 >= 0) {
         P.offE[l][t] = cnt;
         cnt += nk(id[t]) * nk(id[t - 1]);
@@ -763,7 +775,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   const int nb = s->nblk, nmax = s->nmax;
   F->nblk = nb;
   FactorPlan P;
-  factor_plan(s->bsz, P, s->nnz_off > 0);
+  factor_plan(s->bsz, P, s->nnz_off > 0, F->lean);
   const auto& ids = P.ids;
   const int nlev = P.nlev;
   F->nlev = nlev;
@@ -808,7 +820,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
   struct Rng {
     size_t off, n;
   };
-  std::vector<Rng> rinv(nlev + 1), rg1(nlev), rg2(nlev), rfw(nlev), rbw(nlev);
+  std::vector<Rng> rinv(nlev + 1), rg1(nlev), rg2(nlev), rfw(nlev), rbw(nlev), rfy(nlev, Rng{0, 0});
   F->lev.resize(nlev);
   double flops = 0;
   auto tiles = [](int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); };
@@ -876,6 +888,20 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     d.C = C; d.ldc = nk(k); d.row0 = s->boff[k]; d.m = (int)nk(k); d.n = n; d.alpha = alpha; d.beta = beta;
     return d;
   };
+  // Couplings of level l in the lean form: the first level on dense
+  // couplings reads them where the scatter (or level 0's sparse products)
+  // left them, in L, U; every later level in the plan's E, F slots of that
+  // level, which have the couplings' shapes and are otherwise unused there.
+  const bool lean = F->lean;
+  const int l0 = sp0 ? 1 : 0;
+  bool slot_missing = false;
+  auto slot = [&](cx<T>* blk, int64_t off, int l) -> cx<T>* {
+    cx<T>* p = l == l0 ? blk : (off >= 0 ? base + off : nullptr);
+    if (!p) slot_missing = true;
+    return p;
+  };
+  auto Lp = [&](int l, int t) { return slot(F->L[ids[l][t]], P.offE[l][t], l); };
+  auto Up = [&](int l, int t) { return slot(F->U[ids[l][t]], P.offF[l][t], l); };
   for (int l = 0; l < nlev; ++l) {
     const auto& id = ids[l];
     const int T_ = (int)id.size();
@@ -930,6 +956,102 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
       rg2[l] = Rng{vg.size(), 0};
       rfw[l] = Rng{vs.size(), 0};
       rbw[l] = Rng{vs.size(), 0};
+      continue;
+    }
+    if (lean) {
+      // Lean form (no E, F): with G_t = Dinv_t L_t, H_t = Dinv_t U_t of the
+      // eliminated blocks,
+      //   kept t:  D_t -= L_t H_{t-1} + U_t G_{t+1},
+      //            L_t' = -L_t G_{t-1},  U_t' = -U_t H_{t+1}   (into level l+1's slots)
+      // and a sweep's forward level runs y_odd = Dinv_odd b_odd, then
+      // b_even -= L y_{t-1} + U y_{t+1}; backward x_odd = y_odd - G x - H x.
+      rg1[l].off = vg.size();
+      for (int t = 1; t < T_; t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        const int kp = id[t - 1];
+        vg.push_back(mkgemm(F->G[k], m, (int)nk(kp), F->D[k], Lp(l, t), m, m, m, nullptr, nullptr, 0, 0, 0, T(1),
+                            T(0)));
+        L.t1 = std::max(L.t1, tiles(m, (int)nk(kp)));
+        if (t + 1 <= T_ - 1) {
+          const int kn = id[t + 1];
+          vg.push_back(mkgemm(F->H[k], m, (int)nk(kn), F->D[k], Up(l, t), m, m, m, nullptr, nullptr, 0, 0, 0, T(1),
+                              T(0)));
+          L.t1 = std::max(L.t1, tiles(m, (int)nk(kn)));
+        }
+      }
+      if (l == nlev - 1) {
+        // the two-block tail's F0 = U_0 Dinv_1, kept in H of the root block
+        // (the root block is never eliminated, so its H is free)
+        const int k0 = id[0], k1 = id[1];
+        const int n0 = (int)nk(k0), n1 = (int)nk(k1);
+        vg.push_back(mkgemm(F->H[k0], n0, n1, Up(l, 0), F->D[k1], n1, n0, n1, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
+        L.t1 = std::max(L.t1, tiles(n0, n1));
+      }
+      rg1[l].n = vg.size() - rg1[l].off;
+      rg2[l].off = vg.size();
+      for (int t = 0; t < T_; t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
+        if (hasE || hasF) {
+          const int kp = hasE ? id[t - 1] : -1, kn = hasF ? id[t + 1] : -1;
+          vg.push_back(mkgemm(F->D[k], m, m, hasE ? Lp(l, t) : nullptr, hasE ? F->H[kp] : nullptr,
+                              hasE ? (int)nk(kp) : 0, m, hasE ? (int)nk(kp) : 0, hasF ? Up(l, t) : nullptr,
+                              hasF ? F->G[kn] : nullptr, hasF ? (int)nk(kn) : 0, m, hasF ? (int)nk(kn) : 0, T(-1),
+                              T(1)));
+          L.t2 = std::max(L.t2, tiles(m, m));
+        }
+        if (t - 2 >= 0) {
+          const int kp = id[t - 1], kpp = id[t - 2];
+          vg.push_back(mkgemm(Lp(l + 1, t / 2), m, (int)nk(kpp), Lp(l, t), F->G[kp], (int)nk(kp), m, (int)nk(kp),
+                              nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
+          L.t2 = std::max(L.t2, tiles(m, (int)nk(kpp)));
+        }
+        if (t + 2 <= T_ - 1) {
+          const int kn = id[t + 1], knn = id[t + 2];
+          vg.push_back(mkgemm(Up(l + 1, t / 2), m, (int)nk(knn), Up(l, t), F->H[kn], (int)nk(kn), m, (int)nk(kn),
+                              nullptr, nullptr, 0, 0, 0, T(-1), T(0)));
+          L.t2 = std::max(L.t2, tiles(m, (int)nk(knn)));
+        }
+      }
+      rg2[l].n = vg.size() - rg2[l].off;
+      rfy[l].off = vs.size();
+      for (int t = 1; t < T_; t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        vs.push_back(mkskin(vecp(F->Xv, k), m, {Prod{F->D[k], m, vecp(F->Bv, k), m, T(1)}}, T(0)));
+        L.mfy = std::max(L.mfy, m);
+      }
+      rfy[l].n = vs.size() - rfy[l].off;
+      rfw[l].off = vs.size();
+      for (int t = 0; t < T_; t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        const bool hasE = t - 1 >= 0, hasF = t + 1 <= T_ - 1;
+        if (!hasE && !hasF) continue;
+        vs.push_back(mkskin(vecp(F->Bv, k), m,
+                            {Prod{hasE ? Lp(l, t) : nullptr, m, hasE ? vecp(F->Xv, id[t - 1]) : nullptr,
+                                  hasE ? (int)nk(id[t - 1]) : 0, T(-1)},
+                             Prod{hasF ? Up(l, t) : nullptr, m, hasF ? vecp(F->Xv, id[t + 1]) : nullptr,
+                                  hasF ? (int)nk(id[t + 1]) : 0, T(-1)}},
+                            T(1)));
+        L.mfw = std::max(L.mfw, m);
+      }
+      rfw[l].n = vs.size() - rfw[l].off;
+      rbw[l].off = vs.size();
+      for (int t = 1; t < T_; t += 2) {
+        const int k = id[t];
+        const int m = (int)nk(k);
+        const bool hasR = t + 1 <= T_ - 1;
+        vs.push_back(mkskin(vecp(F->Xv, k), m,
+                            {Prod{F->G[k], m, vecp(F->Xv, id[t - 1]), (int)nk(id[t - 1]), T(-1)},
+                             Prod{hasR ? F->H[k] : nullptr, m, hasR ? vecp(F->Xv, id[t + 1]) : nullptr,
+                                  hasR ? (int)nk(id[t + 1]) : 0, T(-1)}},
+                            T(1)));
+        L.mbw = std::max(L.mbw, m);
+      }
+      rbw[l].n = vs.size() - rbw[l].off;
       continue;
     }
     // g1: E_t = L_t Dinv_{t-1}, F_t = U_t Dinv_{t+1}
@@ -1045,7 +1167,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     const auto& id = ids[nlev - 1];
     const int k0 = id[0], k1 = id[1];
     const int n0 = (int)nk(k0), n1 = (int)nk(k1);
-    cx<T>* F0 = base + P.offF[nlev - 1][0];
+    cx<T>* F0 = lean ? F->H[k0] : base + P.offF[nlev - 1][0];
     rtail1 = vg.size();
     vg.push_back(mkgemm(F->G[k0], n0, n1, F->D[k0], F0, n0, n0, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
     vg.push_back(mkgemm(F->H[k1], n1, n0, F->G[k1], F->D[k0], n0, n1, n0, nullptr, nullptr, 0, 0, 0, T(1), T(0)));
@@ -1063,8 +1185,10 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     F->solve_bytes = sb;
     F->tail = true;
   }
+  if (slot_missing) return fail(KORE_ERR_ARG, "internal: lean factor plan misses a coupling slot");
   F->flops = flops;
   F->solve_launches = F->tail ? 2 * nlev - 1 : 2 * nlev + 1;
+  if (lean) F->solve_launches += std::max(0, (F->tail ? nlev - 1 : nlev) - l0);   // the y launches
   size_t rlev0 = 0;
   if (sp0) {
     // level 0 of a sweep: y_odd = Dinv_odd b_odd (forward; x_odd backward),
@@ -1123,6 +1247,7 @@ template <class T> static int32_t factor_build(kore_sys* s, Factor<T>* F) {
     L.g1 = dg + rg1[l].off; L.ng1 = (int)rg1[l].n;
     L.g2 = dg + rg2[l].off; L.ng2 = (int)rg2[l].n;
     L.fw = dsk + rfw[l].off; L.nfw = (int)rfw[l].n;
+    L.fy = dsk + rfy[l].off; L.nfy = (int)rfy[l].n;
     L.bw = dsk + rbw[l].off; L.nbw = (int)rbw[l].n;
     L.sc = dsc + L.sc_off;
     L.sr = dsr + L.sr_off;
@@ -1143,9 +1268,12 @@ template <class T> static int64_t factor_count_bytes(const std::vector<int>& bsz
   const int nb = (int)bsz.size();
   int nmax = 0;
   for (int b : bsz) nmax = std::max(nmax, b);
-  FactorPlan P;
-  factor_plan(bsz, P, true);   // degree blocks couple through the matrix's off-diagonal blocks
-  return P.cnt * (int64_t)sizeof(cx<T>) + 2 * n * MAXR * (int64_t)sizeof(cx<T>) +
+  // degree blocks couple through the matrix's off-diagonal blocks; the
+  // larger of the two forms' plans
+  FactorPlan P, Q;
+  factor_plan(bsz, P, true, false);
+  factor_plan(bsz, Q, true, true);
+  return std::max(P.cnt, Q.cnt) * (int64_t)sizeof(cx<T>) + 2 * n * MAXR * (int64_t)sizeof(cx<T>) +
          (int64_t)nb * (nmax + 1) * 4 + 3 * nb * 8 + 4096;
 }
 
@@ -1237,17 +1365,18 @@ template <class T> static int32_t bcr_sweep_launches(kore_sys* s, Factor<T>* F, 
   // each) take the kernel with registers for one product (a two-product
   // kernel for the forward levels measured no faster)
   const bool fewp = std::is_same<T, double>::value && nrhs == 1 && s->nmax <= 256;
-  auto lev0_inverse = [&]() -> int32_t {
+  auto inverse_blocks = [&](const SkinnyDesc<T>* d, int nd, int m) -> int32_t {
     Timed tm(c, KORE_TIMER_SKINNY);
     if constexpr (std::is_same<T, double>::value) {
       if (fewp) {
-        HIP_TRY(launch_skinny_np<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, 1, st));
+        HIP_TRY(launch_skinny_np<T>(d, nd, m, 1, st));
         return KORE_OK;
       }
     }
-    HIP_TRY(launch_skinny<T>(F->lev0_dinv, F->lev0_n, F->lev0_m, nrhs, st));
+    HIP_TRY(launch_skinny<T>(d, nd, m, nrhs, st));
     return KORE_OK;
   };
+  auto lev0_inverse = [&]() { return inverse_blocks(F->lev0_dinv, F->lev0_n, F->lev0_m); };
   for (int l = 0; l < ltop; ++l) {
     Level<T>& L = F->lev[l];
     if (l == 0 && sp0) {
@@ -1256,6 +1385,7 @@ template <class T> static int32_t bcr_sweep_launches(kore_sys* s, Factor<T>* F, 
       HIP_TRY(launch_offsweep<T>(s->off, s->d_blk.as<int32_t>(), n, 0, F->Xv, n, F->Bv, n, nrhs, st));
       continue;
     }
+    if (L.nfy) TRY(inverse_blocks(L.fy, L.nfy, L.mfy));   // lean form: y_odd = Dinv_odd b_odd
     Timed tm(c, KORE_TIMER_SKINNY);
     HIP_TRY(launch_skinny<T>(L.fw, L.nfw, L.mfw, nrhs, st));
   }
@@ -2320,6 +2450,14 @@ int32_t kore_sys_query(const kore_sys* s, kore_sys_state* out) {
   return KORE_OK;
 }
 
+int32_t kore_sys_set_factor_form(kore_sys* s, int32_t form) {
+  if (!s) return fail(KORE_ERR_ARG, "NULL argument");
+  if (form != KORE_FORM_AUTO && form != KORE_FORM_SWEEP && form != KORE_FORM_LEAN)
+    return fail(KORE_ERR_ARG, "unknown factor form %d", form);
+  s->form = form;
+  return KORE_OK;
+}
+
 int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
   if (!s || !out) return fail(KORE_ERR_ARG, "NULL argument");
   kore_ctx* c = s->ctx;
@@ -2339,6 +2477,15 @@ int32_t kore_sys_norm(kore_sys* s, double sre, double sim, double* out) {
 
 }  // extern "C"
 
+// Lean form (no E, F products) for this system's next factorisation.  Auto:
+// lean from degree blocks of LEAN_AUTO_MIN unknowns, where the quarter of the
+// GEMM flops it saves outweighs its extra sweep launch per level.
+static bool factor_lean(const kore_sys* s) {
+  if (s->form == KORE_FORM_LEAN) return true;
+  if (s->form == KORE_FORM_SWEEP) return false;
+  return s->nmax >= LEAN_AUTO_MIN;
+}
+
 // Queues the factorisation of A - sigma B on the context's stream (plan and
 // tile choice first, which may wait for the device); e0/e1 bracket the
 // factorisation proper when given.  *dinfo: the device pivot flags.
@@ -2347,9 +2494,11 @@ static int32_t factor_enqueue(kore_sys* s, double sre, double sim, int32_t preci
   kore_ctx* c = s->ctx;
   // keep the other precision's plan (and its device memory) when the
   // precision changes, so that alternating precisions does not rebuild plans
-  if (s->f && s->f->prec != precision) {
+  const bool lean = factor_lean(s);
+  auto other = [&](const std::unique_ptr<FactorBase>& f) { return f && (f->prec != precision || f->lean != lean); };
+  if (other(s->f)) {
     std::swap(s->f, s->fspare);
-    if (s->f && s->f->prec != precision) s->f.reset();
+    if (other(s->f)) s->f.reset();
   }
   s->sre = sre;
   s->sim = sim;
@@ -2365,6 +2514,7 @@ static int32_t factor_enqueue(kore_sys* s, double sre, double sim, int32_t preci
     if (!s->f) {
       auto F = std::make_unique<Factor<double>>();
       F->prec = precision;
+      F->lean = lean;
       rc = factor_build<double>(s, F.get());
       if (rc >= 0) s->f = std::move(F);
     }
@@ -2380,6 +2530,7 @@ static int32_t factor_enqueue(kore_sys* s, double sre, double sim, int32_t preci
     if (!s->f) {
       auto F = std::make_unique<Factor<float>>();
       F->prec = precision;
+      F->lean = lean;
       rc = factor_build<float>(s, F.get());
       if (rc >= 0) s->f = std::move(F);
     }
@@ -2461,6 +2612,7 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     info->flops_sparse = s->f->flops_sparse;
     info->solve_bytes = s->f->solve_bytes;
     info->solve_launches = s->f->solve_launches;
+    info->form = s->f->lean ? KORE_FORM_LEAN : KORE_FORM_SWEEP;
     info->bytes = s->f->bytes;
     info->ms = ms;
   }

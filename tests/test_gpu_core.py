@@ -160,6 +160,55 @@ def test_solve_large_blocks(ctx, case, precision, nrhs):
         assert info.iterations == 1
 
 
+@pytest.mark.parametrize("case", ["two_blocks", "four_blocks", "tiny", "hydro_N100", "thermal_N60"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lean_form_matches_sweep_form(ctx, case, precision):
+    """The lean factor form (no E = L Dinv, F = U Dinv products; kept blocks
+    updated through G, H; a forward sweep level in two launches) solves as the
+    sweep form does: both match SuperLU to 1e-10 on their plain sweep (FP64)
+    or after refinement (complex64), and agree with each other to rounding.
+    Two blocks: the tail alone (its F0 in the root block's H); four: one
+    level before the tail; the others several levels after the sparse level
+    0, thermal with two block sizes.  The lean form runs fewer GEMM flops and
+    one more launch per level swept before the tail."""
+    from kore_amd.params import Params
+    from kore_amd import assembly, backend
+    kw = dict(two_blocks=dict(N=20, lmax=2), four_blocks=dict(N=20, lmax=4), tiny=dict(N=12, lmax=8),
+              hydro_N100=dict(N=100, lmax=14),
+              thermal_N60=dict(N=60, lmax=16, thermal=1, Prandtl=1.0, Ra_gap=1e5))[case]
+    par = Params(m=1, symm=1, Ek=1e-4, **kw)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.4j
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((A.shape[0], 2)) + 1j * rng.standard_normal((A.shape[0], 2))
+    xr = sla.splu((A - sigma * B).tocsc()).solve(b)
+    out, infos = {}, {}
+    for form in (backend.FORM_SWEEP, backend.FORM_LEAN):
+        say("case: factor form", form, case, "sizmat", A.shape[0], "precision", precision)
+        S = _system(ctx, par, A, B)
+        S.set_factor_form(form)
+        infos[form] = S.factor(sigma, precision)
+        for nrhs in (1, 2):
+            x, info = S.solve(b[:, :nrhs], ir_tol=1e-14, ir_maxit=0 if precision == 0 else 20)
+            err = np.linalg.norm(x - xr[:, :nrhs], axis=0) / np.linalg.norm(xr[:, :nrhs], axis=0)
+            say("  form", form, "nrhs", nrhs, "rel err", err, "sweeps", info.iterations)
+            assert err.max() <= 1e-10, (form, nrhs)
+        out[form] = x
+        S.close()
+    fs, fl = infos[backend.FORM_SWEEP], infos[backend.FORM_LEAN]
+    assert fs.form == backend.FORM_SWEEP and fl.form == backend.FORM_LEAN
+    assert fl.levels == fs.levels
+    if precision == 0:
+        d = np.linalg.norm(out[backend.FORM_LEAN] - out[backend.FORM_SWEEP], axis=0) / np.linalg.norm(xr, axis=0)
+        assert d.max() <= 1e-11
+    # one y launch per level swept one by one (all but the tail's) on dense couplings
+    dense_swept = max(0, fs.levels - 1 - (1 if fs.levels >= 2 else 0))
+    assert fl.solve_launches - fs.solve_launches == dense_swept
+    if fs.levels >= 3:
+        assert fl.flops_gemm < fs.flops_gemm
+    assert fl.flops_inv == fs.flops_inv
+
+
 def test_eigs_blocks_wider_than_256(ctx):
     """FP64 eigensolve (single-vector Krylov, no refinement) on degree blocks
     of 300 rows: level 0 of every sweep takes the long preloading kernel for
