@@ -327,7 +327,8 @@ def serving_roofline(inst, head_name):
     rl = dict(cands[dom][1])
     rl["frac"] = rl["achieved"] / rl["peak"]
     rl["traffic"] = None           # HBM bytes per launch: rocprofv3 --pmc passes, committed under profiles/
-    rl["traffic_profile"] = "profiles/serving_gemm_c64_roofline.md"
+    rl["traffic_profile"] = {"skinny": "profiles/serving_sweep_roofline.md"}.get(
+        dom, "profiles/serving_gemm_c64_roofline.md")
     rl["other_kernels"] = {k: dict(achieved=round(v[1]["achieved"], 3), unit=v[1]["unit"],
                                    frac=round(v[1]["achieved"] / v[1]["peak"], 4), launches=v[1]["launches"],
                                    avg_launch_ms=round(v[1]["avg_launch_ms"], 4))
@@ -406,6 +407,8 @@ def run_serving(args):
         reps.setdefault(((int(max(sizes)) + 63) // 64, r.kind), r)
     log("warm-up: %d requests, then %d of distinct kind and block-size class, per path" % (
         max(1, args.warmup), len(reps)))
+    # (the server runs the first request of each precision and class alone,
+    # so that the tuner's timing sees no other request's kernels)
     for prec in (backend.PREC_FP64, backend.PREC_C64):
         srv = serving.Server(slots=args.slots, precision=prec, block=args.serving_block)
         srv.run(at0(trace[:max(1, args.warmup)]))

@@ -230,6 +230,16 @@ int32_t kore_sys_query(const kore_sys* sys, kore_sys_state* out);
  * for callers that know whether a factor serves few sweeps (a forced
  * problem: LEAN) or many (an eigensolve of small blocks: SWEEP). */
 int32_t kore_sys_set_factor_form(kore_sys* sys, int32_t form);
+/* Threads one call of the library's parallel host loops may use
+ * (kore_sys_prepare / kore_sys_create, pinned-staging copies); n <= 0
+ * restores the default: KORE_AMD_HOST_THREADS, else the process's CPU share
+ * (cgroup CPU quota, else affinity mask), at most 16.  Process-wide.  No
+ * counterpart in the reference: for callers that run several set-ups at once
+ * (a serving loop) and must keep the process within its CPU quota. */
+int32_t kore_set_host_threads(int32_t n);
+/* The current per-call host thread count, and the process's CPU share. */
+int32_t kore_host_threads(void);
+int32_t kore_cpu_share(void);
 /* ||A - sigma B||_F of the stored (unscaled) matrices. */
 int32_t kore_sys_norm(kore_sys* sys, double sre, double sim, double* out);
 

@@ -109,6 +109,9 @@ SIGNATURES = [
     ("kore_sys_update_a", _I32, [_P, _PD]),
     ("kore_sys_norm", _I32, [_P, _D, _D, _PD]),
     ("kore_sys_set_factor_form", _I32, [_P, _I32]),
+    ("kore_set_host_threads", _I32, [_I32]),
+    ("kore_host_threads", _I32, []),
+    ("kore_cpu_share", _I32, []),
     ("kore_sys_query", _I32, [_P, C.POINTER(SysState)]),
     ("kore_factor_async", _I32, [_P, _D, _D, _I32, _P]),
     ("kore_factor_check", _I32, [_P]),
@@ -208,6 +211,22 @@ def mem_stats(device: int = 0):
     allocs = C.c_int64(0)
     _check(lib().kore_mem_stats(int(device), C.byref(cached), C.byref(allocs)))
     return cached.value, allocs.value
+
+
+def set_host_threads(n: int):
+    """Threads per call of the library's parallel host loops (system set-up,
+    staging copies); n <= 0 restores the default (kore_set_host_threads)."""
+    _check(lib().kore_set_host_threads(int(n)))
+
+
+def host_threads() -> int:
+    return int(lib().kore_host_threads())
+
+
+def cpu_share() -> int:
+    """The process's CPU share as the library sees it (cgroup CPU quota, else
+    the affinity mask)."""
+    return int(lib().kore_cpu_share())
 
 
 def mem_trim(device: int = 0):

@@ -26,6 +26,8 @@
 #include <thread>
 #include <type_traits>
 
+#include <sched.h>
+
 #include "kore_amd.h"
 #include "kore_device.hpp"
 #include "kore_kernels.hpp"
@@ -411,12 +413,53 @@ struct CopySeg {
   size_t bytes;
 };
 
+// Host threads one call of the library's parallel host loops may use:
+// kore_set_host_threads, else KORE_AMD_HOST_THREADS, else the process's CPU
+// share (the cgroup's CPU quota where one is set, the affinity mask
+// otherwise), at most 16.  hardware_concurrency alone counts every core of
+// the machine; a process that runs more threads than its quota is throttled
+// as a whole, its launching threads included.
+static std::atomic<int> g_host_threads{0};
+
+static int cpu_share() {
+  static const int share = [] {
+    int n = (int)std::max(1u, std::thread::hardware_concurrency());
+    cpu_set_t cs;
+    CPU_ZERO(&cs);
+    if (sched_getaffinity(0, sizeof(cs), &cs) == 0) n = std::min(n, std::max(1, CPU_COUNT(&cs)));
+    if (FILE* f = fopen("/sys/fs/cgroup/cpu.max", "r")) {
+      char q[32] = {0};
+      long long per = 0;
+      if (fscanf(f, "%31s %lld", q, &per) == 2 && strcmp(q, "max") != 0 && per > 0) {
+        const long long quota = atoll(q);
+        if (quota > 0) n = (int)std::min<long long>(n, std::max<long long>(1, quota / per));
+      }
+      fclose(f);
+    }
+    return n;
+  }();
+  return share;
+}
+
+static int host_threads() {
+  const int v = g_host_threads.load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  static const int dflt = [] {
+    if (const char* e = std::getenv("KORE_AMD_HOST_THREADS")) {
+      const int x = std::atoi(e);
+      if (x > 0) return std::min(x, 64);
+    }
+    return std::min(cpu_share(), 16);
+  }();
+  return dflt;
+}
+
 // memcpy of a list of segments, the total split evenly over up to 8 threads
 // (at least 1 MiB each)
 static void copy_parallel(const std::vector<CopySeg>& segs) {
   size_t total = 0;
   for (const CopySeg& g : segs) total += g.bytes;
-  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, total >> 20));
+  const int nt = (int)std::min<size_t>(std::min(8, host_threads()), std::max<size_t>(1, total >> 20));
   auto range = [&](size_t lo, size_t hi) {
     size_t base = 0;
     for (const CopySeg& g : segs) {
@@ -1970,8 +2013,7 @@ static int32_t sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t
   };
   const int64_t nnz_a = a_indptr[n];
   s->h_amap.assign((size_t)nnz_a, -1);
-  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
-                                                               std::max<int64_t>(1, n / 4096)}));
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)host_threads(), std::max<int64_t>(1, n / 4096)));
   // per-range upper bounds (entries of A plus B on the range's rows) place
   // each range's output in the pinned arena before anything is built
   std::vector<size_t>& ub = P->ub;
@@ -2449,6 +2491,16 @@ int32_t kore_sys_query(const kore_sys* s, kore_sys_state* out) {
   out->sim = s->sim;
   return KORE_OK;
 }
+
+int32_t kore_set_host_threads(int32_t n) {
+  if (n > 64) return fail(KORE_ERR_ARG, "host threads %d: at most 64", n);
+  g_host_threads.store(n > 0 ? n : 0, std::memory_order_relaxed);
+  return KORE_OK;
+}
+
+int32_t kore_host_threads(void) { return host_threads(); }
+
+int32_t kore_cpu_share(void) { return cpu_share(); }
 
 int32_t kore_sys_set_factor_form(kore_sys* s, int32_t form) {
   if (!s) return fail(KORE_ERR_ARG, "NULL argument");

@@ -25,6 +25,9 @@ Pipeline:
   run concurrently and their kernels overlap on the device; a finished
   request frees its slot and memory at once and the next admissible request
   starts (continuous batching, not a FIFO of whole batches);
+* the first request of each (factor precision, block-size class) in the
+  process runs alone: the library's tile tuner times candidate tiles during
+  that factorisation, and other requests' kernels would skew the timing;
 * results are checked on the device from the resident matrices (forced:
   ||A x - b|| / (||A||_F ||x|| + ||b||); eigen: ||(A - lam B) x|| /
   (||A||_F ||x||)), and latency is measured from arrival to result on the
@@ -33,6 +36,7 @@ Pipeline:
 from __future__ import annotations
 
 import collections
+import contextlib
 import dataclasses
 import math
 import queue
@@ -46,6 +50,11 @@ import scipy.sparse as ss
 
 from . import assembly, backend, eigs
 from .params import Params
+
+try:
+    from threadpoolctl import threadpool_limits
+except ImportError:                      # pragma: no cover
+    threadpool_limits = None
 
 
 @dataclasses.dataclass
@@ -233,13 +242,25 @@ def residual_device(S, req: Request, x=None, lam=None, X=None, b=None):
     return float(np.max(r / (nA * np.linalg.norm(X, axis=0))))
 
 
+# (factor precision, block-size class of 64) pairs whose level-0 column tile
+# the library has measured in this process: its tuner times candidate tiles
+# at the first factorisation of each pair, so that factorisation runs with no
+# other request on the device (another slot's kernels would skew the timing)
+_TUNED = set()
+
+
+def tuner_key(par, precision):
+    _, sizes, _ = backend.layout_blocks(backend.layout_from_params(par))
+    return (int(precision), (int(max(sizes)) + 63) // 64)
+
+
 class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
     def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE,
-                 prepare_systems=True, lookahead=None):
+                 prepare_systems=True, lookahead=None, host_threads=2, blas_threads=1):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -263,6 +284,7 @@ class Server:
         self.mem_peak = 0
         self.max_inflight = 0
         self._inflight = 0
+        self._alone = False          # a request whose tile the tuner measures runs by itself
         self._cv = threading.Condition()
         self.asm_pool = ThreadPoolExecutor(max_workers=assembly_workers)
         # operators of repeated structures from the process-wide cache (None:
@@ -271,6 +293,16 @@ class Server:
         self.operator_cache = operator_cache
         self.prepare_systems = prepare_systems
         self.lookahead = lookahead if lookahead is not None else assembly_workers + slots
+        # Host CPU is shared by the assembly threads, the set-ups and the slot
+        # threads that launch the kernels, within the process's CPU quota.
+        # While serving: `host_threads` threads per system set-up (measured at
+        # N = 500: 2 threads take 106 ms of CPU for a 58 ms set-up, 16 threads
+        # 230 ms for 28 ms), and `blas_threads` BLAS threads (an OpenBLAS pool
+        # keeps every worker spinning for about 0.1 s after each call, even a
+        # vector norm, which costs a 16-thread pool 1.5 CPU-seconds per call);
+        # None leaves either setting alone.
+        self.host_threads = host_threads
+        self.blas_threads = blas_threads
 
     def close(self):
         self.asm_pool.shutdown(wait=True)
@@ -370,6 +402,18 @@ class Server:
 
     def run(self, trace: List[Request]) -> List[Result]:
         """Serve the trace (arrival times relative to now); returns results by id."""
+        blas = (threadpool_limits(limits=self.blas_threads, user_api="blas")
+                if threadpool_limits is not None and self.blas_threads else contextlib.nullcontext())
+        prev = backend.host_threads()
+        if self.host_threads:
+            backend.set_host_threads(self.host_threads)
+        try:
+            with blas:
+                return self._run(trace)
+        finally:
+            backend.set_host_threads(prev)
+
+    def _run(self, trace: List[Request]) -> List[Result]:
         t0 = time.perf_counter()
         results = {}
         lock = threading.Lock()
@@ -395,7 +439,7 @@ class Server:
         sub.start()
         threads = []
 
-        def worker(slot, req, par, A0, B0, need, plan):
+        def worker(slot, req, par, A0, B0, need, plan, key):
             try:
                 res = self._execute(slot, req, par, A0, B0, t0, plan)
             except Exception as e:  # report, keep serving
@@ -406,6 +450,9 @@ class Server:
             with self._cv:
                 self.mem_in_use -= need
                 self._inflight -= 1
+                if key is not None:
+                    _TUNED.add(key)
+                    self._alone = False
                 self.free_slots.put(slot)
                 self._cv.notify_all()
             if self.log:
@@ -429,15 +476,22 @@ class Server:
                 if plan is not None:
                     plan.close()
                 continue
+            # the factor precision of the request's first factorisation
+            key = tuner_key(par, self.precision if req.kind == "forced" else backend.PREC_FP64)
             with self._cv:
-                while self.free_slots.empty() or self.mem_in_use + need > self.mem_budget:
+                while (self.free_slots.empty() or self.mem_in_use + need > self.mem_budget or self._alone
+                       or (key not in _TUNED and self._inflight > 0)):
                     self._cv.wait()
+                if key in _TUNED:
+                    key = None
+                else:
+                    self._alone = True
                 slot = self.free_slots.get()
                 self.mem_in_use += need
                 self.mem_peak = max(self.mem_peak, self.mem_in_use)
                 self._inflight += 1
                 self.max_inflight = max(self.max_inflight, self._inflight)
-            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan), daemon=True)
+            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan, key), daemon=True)
             th.start()
             threads.append(th)
         sub.join()
