@@ -248,6 +248,33 @@ def test_prepared_system_equals_created(ctx, hydro_small):
     S2.close()
 
 
+def test_prepare_threads_do_not_change_the_system(ctx):
+    """The host set-up splits rows over kore_set_host_threads threads (one per
+    4096 rows at most); systems prepared with one thread and with six factor
+    and solve bit for bit alike (25 600 unknowns, so six ranges)."""
+    from kore_amd.params import Params
+    from kore_amd import assembly, backend
+    par = Params(m=1, symm=1, Ek=1e-4, N=64, lmax=400)
+    A, B, _ = assembly.assemble(par)
+    lay = backend.layout_from_params(par)
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((A.shape[0], 2)) + 1j * rng.standard_normal((A.shape[0], 2))
+    out = []
+    try:
+        for nt in (1, 6):
+            backend.set_host_threads(nt)
+            plan = backend.SystemPlan(lay, A, B)
+            S = backend.System(ctx, lay, None, None, plan=plan)
+            S.factor(0.3 + 0.8j, 0)
+            out.append(S.solve(b)[0])
+            S.close()
+    finally:
+        backend.set_host_threads(0)
+    assert np.array_equal(out[0], out[1])
+    xr = sla.splu((A - (0.3 + 0.8j) * B).tocsc()).solve(b)
+    assert np.linalg.norm(out[0] - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
 def test_eigenvectors_match_oracle_up_to_phase(ctx, hydro_small):
     """The eigenvectors themselves, not only their residuals: each device
     eigenvector spans the same line as the oracle's (SuperLU + Krylov-Schur)

@@ -58,6 +58,28 @@ def test_factor_bytes_prediction():
     assert 0.45 < c64 / big < 0.55
 
 
+def test_host_thread_setting():
+    """kore_set_host_threads: a positive count is used as given, 0 restores
+    the default (the process's CPU share, at most 16, unless
+    KORE_AMD_HOST_THREADS sets it), more than 64 is refused."""
+    import os
+    share = backend.cpu_share()
+    assert 1 <= share <= (os.cpu_count() or share)
+    default = backend.host_threads()
+    try:
+        backend.set_host_threads(3)
+        assert backend.host_threads() == 3
+        with pytest.raises(backend.KoreError):
+            backend.set_host_threads(65)
+        assert backend.host_threads() == 3
+        backend.set_host_threads(0)
+        assert backend.host_threads() == default
+        if "KORE_AMD_HOST_THREADS" not in os.environ:
+            assert default == min(share, 16)
+    finally:
+        backend.set_host_threads(0)
+
+
 class NumpyWorkspace:
     """Stand-in for backend.Workspace: same methods, numpy arrays, OP by SuperLU."""
 
