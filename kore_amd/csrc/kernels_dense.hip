@@ -543,9 +543,19 @@ __device__ __forceinline__ void gj_outer_range(int n, int p0, int& q0, int& wo) 
   }
 }
 
-template <class T, int NMAX>
-__global__ __launch_bounds__(512) void gj_panel_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
-  constexpr int NT = 512, NW = NT / 64, WP = GJM_WP;
+// NT threads, NT / NMAX per row.  A column step is a dependent chain (pivot
+// key, wave and workgroup reductions, pivot-row read, reciprocal) plus the
+// row updates.  Blocks of up to 256 rows: complex128 takes 512 threads, two
+// per row, so that a second wave per SIMD hides the chain's latencies;
+// complex64, whose updates cost half as much, takes 256 threads, one per row
+// (measured at 200-row blocks: 256 threads make the complex64 factorisation
+// 1.6 % faster and the complex128 one 4 % slower).  Larger blocks: 512.
+template <class T>
+constexpr int GJ_PANEL_NT256 = sizeof(T) == 4 ? 256 : 512;
+
+template <class T, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void gj_panel_kernel(const InvDesc<T>* __restrict__ descs, int p0) {
+  constexpr int NW = NT / 64, WP = GJM_WP;
   constexpr int TPR = NT / NMAX, CPT = WP / TPR;
   static_assert(TPR >= 1 && CPT * TPR == WP, "panel split");
   // candidate rows; the two column slices of a row are one complex apart
@@ -852,9 +862,10 @@ static hipError_t launch_gj_inverse_multi(const InvDesc<T>* d_descs, int ndesc, 
   const int nt = (nmax + 63) / 64;
   for (int p0 = 0; p0 < nmax; p0 += GJM_WP) {
     if (nmax <= 256)
-      hipLaunchKernelGGL((gj_panel_kernel<T, 256>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
+      hipLaunchKernelGGL((gj_panel_kernel<T, 256, GJ_PANEL_NT256<T>>), dim3(ndesc), dim3(GJ_PANEL_NT256<T>), 0, s,
+                         d_descs, p0);
     else
-      hipLaunchKernelGGL((gj_panel_kernel<T, 512>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
+      hipLaunchKernelGGL((gj_panel_kernel<T, 512, 512>), dim3(ndesc), dim3(512), 0, s, d_descs, p0);
     // column tiles of the two ranges an update covers: at most 3 in an outer
     // panel of 128 columns, at most nt + 1 in a block below GJ_OUTER_MIN rows
     // (at most 5 when such a block shares a launch with larger ones)
