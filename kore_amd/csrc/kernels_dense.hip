@@ -609,17 +609,19 @@ __global__ __launch_bounds__(NT) void gj_panel_kernel(const InvDesc<T>* __restri
       }
       if (lane == 0) sKey[par][wave] = key;
       __syncthreads();
-      unsigned long long best = sKey[par][0];
-      int q = 0;
+      // the workgroup's best key by a tree over the waves' keys; the wave
+      // holding the pivot row (whose candidate copy is read below) follows
+      // from the row itself
+      unsigned long long kk[NW];
 #pragma unroll
-      for (int qq = 1; qq < NW; ++qq) {
-        const unsigned long long kq = sKey[par][qq];
-        if (kq > best) {
-          best = kq;
-          q = qq;
-        }
-      }
+      for (int qq = 0; qq < NW; ++qq) kk[qq] = sKey[par][qq];
+#pragma unroll
+      for (int st = 1; st < NW; st *= 2)
+#pragma unroll
+        for (int i = 0; i + st < NW; i += 2 * st) kk[i] = umax64(kk[i], kk[i + st]);
+      const unsigned long long best = kk[0];
       const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
+      const int q = p / (64 / TPR);
       if (tid == 0) sPiv[c] = p;
       if (r == p) role = p0 + c;
       const cx<T>* __restrict__ prow = sCand[par][q];
