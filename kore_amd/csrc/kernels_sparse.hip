@@ -124,9 +124,9 @@ __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>
   }
 }
 
-// nc: columns per workgroup tile, 4 or 8 (complex128; 64 KiB of LDS at 8)
-// or 4, 8 or 16 (complex64); chosen per block-size class by the library's
-// tuner.
+// nc: columns per workgroup tile, 4, 8 or 16 (LDS: 16 B x 512 rows x nc for
+// complex128, so 128 KiB of gfx950's 160 KiB at 16; half that for complex64);
+// chosen per block-size class by the library's tuner.
 template <class T>
 hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc, int max_m, int max_n, int nc,
                         hipStream_t s) {
@@ -137,8 +137,8 @@ hipError_t launch_sprow(const OffCsr& o, const SpRowDesc<T>* d_descs, int ndesc,
     hipLaunchKernelGGL((sprow_kernel<T, 4>), grid, dim3(256), 0, s, o, d_descs);
   else if (nc == 8)
     hipLaunchKernelGGL((sprow_kernel<T, 8>), grid, dim3(256), 0, s, o, d_descs);
-  else if (nc == 16 && sizeof(T) == 4)
-    hipLaunchKernelGGL((sprow_kernel<T, sizeof(T) == 4 ? 16 : 8>), grid, dim3(256), 0, s, o, d_descs);
+  else if (nc == 16)
+    hipLaunchKernelGGL((sprow_kernel<T, 16>), grid, dim3(256), 0, s, o, d_descs);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -1743,7 +1743,7 @@ struct Tuner {
 static Tuner g_tuner;
 
 static std::vector<int> sprow_tiles(int prec) {
-  return prec == KORE_PREC_C64 ? std::vector<int>{4, 8, 16} : std::vector<int>{4, 8};
+  return std::vector<int>{4, 8, 16};
 }
 
 static int tuner_lookup(int prec, int nmax) {

@@ -582,14 +582,14 @@ def test_sweep_graph_replays_match_eager(ctx, hydro_small, precision):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_sprow_tiles_bit_identical(ctx, hydro_small, precision, monkeypatch):
     """Every column tile the tuner may choose for the level-0 sparse-row
-    products (4 or 8 columns for complex128, 4, 8 or 16 for complex64) gives
+    products (4, 8 or 16 columns) gives
     the same factor bit for bit: each output entry sums the row's entries in
     the same order whatever the tile.  Forced tiles (KORE_AMD_SPROW_NC) and
     the measured choice (no variable) are compared by their solves."""
     par, A, B = hydro_small
     rng = np.random.default_rng(11)
     b = rng.standard_normal((A.shape[0], 2)) + 1j * rng.standard_normal((A.shape[0], 2))
-    tiles = [4, 8] if precision == 0 else [4, 8, 16]
+    tiles = [4, 8, 16]
     out = {}
     for nc in tiles + [None]:
         say("case: sprow tile", nc, "precision", precision)
