@@ -271,8 +271,15 @@ def test_prepare_threads_do_not_change_the_system(ctx):
     finally:
         backend.set_host_threads(0)
     assert np.array_equal(out[0], out[1])
-    xr = sla.splu((A - (0.3 + 0.8j) * B).tocsc()).solve(b)
-    assert np.linalg.norm(out[0] - xr) <= 1e-10 * np.linalg.norm(xr)
+    # the shift lies close to an eigenvalue (|x| ~ 1e14 |b|): the solve is
+    # checked by its normwise backward error, which the conditioning does not
+    # inflate
+    M = (A - (0.3 + 0.8j) * B).tocsr()
+    nM = np.sqrt(np.sum(np.abs(M.data) ** 2))
+    for j in range(2):
+        x = out[0][:, j]
+        berr = np.linalg.norm(M @ x - b[:, j]) / (nM * np.linalg.norm(x) + np.linalg.norm(b[:, j]))
+        assert berr <= 1e-12, (j, berr)
 
 
 def test_eigenvectors_match_oracle_up_to_phase(ctx, hydro_small):
