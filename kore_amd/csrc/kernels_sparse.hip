@@ -65,6 +65,7 @@ hipError_t launch_offvals(const CsrDev& m, double sre, double sim, const double*
 // two rows of C (blocks of at most 512 rows) and SR_NC columns.
 // ---------------------------------------------------------------------------
 static constexpr int SR_MAXN = 512;   // rows of a degree block (and of an X tile)
+static constexpr int SR_U = 8;        // sparse entries whose loads are in flight together
 
 template <class T, int NC>
 __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>* __restrict__ descs) {
@@ -96,9 +97,29 @@ __global__ __launch_bounds__(256) void sprow_kernel(OffCsr o, const SpRowDesc<T>
       const int64_t row = d.row0 + r;
       const int s = o.ptr[row], nl = o.nlo[row];
       const int q0 = d.side[p] == 0 ? s : s + nl, q1 = d.side[p] == 0 ? s + nl : o.ptr[row + 1];
-      for (int q = q0; q < q1; ++q) {
+      const int xc0 = (int)d.xcol0[p];
+      // entries in groups of SR_U: the group's loads are all issued before
+      // the first is used, so a row pays the memory latency once per group
+      // (the entries are still added in order, as one at a time would)
+      int q = q0;
+      for (; q + SR_U <= q1; q += SR_U) {
+        cx<double> vv[SR_U];
+        int cc[SR_U];
+#pragma unroll
+        for (int u = 0; u < SR_U; ++u) {
+          cc[u] = o.col[q + u];
+          vv[u] = o.val[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < SR_U; ++u) {
+          const int c = cc[u] - xc0;
+#pragma unroll
+          for (int jj = 0; jj < NC; ++jj) acc[h][jj] = cfma(acc[h][jj], vv[u], up(sX[jj * SR_MAXN + c]));
+        }
+      }
+      for (; q < q1; ++q) {
         const cx<double> v = o.val[q];
-        const int c = (int)(o.col[q] - d.xcol0[p]);
+        const int c = o.col[q] - xc0;
 #pragma unroll
         for (int jj = 0; jj < NC; ++jj) acc[h][jj] = cfma(acc[h][jj], v, up(sX[jj * SR_MAXN + c]));
       }
@@ -163,7 +184,20 @@ __global__ __launch_bounds__(256) void spcol_kernel(OffCsc cc, const cx<double>*
     const int r = r0 + lane;
     cx<double> acc = czero<double>();
     if (r < d.m) {
-      for (int k = k0; k < k1; ++k) {
+      // entries in groups of SR_U, loads first (as in sprow_kernel)
+      int k = k0;
+      for (; k + SR_U <= k1; k += SR_U) {
+        cx<double> vv[SR_U];
+        cx<T> xx[SR_U];
+#pragma unroll
+        for (int u = 0; u < SR_U; ++u) {
+          vv[u] = oval[cc.q[k + u]];
+          xx[u] = gld(d.X + (int64_t)cc.row[k + u] * d.ldx + r);
+        }
+#pragma unroll
+        for (int u = 0; u < SR_U; ++u) acc = cfma(acc, up(xx[u]), vv[u]);
+      }
+      for (; k < k1; ++k) {
         const cx<double> v = oval[cc.q[k]];
         acc = cfma(acc, up(gld(d.X + (int64_t)cc.row[k] * d.ldx + r)), v);
       }
