@@ -262,6 +262,26 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return umax64(umax64(readlane_u64(v, 15), readlane_u64(v, 31)), umax64(readlane_u64(v, 47), readlane_u64(v, 63)));
 }
 
+// 32-bit pivot key of the multi-workgroup panel: valid flag (bit 31), the
+// magnitude's exponent and top 11 mantissa bits (bits 30-9), 511 - row.  The
+// chosen pivot is within a factor 1 - 2^-11 of the column's largest entry
+// (threshold partial pivoting; ties go to the lower row); one max per
+// reduction stage instead of a 64-bit compare and two selects.
+__device__ __forceinline__ unsigned pivot_key32(double mag, int r) {
+  if (!(mag <= 1.7976931348623157e308)) mag = 0.0;   // NaN / inf never win
+  const unsigned long long b = (unsigned long long)__double_as_longlong(mag);
+  return 0x80000000u | (((unsigned)(b >> 41) & 0x3FFFFFu) << 9) | (unsigned)(511 - r);
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, dpp_u32<0xB1>(v));        // quad_perm [1,0,3,2]
+  v = max(v, dpp_u32<0x4E>(v));        // quad_perm [2,3,0,1]
+  v = max(v, dpp_u32<0x141>(v));       // row_half_mirror
+  v = max(v, dpp_u32<0x140>(v));       // row_mirror
+  const unsigned a = max((unsigned)__builtin_amdgcn_readlane((int)v, 15), (unsigned)__builtin_amdgcn_readlane((int)v, 31));
+  const unsigned b = max((unsigned)__builtin_amdgcn_readlane((int)v, 47), (unsigned)__builtin_amdgcn_readlane((int)v, 63));
+  return max(a, b);
+}
+
 // Orderable pivot key: valid flag, magnitude (as double; the low 9 mantissa
 // bits give way to the row so that ties go to the lower row), row.
 __device__ __forceinline__ unsigned long long pivot_key(double mag, int r) {
@@ -563,7 +583,7 @@ __global__ __launch_bounds__(NT) void gj_panel_kernel(const InvDesc<T>* __restri
   // different LDS banks
   constexpr int SL = CPT + (TPR > 1 ? 1 : 0);
   __shared__ cx<T> sRow[2][TPR * SL];
-  __shared__ unsigned long long sKey[2][NW];
+  __shared__ unsigned sKey[2][NW];
   __shared__ int sPiv[WP];
   __shared__ int sSing;
   const InvDesc<T> d = descs[blockIdx.x];
@@ -600,21 +620,21 @@ __global__ __launch_bounds__(NT) void gj_panel_kernel(const InvDesc<T>* __restri
   auto step = [&](const int c) {
     if (c < w) {                          // uniform
       const int par = c & 1;
-      unsigned long long key = 0;
-      if (live && h == c / CPT && role == -1) key = pivot_key((double)cabs1(pick<CPT>(v, c % CPT)), r);
-      key = wave_max_u64(key);
+      unsigned key = 0;
+      if (live && h == c / CPT && role == -1) key = pivot_key32((double)cabs1(pick<CPT>(v, c % CPT)), r);
+      key = wave_max_u32(key);
       if (lane == 0) sKey[par][wave] = key;
       __syncthreads();
       // the workgroup's best key by a tree over the waves' keys
-      unsigned long long kk[NW];
+      unsigned kk[NW];
 #pragma unroll
       for (int qq = 0; qq < NW; ++qq) kk[qq] = sKey[par][qq];
 #pragma unroll
       for (int st = 1; st < NW; st *= 2)
 #pragma unroll
-        for (int i = 0; i + st < NW; i += 2 * st) kk[i] = umax64(kk[i], kk[i + st]);
-      const unsigned long long best = kk[0];
-      const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
+        for (int i = 0; i + st < NW; i += 2 * st) kk[i] = max(kk[i], kk[i + st]);
+      const unsigned best = kk[0];
+      const int p = best ? 511 - (int)(best & 0x1FFu) : 0;
       if (tid == 0) sPiv[c] = p;
       if (r == p) role = p0 + c;
       // only the pivot row's lanes write it, behind a second barrier: one
