@@ -216,17 +216,17 @@ template <class T> __device__ __forceinline__ T tiny_pivot() { return sizeof(T) 
 // batched Gauss-Jordan inverse with partial pivoting
 // ---------------------------------------------------------------------------
 // One workgroup of 512 threads per block.  Rows are never moved: column j is
-// eliminated with pivot row r_j (largest |re|+|im| among the rows not used
-// yet, first index on ties), so that at the end
+// eliminated with pivot row r_j (within a factor 1 - 2^-11 of the largest
+// |re|+|im| among the rows not used yet, first index on ties), so that at the end
 //     A^{-1}[k, l] = M[r_k, r^{-1}(l)],
 // applied once through the caller's scratch block.
 //
 // Panel J = [p0, p0+w) with pivot rows P:
 //   1. Gauss-Jordan on the panel columns over all rows, in registers (each
 //      row owned by TPR adjacent lanes, CPT columns each).  One column costs
-//      one barrier: the pivot is a packed 64-bit magnitude|row key reduced
-//      within each wave by DPP, and each wave's best row travels with its
-//      key through LDS.
+//      two barriers: the pivot is a packed 32-bit magnitude|row key reduced
+//      within each wave by DPP and across the waves through LDS; then the
+//      pivot row's lanes store it for all to read.
 //      Afterwards the panel holds C: rows P hold X = M[P, J]^{-1} (permuted),
 //      the other rows hold -M[i, J] X -- the final columns J of M.
 //   2. R = M[P, :] (old pivot rows) is staged in LDS.
@@ -236,33 +236,14 @@ template <class T> __device__ __forceinline__ T tiny_pivot() { return sizeof(T) 
 template <int CTRL> __device__ __forceinline__ unsigned dpp_u32(unsigned x) {
   return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
 }
-template <int CTRL> __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
-  return ((unsigned long long)dpp_u32<CTRL>((unsigned)(x >> 32)) << 32) | dpp_u32<CTRL>((unsigned)x);
-}
 template <int CTRL> __device__ __forceinline__ double dpp_t(double x) {
   return __hiloint2double((int)dpp_u32<CTRL>((unsigned)__double2hiint(x)), (int)dpp_u32<CTRL>((unsigned)__double2loint(x)));
 }
 template <int CTRL> __device__ __forceinline__ float dpp_t(float x) {
   return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(x)));
 }
-__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
-  return ((unsigned long long)hi << 32) | lo;
-}
-__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
-// Maximum over the wavefront, returned to every lane: quad swaps and row
-// mirrors leave each row of 16 lanes with its maximum, then the four rows
-// are combined through lanes 15, 31, 47 and 63.
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-  v = umax64(v, dpp_u64<0xB1>(v));     // quad_perm [1,0,3,2]
-  v = umax64(v, dpp_u64<0x4E>(v));     // quad_perm [2,3,0,1]
-  v = umax64(v, dpp_u64<0x141>(v));    // row_half_mirror
-  v = umax64(v, dpp_u64<0x140>(v));    // row_mirror
-  return umax64(umax64(readlane_u64(v, 15), readlane_u64(v, 31)), umax64(readlane_u64(v, 47), readlane_u64(v, 63)));
-}
 
-// 32-bit pivot key of the multi-workgroup panel: valid flag (bit 31), the
+// 32-bit pivot key of the Gauss-Jordan kernels: valid flag (bit 31), the
 // magnitude's exponent and top 11 mantissa bits (bits 30-9), 511 - row.  The
 // chosen pivot is within a factor 1 - 2^-11 of the column's largest entry
 // (threshold partial pivoting; ties go to the lower row); one max per
@@ -272,6 +253,9 @@ __device__ __forceinline__ unsigned pivot_key32(double mag, int r) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(mag);
   return 0x80000000u | (((unsigned)(b >> 41) & 0x3FFFFFu) << 9) | (unsigned)(511 - r);
 }
+// Maximum over the wavefront, returned to every lane: quad swaps and row
+// mirrors leave each row of 16 lanes with its maximum, then the four rows
+// are combined through lanes 15, 31, 47 and 63.
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
   v = max(v, dpp_u32<0xB1>(v));        // quad_perm [1,0,3,2]
   v = max(v, dpp_u32<0x4E>(v));        // quad_perm [2,3,0,1]
@@ -282,13 +266,6 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
   return max(a, b);
 }
 
-// Orderable pivot key: valid flag, magnitude (as double; the low 9 mantissa
-// bits give way to the row so that ties go to the lower row), row.
-__device__ __forceinline__ unsigned long long pivot_key(double mag, int r) {
-  if (!(mag <= 1.7976931348623157e308)) mag = 0.0;   // NaN / inf never win
-  const unsigned long long b = (unsigned long long)__double_as_longlong(mag);
-  return (1ULL << 63) | (b & ~0x1FFULL) | (unsigned long long)(511 - r);
-}
 
 template <class T, int WP, int NMAX>
 __global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __restrict__ descs) {
@@ -300,8 +277,8 @@ __global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __res
   constexpr int LDR = NMAX + 4;          // R: row k at sR[k*LDR + j]
   static_assert(TPR >= 1 && CPT * TPR == WP, "panel split");
   __shared__ cx<T> sR[WP * LDR];
-  __shared__ cx<T> sCand[2][NW][WP];      // each wave's best candidate row (double-buffered)
-  __shared__ unsigned long long sKey[2][NW];
+  __shared__ cx<T> sRow[2][WP];           // the pivot row (double-buffered)
+  __shared__ unsigned sKey[2][NW];
   __shared__ short sRole[NMAX];          // -1 unused, -2 used earlier, c >= 0: pivot c of this panel
   __shared__ int sPiv[WP];
   __shared__ int sSing;
@@ -328,40 +305,41 @@ __global__ __launch_bounds__(512) void gj_inverse_kernel(const InvDesc<T>* __res
       const int j = h * CPT + jj;
       v[jj] = (live && j < w) ? M[(int64_t)(p0 + j) * ld + r] : czero<T>();
     }
-    // One barrier per column: every wave publishes its best candidate row
-    // with its key, and after the barrier all threads pick the best of the
-    // eight and read the pivot row from that wave's slot.
+    // Two barriers per column: every wave publishes its best 32-bit key
+    // (pivot_key32: threshold pivoting within 1 - 2^-11 of the largest
+    // entry); after the first barrier all threads pick the best of the
+    // eight, the pivot row's lanes store it, and after the second barrier
+    // all read it (one row of stores per column instead of one per wave).
 #pragma unroll
     for (int c = 0; c < WP; ++c) {
       if (c < w) {                        // uniform
         const int par = c & 1;
-        unsigned long long key = 0;
-        if (live && h == c / CPT && role == -1) key = pivot_key((double)cabs1(v[c % CPT]), r);
-        key = wave_max_u64(key);
-        if (key != 0 && r == 511 - (int)(key & 0x1FFULL)) {
-#pragma unroll
-          for (int jj = 0; jj < CPT; ++jj) sCand[par][wave][h * CPT + jj] = v[jj];
-        }
+        unsigned key = 0;
+        if (live && h == c / CPT && role == -1) key = pivot_key32((double)cabs1(v[c % CPT]), r);
+        key = wave_max_u32(key);
         if (lane == 0) sKey[par][wave] = key;
         __syncthreads();
-        unsigned long long best = sKey[par][0];
-        int q = 0;
+        unsigned kk[NW];
 #pragma unroll
-        for (int qq = 1; qq < NW; ++qq) {
-          const unsigned long long kq = sKey[par][qq];
-          if (kq > best) {
-            best = kq;
-            q = qq;
-          }
-        }
+        for (int qq = 0; qq < NW; ++qq) kk[qq] = sKey[par][qq];
+#pragma unroll
+        for (int st = 1; st < NW; st *= 2)
+#pragma unroll
+          for (int i = 0; i + st < NW; i += 2 * st) kk[i] = max(kk[i], kk[i + st]);
+        const unsigned best = kk[0];
         // every column has an unused row left, so a valid key always exists
-        const int p = best ? 511 - (int)(best & 0x1FFULL) : 0;
+        const int p = best ? 511 - (int)(best & 0x1FFu) : 0;
         if (tid == 0) {
           sRole[p] = (short)c;
           sPiv[c] = p;
         }
         if (r == p) role = c;
-        const cx<T>* __restrict__ prow = sCand[par][q];
+        if (best != 0 && r == p) {
+#pragma unroll
+          for (int jj = 0; jj < CPT; ++jj) sRow[par][h * CPT + jj] = v[jj];
+        }
+        __syncthreads();
+        const cx<T>* __restrict__ prow = sRow[par];
         cx<T> pv = prow[c];
         T den = pv.re * pv.re + pv.im * pv.im;
         if (!(den > T(0))) {
