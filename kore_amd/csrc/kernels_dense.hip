@@ -17,8 +17,10 @@ namespace kore {
 // batched complex GEMM
 // ---------------------------------------------------------------------------
 // Workgroup = 4 waves, output tile 64x64, each wave a 32x32 quadrant made of
-// 2x2 MFMA 16x16 tiles; K staged through LDS in slabs of 16.
+// 2x2 MFMA 16x16 tiles; K staged through LDS in slabs of GT_K = 16 (32 measured
+// slower: the larger LDS and register footprint halves the workgroups per CU).
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
+constexpr int GT_EPT = GT_K * GT_M / 256;   // slab elements of A (and of B) per thread
 constexpr int GT_PADB = 1;
 
 // One 64 x 64 tile of C = alpha sum_p A_p B_p + beta C for the workgroup
@@ -61,27 +63,27 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
         csum[a][b] = A4{0, 0, 0, 0};
       }
 
-    // Slabs of 16 k-columns over the (up to two) products, register
+    // Slabs of GT_K k-columns over the (up to two) products, register
     // double-buffered: slab s+1 is requested from memory before slab s is
     // multiplied, so the load latency hides behind the matrix-core work.
-    cx<T> ra[4], rb[4];
+    cx<T> ra[GT_EPT], rb[GT_EPT];
     auto load = [&](int p, int k0) {      // selects, not indexing: d stays in registers
       const cx<T>* __restrict__ A = p ? d.A[1] : d.A[0];
       const cx<T>* __restrict__ B = p ? d.B[1] : d.B[0];
       const int K = p ? d.k[1] : d.k[0];
       const int64_t lda = p ? d.lda[1] : d.lda[0], ldb = p ? d.ldb[1] : d.ldb[0];
-      if (d.opA == 0) {        // A slab element (k = tid/16, i = 4 (tid%16) + q)
-        const int gk = k0 + (tid >> 4), ii = (tid & 15) * 4;
+      if (d.opA == 0) {        // A slab element (k = tid / (64/EPT), i = EPT (tid % (64/EPT)) + q)
+        const int gk = k0 + tid / (GT_M / GT_EPT), ii = (tid % (GT_M / GT_EPT)) * GT_EPT;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < GT_EPT; ++q) {
           const int gi = i0 + ii + q;
           ra[q] = czero<T>();
           if (gk < K && gi < d.m) ra[q] = gld(A + (int64_t)gk * lda + gi);
         }
       } else {                 // A_eff(i,k) = conj(A_store(k,i)), A_store column-major K x m
-        const int gi = i0 + (tid >> 2), kk = (tid & 3) * 4;
+        const int gi = i0 + (tid >> 2), kk = (tid & 3) * GT_EPT;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < GT_EPT; ++q) {
           const int gk = k0 + kk + q;
           ra[q] = czero<T>();
           if (gk < K && gi < d.m) {
@@ -90,9 +92,9 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
           }
         }
       }
-      const int gj = j0 + (tid >> 2), kk = (tid & 3) * 4;   // B slab element (j, k)
+      const int gj = j0 + (tid >> 2), kk = (tid & 3) * GT_EPT;   // B slab element (j, k)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < GT_EPT; ++q) {
         const int gk = k0 + kk + q;
         rb[q] = czero<T>();
         if (gk < K && gj < d.n) rb[q] = gld(B + (int64_t)gj * ldb + gk);
@@ -104,18 +106,18 @@ __device__ __forceinline__ void gemm_tile_body(const GemmDesc<T>& d, const int t
     while (p < 2) {
       __syncthreads();                      // the previous slab has been consumed
       if (d.opA == 0) {
-        const int kk = tid >> 4, ii = (tid & 15) * 4;
+        const int kk = tid / (GT_M / GT_EPT), ii = (tid % (GT_M / GT_EPT)) * GT_EPT;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sA[kk][ii + q] = ra[q];
+        for (int q = 0; q < GT_EPT; ++q) sA[kk][ii + q] = ra[q];
       } else {
-        const int ii = tid >> 2, kk = (tid & 3) * 4;
+        const int ii = tid >> 2, kk = (tid & 3) * GT_EPT;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sA[kk + q][ii] = ra[q];
+        for (int q = 0; q < GT_EPT; ++q) sA[kk + q][ii] = ra[q];
       }
       {
-        const int jj = tid >> 2, kk = (tid & 3) * 4;
+        const int jj = tid >> 2, kk = (tid & 3) * GT_EPT;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sB[jj][kk + q] = rb[q];
+        for (int q = 0; q < GT_EPT; ++q) sB[jj][kk + q] = rb[q];
       }
       __syncthreads();
       int pn = p, kn = k0 + GT_K;           // next slab
