@@ -568,6 +568,8 @@ def main():
     ap.add_argument("--ncv", type=int, default=None)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--factor-only", action="store_true",
+                    help="eigen workloads: only factor the workload's system --steps times per path")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the instrumented roofline pass of the serving line (the command profiled with rocprofv3)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -601,6 +603,23 @@ def main():
     ctx = backend.Context(0)
     S = backend.System(ctx, backend.layout_from_params(par), A, B)
     _, sizes, _ = backend.layout_blocks(backend.layout_from_params(par))
+    if args.factor_only:
+        # the workload's factorisation alone, --steps times per path (the
+        # command profiled for the factorisation's counters)
+        out = {}
+        for name in args.paths.split(","):
+            prec = backend.PREC_C64 if name == "mixed" else backend.PREC_FP64
+            ms = []
+            for i in range(args.steps):
+                fi = S.factor(wl["sigma"], prec)
+                ms.append(fi.ms)
+                log("  %s factorisation %d: %.2f ms" % (name, i, fi.ms))
+            out[name] = dict(factor_ms=ms, flops=fi.flops, flops_inv=fi.flops_inv, flops_gemm=fi.flops_gemm,
+                             flops_sparse=fi.flops_sparse, form=int(fi.form))
+        print(json.dumps({"workload": args.workload, "factor_only": out}), flush=True)
+        S.close()
+        ctx.close()
+        return
     paths = {}
     for name in args.paths.split(","):
         prec = backend.PREC_C64 if name == "mixed" else backend.PREC_FP64
