@@ -551,7 +551,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="serving", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--requests", type=int, default=1024)
-    ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--e-range", default="1e-7,1e-5")
     ap.add_argument("--n-cap", type=int, default=500)
     ap.add_argument("--load", type=float, default=0.8, help="arrival rate as a fraction of the FP64 capacity")

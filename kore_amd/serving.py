@@ -17,7 +17,8 @@ Pipeline:
   its operators (A0, B0), from the operator cache when its structure repeats,
   and the host half of its system's set-up (kore_sys_prepare), so that a slot
   only uploads and computes; at most ``lookahead`` requests are prepared and
-  not yet dispatched;
+  not yet dispatched, and they are dispatched in the order their host work
+  completes (a full assembly holds up no later request);
 * admission: a request enters a device slot when a slot is free and its
   predicted device footprint (library estimate of the factor bytes plus the
   system and Krylov workspace) fits in the HBM budget;
@@ -148,6 +149,37 @@ def assemble_request(req: Request):
     return par, A0, B0
 
 
+_AXPY_THREADS = 4
+_AXPY_POOL = None
+_AXPY_LOCK = threading.Lock()
+
+
+def _affine_values(a0, a1, e):
+    """a0 + e a1 over the value arrays of one pattern, split over a few
+    threads for large arrays (numpy releases the interpreter lock in these
+    loops): at N = 500 (10.7 million values) one thread takes 70-100 ms of
+    a request's latency.  Each entry is computed as before, e a1 then + a0."""
+    global _AXPY_POOL
+    out = np.empty_like(a0)
+    n = a0.size
+    if n < (1 << 20):
+        np.multiply(a1, e, out=out)
+        np.add(out, a0, out=out)
+        return out
+    with _AXPY_LOCK:
+        if _AXPY_POOL is None:
+            _AXPY_POOL = ThreadPoolExecutor(max_workers=_AXPY_THREADS)
+    step = (n + _AXPY_THREADS - 1) // _AXPY_THREADS
+
+    def part(lo):
+        hi = min(n, lo + step)
+        np.multiply(a1[lo:hi], e, out=out[lo:hi])
+        np.add(out[lo:hi], a0[lo:hi], out=out[lo:hi])
+
+    list(_AXPY_POOL.map(part, range(0, n, step)))
+    return out
+
+
 class OperatorCache:
     """Operators of repeated structures: in the hydrodynamic problem of the
     serving trace the Ekman number multiplies only the viscous terms (the
@@ -188,7 +220,7 @@ class OperatorCache:
             first = self._first.get(k) if aff is None else None
         if aff is not None:
             indptr, indices, a0, a1, B, shape = aff
-            data = a0 + req.Ek * a1
+            data = _affine_values(a0, a1, req.Ek)
             A = ss.csr_matrix((data, indices, indptr), shape=shape, copy=False)
             with self._lock:
                 self.hits += 1
@@ -260,7 +292,7 @@ class Server:
     def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE,
-                 prepare_systems=True, lookahead=None, host_threads=2, blas_threads=1):
+                 prepare_systems=True, lookahead=None, host_threads=None, blas_threads=1):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -295,12 +327,14 @@ class Server:
         self.lookahead = lookahead if lookahead is not None else assembly_workers + slots
         # Host CPU is shared by the assembly threads, the set-ups and the slot
         # threads that launch the kernels, within the process's CPU quota.
-        # While serving: `host_threads` threads per system set-up (measured at
-        # N = 500: 2 threads take 106 ms of CPU for a 58 ms set-up, 16 threads
-        # 230 ms for 28 ms), and `blas_threads` BLAS threads (an OpenBLAS pool
-        # keeps every worker spinning for about 0.1 s after each call, even a
-        # vector norm, which costs a 16-thread pool 1.5 CPU-seconds per call);
-        # None leaves either setting alone.
+        # While serving: `blas_threads` BLAS threads (an OpenBLAS pool keeps
+        # every worker spinning for about 0.1 s after each call, even a vector
+        # norm, which costs a 16-thread pool 1.5 CPU-seconds per call and
+        # throttled the launching threads), and `host_threads` threads per
+        # system set-up (None: the library's default, the CPU share up to 16;
+        # measured at N = 500 with BLAS on one thread: 16 threads set a system
+        # up in 32 ms against 80 ms with 2 under a Poisson trace, at the same
+        # saturated capacity); None leaves either setting alone.
         self.host_threads = host_threads
         self.blas_threads = blas_threads
 
@@ -314,10 +348,12 @@ class Server:
         """Host work of a request, on the assembly threads ahead of a free
         slot: its operators (from the cache when the structure repeats) and
         the host half of its system's set-up (kore_sys_prepare)."""
+        t0 = time.perf_counter()
         par, A0, B0 = (self.operator_cache.assemble(req) if self.operator_cache is not None
                        else assemble_request(req))
+        t1 = time.perf_counter()
         plan = backend.SystemPlan(backend.layout_from_params(par), A0, B0) if self.prepare_systems else None
-        return par, A0, B0, plan
+        return par, A0, B0, plan, (t1 - t0, time.perf_counter() - t1)
 
     def _execute(self, slot, req: Request, par, A0, B0, t0, plan=None) -> Result:
         ctx = self.ctxs[slot]
@@ -420,9 +456,11 @@ class Server:
         order = sorted(trace, key=lambda r: r.arrival)
         # a request's host work (operators, system plan) starts at its arrival,
         # on the assembly threads; at most `lookahead` requests are prepared and
-        # not yet dispatched (each holds its operators and staged system)
-        futs = {}
-        ready = {r.rid: threading.Event() for r in order}
+        # not yet dispatched (each holds its operators and staged system).
+        # Requests are dispatched in the order their host work completes, so
+        # that a full assembly (a structure not yet cached: 0.5-1.6 s at
+        # N = 400-500) holds up no request that arrived after it.
+        ready_q = queue.Queue()
         window = threading.Semaphore(self.lookahead)
 
         def submitter():
@@ -431,20 +469,23 @@ class Server:
                 if dt > 0:
                     time.sleep(dt)
                 window.acquire()
-                with lock:
-                    futs[req.rid] = self.asm_pool.submit(self._prepare, req)
-                ready[req.rid].set()
+                fut = self.asm_pool.submit(self._prepare, req)
+                fut.add_done_callback(lambda f, req=req: ready_q.put((req, f)))
 
         sub = threading.Thread(target=submitter, daemon=True)
         sub.start()
         threads = []
 
-        def worker(slot, req, par, A0, B0, need, plan, key):
+        def worker(slot, req, par, A0, B0, need, plan, key, host):
             try:
                 res = self._execute(slot, req, par, A0, B0, t0, plan)
             except Exception as e:  # report, keep serving
                 res = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0, float("inf"), False,
                              error=repr(e))
+            # before the slot: the operators and the host set-up, then the
+            # wait for a host worker and for a slot
+            res.stages["operators"], res.stages["host_setup"] = host
+            res.stages["wait"] = max(0.0, res.latency - res.service - host[0] - host[1])
             with lock:
                 results[req.rid] = res
             with self._cv:
@@ -459,12 +500,10 @@ class Server:
                 self.log("req %d %s done: latency %.3f s, service %.3f s, residual %.1e" % (
                     req.rid, req.kind, res.latency, res.service, res.residual))
 
-        for req in order:
-            ready[req.rid].wait()               # arrived and submitted
-            with lock:
-                fut = futs.pop(req.rid)
+        for _ in range(len(order)):
+            req, fut = ready_q.get()            # arrived, host work done
             try:
-                par, A0, B0, plan = fut.result()
+                par, A0, B0, plan, host = fut.result()
             finally:
                 window.release()
             # FP64 bytes even on the mixed path: eigensolves and refinement
@@ -491,7 +530,7 @@ class Server:
                 self.mem_peak = max(self.mem_peak, self.mem_in_use)
                 self._inflight += 1
                 self.max_inflight = max(self.max_inflight, self._inflight)
-            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan, key), daemon=True)
+            th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan, key, host), daemon=True)
             th.start()
             threads.append(th)
         sub.join()
