@@ -721,6 +721,190 @@ hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<doubl
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// one Gram-Schmidt update fused with the inner products that follow it
+//   W <- W - V H   (W: n x NBW, V: n x na, H: na x NBW, column-major, ld ldh)
+// then per row chunk the partials of  V^H W  (MODE 0: the next pass's
+// coefficients, na x NBW) or  W^H W  (MODE 1: the block's Gram matrix,
+// NBW x NBW), in tsdot's layout, for sum_parts_kernel / normalize1_kernel.
+// ---------------------------------------------------------------------------
+// One workgroup per row chunk and all na columns: each tile of TR rows of V is
+// staged through LDS once and serves both the update (TPR threads per row, a
+// shuffle reduction over them) and the products (one thread per column, four
+// row slices), so a pass reads the basis once where an update launch and a
+// tsdot launch read it twice.
+template <int NBW, int NAMAX, int MODE>
+__global__ __launch_bounds__(256) void ortho_step_kernel(const cx<double>* __restrict__ V, int64_t ldv, int na,
+                                                         cx<double>* __restrict__ W, int64_t ldw,
+                                                         const cx<double>* __restrict__ H, int ldh, int64_t n,
+                                                         int64_t rows_per_chunk, cx<double>* __restrict__ partial) {
+  constexpr int TR = NAMAX <= 128 ? 32 : 16, TPR = 256 / TR, LDV = NAMAX + 1, NG = NAMAX / 64;
+  constexpr int RB = 4;                   // tile loads in flight per thread
+  static_assert(3 * 64 * NG * NBW <= TR * LDV && TR * NBW * NBW <= TR * LDV, "reduction fits the tile");
+  __shared__ cx<double> sV[TR * LDV];     // the V tile; at the end, the slice partials
+  __shared__ cx<double> sH[NAMAX * NBW];
+  __shared__ cx<double> sW[TR][NBW];
+  const int tid = threadIdx.x;
+  const int ur = tid / TPR, us = tid % TPR;        // update role: row ur, columns us + k TPR
+  const int ci = tid & 63, rs = tid >> 6;          // product role: columns ci + 64 g, rows rs + 4 k
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+  const int64_t r1 = r0 + rows_per_chunk < n ? r0 + rows_per_chunk : n;
+  for (int idx = tid; idx < na * NBW; idx += 256) {
+    const int c = idx % na, b = idx / na;
+    sH[c * NBW + b] = H[(int64_t)b * ldh + c];
+  }
+  cx<double> acc[NG][NBW];
+  cx<double> gacc[NBW][NBW];
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) acc[g][b] = czero<double>();
+#pragma unroll
+  for (int a = 0; a < NBW; ++a)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) gacc[a][b] = czero<double>();
+  const int tile = TR * na;
+  for (int64_t rt = r0; rt < r1; rt += TR) {
+    __syncthreads();                      // the previous tile's readers are done; sH is visible
+    for (int base = 0; base < tile; base += RB * 256) {
+      cx<double> x[RB];
+#pragma unroll
+      for (int t = 0; t < RB; ++t) {
+        const int idx = base + t * 256 + tid;
+        const int col = idx / TR, row = idx - col * TR;
+        x[t] = czero<double>();
+        if (idx < tile && rt + row < r1) x[t] = V[(int64_t)col * ldv + rt + row];
+      }
+#pragma unroll
+      for (int t = 0; t < RB; ++t) {
+        const int idx = base + t * 256 + tid;
+        const int col = idx / TR, row = idx - col * TR;
+        if (idx < tile) sV[row * LDV + col] = x[t];
+      }
+    }
+    __syncthreads();
+    cx<double> s[NBW];
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) s[b] = czero<double>();
+    for (int c = us; c < na; c += TPR) {
+      const cx<double> v = sV[ur * LDV + c];
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) s[b] = cfma(s[b], v, sH[c * NBW + b]);
+    }
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) {
+        s[b].re += __shfl_xor(s[b].re, off);
+        s[b].im += __shfl_xor(s[b].im, off);
+      }
+    if (us == 0) {
+      const int64_t row = rt + ur;
+      cx<double> x[NBW];
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) {
+        x[b] = czero<double>();
+        if (row < r1) {
+          cx<double>* p = W + (int64_t)b * ldw + row;
+          x[b] = csub(*p, s[b]);
+          *p = x[b];
+        }
+        sW[ur][b] = x[b];
+      }
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int a = 0; a < NBW; ++a)
+#pragma unroll
+          for (int b = 0; b < NBW; ++b) gacc[a][b] = cfma_conj(gacc[a][b], x[a], x[b]);
+      }
+    }
+    if constexpr (MODE == 0) {
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int col = ci + 64 * g;
+        if (col < na) {
+          for (int r = rs; r < TR; r += 4) {
+            const cx<double> v = sV[r * LDV + col];
+#pragma unroll
+            for (int b = 0; b < NBW; ++b) acc[g][b] = cfma_conj(acc[g][b], v, sW[r][b]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (MODE == 0) {
+    if (rs > 0) {
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int b = 0; b < NBW; ++b) sV[(((rs - 1) * 64 + ci) * NG + g) * NBW + b] = acc[g][b];
+    }
+    __syncthreads();
+    if (rs == 0) {
+      cx<double>* out = partial + (int64_t)blockIdx.x * na * NBW;
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int col = ci + 64 * g;
+        if (col < na) {
+#pragma unroll
+          for (int b = 0; b < NBW; ++b) {
+            cx<double> t = acc[g][b];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) t = cadd(t, sV[((q * 64 + ci) * NG + g) * NBW + b]);
+            out[(int64_t)b * na + col] = t;
+          }
+        }
+      }
+    }
+  } else {
+    if (us == 0) {
+#pragma unroll
+      for (int a = 0; a < NBW; ++a)
+#pragma unroll
+        for (int b = 0; b < NBW; ++b) sV[(ur * NBW + a) * NBW + b] = gacc[a][b];
+    }
+    __syncthreads();
+    if (tid < NBW * NBW) {
+      const int a = tid % NBW, b = tid / NBW;
+      cx<double> t = czero<double>();
+      for (int q = 0; q < TR; ++q) t = cadd(t, sV[(q * NBW + a) * NBW + b]);
+      partial[(int64_t)blockIdx.x * NBW * NBW + b * NBW + a] = t;   // column-major NBW x NBW
+    }
+  }
+}
+
+template <int NBW, int MODE>
+static hipError_t launch_ortho_step_nbw(const cx<double>* V, int64_t ldv, int na, cx<double>* W, int64_t ldw,
+                                        const cx<double>* H, int ldh, int64_t n, int nchunk, int64_t rows_per_chunk,
+                                        cx<double>* partial, hipStream_t s) {
+  const dim3 grid((unsigned)nchunk);
+  if (na <= 64)
+    hipLaunchKernelGGL((ortho_step_kernel<NBW, 64, MODE>), grid, dim3(256), 0, s, V, ldv, na, W, ldw, H, ldh, n,
+                       rows_per_chunk, partial);
+  else if (na <= 128)
+    hipLaunchKernelGGL((ortho_step_kernel<NBW, 128, MODE>), grid, dim3(256), 0, s, V, ldv, na, W, ldw, H, ldh, n,
+                       rows_per_chunk, partial);
+  else
+    hipLaunchKernelGGL((ortho_step_kernel<NBW, 256, MODE>), grid, dim3(256), 0, s, V, ldv, na, W, ldw, H, ldh, n,
+                       rows_per_chunk, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_ortho_step(const cx<double>* V, int64_t ldv, int na, cx<double>* W, int64_t ldw, int nbw,
+                             const cx<double>* H, int ldh, int64_t n, int nchunk, int64_t rows_per_chunk, int gram,
+                             cx<double>* partial, hipStream_t s) {
+  if (na < 1 || na > ORTHO_STEP_MAXA || nbw < 1 || nbw > 2 || ldh < na || nchunk < 1 || rows_per_chunk < 1 ||
+      (int64_t)nchunk * rows_per_chunk < n)
+    return hipErrorInvalidValue;
+  if (nbw == 1)
+    return gram ? launch_ortho_step_nbw<1, 1>(V, ldv, na, W, ldw, H, ldh, n, nchunk, rows_per_chunk, partial, s)
+                : launch_ortho_step_nbw<1, 0>(V, ldv, na, W, ldw, H, ldh, n, nchunk, rows_per_chunk, partial, s);
+  return gram ? launch_ortho_step_nbw<2, 1>(V, ldv, na, W, ldw, H, ldh, n, nchunk, rows_per_chunk, partial, s)
+              : launch_ortho_step_nbw<2, 0>(V, ldv, na, W, ldw, H, ldh, n, nchunk, rows_per_chunk, partial, s);
+}
+
 // out[i] = sum_p in[p*len + i]: 8 outputs x 32 part-slices per workgroup, so
 // that each thread adds at most nparts/32 values (short load chains even for
 // a handful of outputs); the slice sums are combined through LDS in a fixed

@@ -165,6 +165,14 @@ hipError_t launch_colmaxabs(const cx<double>* x, int64_t ldx, int64_t n, int nrh
 // partial[c] (na x nb, column-major) = V[chunk c rows, :na]^H W[chunk c rows, :nb], nb <= 16
 hipError_t launch_tsdot(const cx<double>* V, int64_t ldv, int na, const cx<double>* W, int64_t ldw, int nb,
                         int64_t n, int nchunk, int64_t rows_per_chunk, cx<double>* partial, hipStream_t s);
+// W (n x nbw, nbw <= 2) -= V[:, :na] H (H: na x nbw, ld ldh), then per row chunk c
+// partial[c] = V[chunk rows, :na]^H W[chunk rows] (gram = 0, na x nbw) or
+// W[chunk rows]^H W[chunk rows] (gram = 1, nbw x nbw), column-major, as launch_tsdot
+// writes them; na <= ORTHO_STEP_MAXA
+constexpr int ORTHO_STEP_MAXA = 256;
+hipError_t launch_ortho_step(const cx<double>* V, int64_t ldv, int na, cx<double>* W, int64_t ldw, int nbw,
+                             const cx<double>* H, int ldh, int64_t n, int nchunk, int64_t rows_per_chunk, int gram,
+                             cx<double>* partial, hipStream_t s);
 // sum nparts consecutive arrays of len doubles: out[i] = sum_p in[p*len + i]
 hipError_t launch_sum_parts(const double* in, int nparts, int len, double* out, hipStream_t s);
 // partial[b] (b < nblocks) = slice sums of |va - sigma vb|^2 over nnz entries

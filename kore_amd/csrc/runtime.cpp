@@ -2908,6 +2908,14 @@ int32_t kore_solve_gmres(kore_sys* s, const double* b, double* x, double tol, in
 
 static inline cx<double>* wcol(kore_ws* w, int j) { return w->V.as<cx<double>>() + (size_t)j * w->sys->n; }
 
+// Gram-Schmidt updates fused with the inner products that follow them
+// (default); KORE_AMD_ORTHO_FUSED=0 selects separate update and product
+// launches, read at every expansion step so that one process can compare both.
+static bool ortho_fused() {
+  const char* e = std::getenv("KORE_AMD_ORTHO_FUSED");
+  return !e || std::atoi(e) != 0;
+}
+
 // Coefficient block of the expansion (complex entries): h1, h2 (ncol x MAXR
 // each), G, R1, Rinv1, R2, Rinv2 (MAXR x MAXR each), then two int flags.
 static size_t ws_hb_elems(int ncol) { return 2 * (size_t)ncol * MAXR + 5 * MAXR * MAXR + 1; }
@@ -3253,27 +3261,45 @@ int32_t kore_ws_expand(kore_ws* w, int32_t j0, int32_t j1, int32_t bs, int32_t c
       return KORE_WARN_IR_NOT_CONVERGED;
     }
     const int nb = j + bs;
-    // two passes of classical Gram-Schmidt against V[:, 0:nb]
+    const int64_t n = s->n;
+    int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
+    if (nchunk < 1) nchunk = 1;
+    const int64_t chunk = (n + nchunk - 1) / nchunk;
+    cx<double>* part = w->part.as<cx<double>>();
+    // two passes of classical Gram-Schmidt against V[:, 0:nb], then the Gram
+    // partials of the block W = V[:, cw:cw+bs]
     TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h1));
-    TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h1, nb, -1.0, 1.0));
-    TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h2));
-    TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h2, nb, -1.0, 1.0));
+    if (bs <= 2 && nb <= ORTHO_STEP_MAXA && ortho_fused()) {
+      // each update fused with the inner products that follow it: the basis
+      // is read three times per block instead of five
+      Timed tm(c, KORE_TIMER_ORTHO);
+      HIP_TRY(launch_ortho_step(wcol(w, 0), n, nb, wcol(w, cw), n, bs, d_h1, nb, n, nchunk, chunk, 0, part,
+                                c->stream));
+      HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * nb * bs,
+                               reinterpret_cast<double*>(d_h2), c->stream));
+      HIP_TRY(launch_ortho_step(wcol(w, 0), n, nb, wcol(w, cw), n, bs, d_h2, nb, n, nchunk, chunk, 1, part,
+                                c->stream));
+    } else {
+      TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h1, nb, -1.0, 1.0));
+      TRY(ws_dot_dev(w, 0, nb, cw, bs, d_h2));
+      TRY(ws_gemm_dev(w, cw, bs, 0, nb, d_h2, nb, -1.0, 1.0));
+      Timed tm(c, KORE_TIMER_ORTHO);
+      HIP_TRY(launch_tsdot(wcol(w, cw), n, bs, wcol(w, cw), n, bs, n, nchunk, chunk, part, c->stream));
+    }
     // Cholesky QR: V[:, nb:nb+bs] (R2 R1) = W, two passes for a block; a
     // single column is normalised once (its norm is already exact to
     // rounding, so a second pass would only return R2 = 1)
     if (bs == 1) {
-      // Gram partials, then one kernel sums them and scales the column
-      const int64_t n = s->n;
-      int nchunk = (int)std::min<int64_t>(DOT_MAXCHUNK, (n + DOT_ROWS - 1) / DOT_ROWS);
-      if (nchunk < 1) nchunk = 1;
-      const int64_t chunk = (n + nchunk - 1) / nchunk;
-      cx<double>* part = w->part.as<cx<double>>();
+      // one kernel sums the Gram partials and scales the column
       Timed tm(c, KORE_TIMER_ORTHO);
-      HIP_TRY(launch_tsdot(wcol(w, cw), n, 1, wcol(w, cw), n, 1, n, nchunk, chunk, part, c->stream));
       HIP_TRY(launch_normalize1(reinterpret_cast<const double*>(part), nchunk, wcol(w, cw), wcol(w, nb), n, d_R1,
                                 d_Ri1, d_flag, c->stream));
     } else {
-      TRY(ws_dot_dev(w, cw, bs, cw, bs, d_G));
+      {
+        Timed tm(c, KORE_TIMER_ORTHO);
+        HIP_TRY(launch_sum_parts(reinterpret_cast<const double*>(part), nchunk, 2 * bs * bs,
+                                 reinterpret_cast<double*>(d_G), c->stream));
+      }
       HIP_TRY(launch_chol_inv(d_G, bs, bs, d_R1, d_Ri1, d_flag, c->stream));
       TRY(ws_gemm_dev(w, nb, bs, cw, bs, d_Ri1, bs, 1.0, 0.0));
     }

@@ -332,6 +332,56 @@ def test_expand_matches_host_driven(ctx, hydro_small, bs):
     S.close()
 
 
+@pytest.mark.parametrize("bs", [1, 2])
+def test_expand_fused_matches_separate_launches(ctx, bs, monkeypatch):
+    """Gram-Schmidt updates fused with the inner products that follow them
+    (the default) against separate update and product launches
+    (KORE_AMD_ORTHO_FUSED=0), on 70 000 unknowns (row chunks of 69 rows:
+    partial tiles, and empty trailing chunks) with bases of up to 202 columns
+    (the 64-, 128- and 256-column tiles).  The first blocks agree to rounding;
+    over the whole expansion each path's basis is orthonormal and satisfies the
+    Arnoldi relation OP V[:, :m] = V[:, :m+bs] H, with OP applied by separate
+    solves on the host side of the library."""
+    from kore_amd import assembly, backend
+    from kore_amd.params import Params
+    par = Params(m=1, symm=1, Ek=1e-4, N=100, lmax=700)
+    A, B, _ = assembly.assemble(par)
+    sigma = 0.4j
+    say("case: fused vs separate Gram-Schmidt, bs", bs, "sizmat", A.shape[0])
+    S = _system(ctx, par, A, B)
+    S.factor(sigma, 0)
+    m = 200
+    ncol = m + 2 * bs
+    rng = np.random.default_rng(5)
+    X0 = rng.standard_normal((S.n, bs)) + 1j * rng.standard_normal((S.n, bs))
+    X0, _ = np.linalg.qr(X0)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("KORE_AMD_ORTHO_FUSED", fused)
+        ws = backend.Workspace(S, ncol)
+        ws.set(0, X0)
+        H = np.zeros((m + bs, m), dtype=complex, order="F")
+        st, jd, _ = ws.expand(0, m, bs, m + bs, H)
+        assert st == 0 and jd == m
+        V = ws.get(0, m + bs)
+        ws.close()
+        G = V.conj().T @ V
+        orth = np.abs(G - np.eye(m + bs)).max()
+        Y, _ = S.solve(B @ V[:, :m], ir_maxit=0)
+        arn = np.abs(Y - V @ H).max() / np.abs(Y).max()
+        say("  fused", fused, "orthonormality", orth, "Arnoldi relation", arn)
+        assert orth <= 1e-12
+        assert arn <= 1e-10
+        out[fused] = (H, V)
+    (Hf, Vf), (Hs, Vs) = out["1"], out["0"]
+    k = 8 * bs
+    dH = np.abs(Hf[:k + bs, :k] - Hs[:k + bs, :k]).max() / np.abs(Hs[:k + bs, :k]).max()
+    dV = np.abs(Vf[:, :k + bs] - Vs[:, :k + bs]).max() / np.abs(Vs[:, :k + bs]).max()
+    say("  first", k, "columns: H diff", dH, "V diff", dV)
+    assert dH <= 1e-10 and dV <= 1e-10
+    S.close()
+
+
 def test_expand_stop_request(ctx, hydro_small):
     """kore_ws_stop_expand from another thread ends a running expansion after
     its current block: status OK, a whole number of blocks done, and those
