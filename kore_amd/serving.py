@@ -502,16 +502,21 @@ class Server:
 
         for _ in range(len(order)):
             req, fut = ready_q.get()            # arrived, host work done
+            window.release()
             try:
                 par, A0, B0, plan, host = fut.result()
-            finally:
-                window.release()
+            except Exception as e:  # its host work failed: report it, keep serving
+                with lock:
+                    results[req.rid] = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0,
+                                              float("inf"), False, error=repr(e))
+                continue
             # FP64 bytes even on the mixed path: eigensolves and refinement
             # fallbacks hold an FP64 factor
             need = footprint(par, backend.PREC_FP64, req.nev, self.block)
             if need > self.mem_budget:
-                results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
-                                          error="request needs %d bytes, budget %d" % (need, self.mem_budget))
+                with lock:
+                    results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
+                                              error="request needs %d bytes, budget %d" % (need, self.mem_budget))
                 if plan is not None:
                     plan.close()
                 continue

@@ -60,6 +60,27 @@ def test_oversized_request_is_refused_not_crashed():
     srv.close()
 
 
+def test_failed_host_work_is_reported_and_the_rest_served():
+    """A request whose host work raises (a failure forced in its operator
+    assembly) comes back failed with the error; the requests around it are
+    served and meet their checks."""
+    trace = small_trace()[:3]
+    bad = trace[1].rid
+
+    class FailingServer(serving.Server):
+        def _prepare(self, req):
+            if req.rid == bad:
+                raise ValueError("assembly failed for request %d" % req.rid)
+            return super()._prepare(req)
+
+    print("case: failed host work, request", bad, flush=True)
+    srv = FailingServer(slots=2, precision=backend.PREC_FP64)
+    out = srv.run(trace)
+    srv.close()
+    assert not out[1].ok and "assembly failed" in out[1].error
+    assert out[0].ok and out[2].ok, (out[0].error, out[2].error)
+
+
 def test_device_residual_matches_host():
     """kore_residual + kore_sys_norm give the host formula's residuals (forced
     and eigen) for perturbed solutions (residuals well above rounding)."""
