@@ -228,7 +228,7 @@ def cpu_threads():
 
 
 def cpu_block_thomas(par, sigma, nops=None, max_blocks=None):
-    """Stand-in (ii) of the CPU baseline: the oracle's block-Thomas LU
+    """CPU baseline (a stand-in for the sparse direct solver): the oracle's block-Thomas LU
     (numpy/LAPACK zgetrf/zgetrs/zgemm through OpenBLAS, threads = cores) on the
     request's own matrices: factor and one solve timed; an eigensolve is
     estimated as factor + nops solves.  With max_blocks, only the first

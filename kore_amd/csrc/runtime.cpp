@@ -2600,7 +2600,7 @@ static int32_t factor_enqueue(kore_sys* s, double sre, double sim, int32_t preci
 }
 
 // Pivot flags (host copy) -> status; a zero pivot leaves the system unfactored.
-static int32_t factor_verdict(kore_sys* s, const int* hinfo) {
+static int32_t factor_check_info(kore_sys* s, const int* hinfo) {
   s->factored = true;
   for (int k = 0; k < s->nblk; ++k)
     if (hinfo[k] != 0) {
@@ -2654,7 +2654,7 @@ int32_t kore_factor(kore_sys* s, double sre, double sim, int32_t precision, kore
     s->f.reset();
     return fail(KORE_ERR_HIP, "factorisation failed: %s", hipGetErrorString(se));
   }
-  TRY(factor_verdict(s, hinfo.data()));
+  TRY(factor_check_info(s, hinfo.data()));
   if (info) {
     info->levels = s->f->nlev;
     info->nblk = s->nblk;
@@ -2701,7 +2701,7 @@ int32_t kore_factor_check(kore_sys* s) {
   }
   if (!s->info_pending) return s->factored ? KORE_OK : fail(KORE_ERR_STATE, "system is not factored");
   s->info_pending = false;
-  return factor_verdict(s, reinterpret_cast<const int*>(s->info_pin.p));
+  return factor_check_info(s, reinterpret_cast<const int*>(s->info_pin.p));
 }
 
 int32_t kore_factor_release(kore_sys* s) {

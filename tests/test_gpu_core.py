@@ -2,7 +2,7 @@
 
 Tolerances: FP64 path and complex64+FP64-refinement path must reproduce the
 oracle's solutions to relative error <= 1e-10 and eigenvalues to <= 1e-10
-(the bar of the north-star metric); the reference's spin-over golden is
+(the residual bar of the serving metric); the reference's spin-over golden is
 checked at its own rtol 1e-8 (tests/test_spinover.py:21-29 of Kore).
 """
 import sys
