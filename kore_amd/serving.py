@@ -17,8 +17,10 @@ Pipeline:
   its operators (A0, B0), from the operator cache when its structure repeats,
   and the host half of its system's set-up (kore_sys_prepare), so that a slot
   only uploads and computes; at most ``lookahead`` requests are prepared and
-  not yet dispatched, and they are dispatched in the order their host work
-  completes (a full assembly holds up no later request);
+  not yet dispatched, and when a slot frees the first of them (in the order
+  their host work completed, or eigen requests first with eigen_first) that
+  fits the memory left is dispatched (a full assembly holds up no later
+  request);
 * admission: a request enters a device slot when a slot is free and its
   predicted device footprint (library estimate of the factor bytes plus the
   system and Krylov workspace) fits in the HBM budget;
@@ -292,7 +294,7 @@ class Server:
     def __init__(self, device=0, slots=2, mem_budget=200 << 30, precision=backend.PREC_C64, block=1,
                  assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE,
-                 prepare_systems=True, lookahead=None, host_threads=None, blas_threads=1):
+                 prepare_systems=True, lookahead=None, host_threads=None, blas_threads=1, eigen_first=False):
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)
@@ -337,6 +339,9 @@ class Server:
         # saturated capacity); None leaves either setting alone.
         self.host_threads = host_threads
         self.blas_threads = blas_threads
+        # dispatch order among prepared requests when a slot frees: the order
+        # their host work completed in, or eigen requests first
+        self.eigen_first = eigen_first
 
     def close(self):
         self.asm_pool.shutdown(wait=True)
@@ -457,9 +462,10 @@ class Server:
         # a request's host work (operators, system plan) starts at its arrival,
         # on the assembly threads; at most `lookahead` requests are prepared and
         # not yet dispatched (each holds its operators and staged system).
-        # Requests are dispatched in the order their host work completes, so
-        # that a full assembly (a structure not yet cached: 0.5-1.6 s at
-        # N = 400-500) holds up no request that arrived after it.
+        # Prepared requests are dispatched in the order their host work
+        # completed (eigen requests first with eigen_first), so that a full
+        # assembly (a structure not yet cached: 0.5-1.6 s at N = 400-500)
+        # holds up no request that arrived after it.
         ready_q = queue.Queue()
         window = threading.Semaphore(self.lookahead)
 
@@ -500,41 +506,87 @@ class Server:
                 self.log("req %d %s done: latency %.3f s, service %.3f s, residual %.1e" % (
                     req.rid, req.kind, res.latency, res.service, res.residual))
 
-        for _ in range(len(order)):
-            req, fut = ready_q.get()            # arrived, host work done
-            window.release()
-            try:
-                par, A0, B0, plan, host = fut.result()
-            except Exception as e:  # its host work failed: report it, keep serving
-                with lock:
-                    results[req.rid] = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0,
-                                              float("inf"), False, error=repr(e))
+        pending = []    # host work done, not dispatched: (req, par, A0, B0, plan, host, need, key)
+        done = 0        # requests dispatched or reported
+
+        def take_ready(block):
+            """Move requests whose host work has finished into pending; those
+            whose host work failed, or that can never fit the budget, are
+            reported at once."""
+            nonlocal done
+            items = [ready_q.get()] if block else []
+            while True:
+                try:
+                    items.append(ready_q.get_nowait())
+                except queue.Empty:
+                    break
+            for req, fut in items:
+                try:
+                    par, A0, B0, plan, host = fut.result()
+                except Exception as e:  # its host work failed: report it, keep serving
+                    with lock:
+                        results[req.rid] = Result(req.rid, req.kind, time.perf_counter() - (t0 + req.arrival), 0.0,
+                                                  float("inf"), False, error=repr(e))
+                    window.release()
+                    done += 1
+                    continue
+                # FP64 bytes even on the mixed path: eigensolves and refinement
+                # fallbacks hold an FP64 factor
+                need = footprint(par, backend.PREC_FP64, req.nev, self.block)
+                if need > self.mem_budget:
+                    with lock:
+                        results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
+                                                  error="request needs %d bytes, budget %d" % (need, self.mem_budget))
+                    if plan is not None:
+                        plan.close()
+                    window.release()
+                    done += 1
+                    continue
+                # the factor precision of the request's first factorisation
+                key = tuner_key(par, self.precision if req.kind == "forced" else backend.PREC_FP64)
+                pending.append((req, par, A0, B0, plan, host, need, key))
+
+        def rank(item):
+            # completion order of the host work, or eigen requests (the
+            # longest in a slot) before forced ones
+            return (0 if (self.eigen_first and item[0].kind == "eigen") else 1)
+
+        def admissible(item):
+            need, key = item[6], item[7]
+            return (not self.free_slots.empty() and self.mem_in_use + need <= self.mem_budget and not self._alone
+                    and (key in _TUNED or self._inflight == 0))
+
+        while done < len(order):
+            if not pending:
+                take_ready(block=True)
                 continue
-            # FP64 bytes even on the mixed path: eigensolves and refinement
-            # fallbacks hold an FP64 factor
-            need = footprint(par, backend.PREC_FP64, req.nev, self.block)
-            if need > self.mem_budget:
-                with lock:
-                    results[req.rid] = Result(req.rid, req.kind, 0.0, 0.0, float("inf"), False,
-                                              error="request needs %d bytes, budget %d" % (need, self.mem_budget))
-                if plan is not None:
-                    plan.close()
-                continue
-            # the factor precision of the request's first factorisation
-            key = tuner_key(par, self.precision if req.kind == "forced" else backend.PREC_FP64)
+            pick = None
             with self._cv:
-                while (self.free_slots.empty() or self.mem_in_use + need > self.mem_budget or self._alone
-                       or (key not in _TUNED and self._inflight > 0)):
-                    self._cv.wait()
-                if key in _TUNED:
-                    key = None
+                # a request of a class the tuner has not measured goes first,
+                # alone; otherwise the best-ranked request that fits (stable:
+                # ties keep the order the host work completed in)
+                untuned = [it for it in pending if it[7] not in _TUNED]
+                cands = untuned[:1] if untuned else sorted(pending, key=rank)
+                pick = next((it for it in cands if admissible(it)), None)
+                if pick is None:
+                    self._cv.wait(timeout=0.005)    # a slot frees, or more host work completes
                 else:
-                    self._alone = True
-                slot = self.free_slots.get()
-                self.mem_in_use += need
-                self.mem_peak = max(self.mem_peak, self.mem_in_use)
-                self._inflight += 1
-                self.max_inflight = max(self.max_inflight, self._inflight)
+                    req, par, A0, B0, plan, host, need, key = pick
+                    if key in _TUNED:
+                        key = None
+                    else:
+                        self._alone = True
+                    slot = self.free_slots.get()
+                    self.mem_in_use += need
+                    self.mem_peak = max(self.mem_peak, self.mem_in_use)
+                    self._inflight += 1
+                    self.max_inflight = max(self.max_inflight, self._inflight)
+            if pick is None:
+                take_ready(block=False)
+                continue
+            pending.remove(pick)
+            window.release()
+            done += 1
             th = threading.Thread(target=worker, args=(slot, req, par, A0, B0, need, plan, key, host), daemon=True)
             th.start()
             threads.append(th)
