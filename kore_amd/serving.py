@@ -288,6 +288,9 @@ def tuner_key(par, precision):
     return (int(precision), (int(max(sizes)) + 63) // 64)
 
 
+MAX_SLOTS = 4
+
+
 class Server:
     """Continuous-batching request loop over ``slots`` device contexts."""
 
@@ -295,6 +298,11 @@ class Server:
                  assembly_workers=6, tol=1e-12, ir_tol=1e-13, keep_results=False, log=None,
                  gmres_tol=1e-15, gmres_maxit=60, instrument=False, operator_cache=OPERATOR_CACHE,
                  prepare_systems=True, lookahead=None, host_threads=None, blas_threads=1, eigen_first=False):
+        # one HIP stream per slot; the process gets four hardware queues, and a
+        # run with a fifth slot under the serving trace stopped making progress
+        # (cause not found), so the loop takes at most MAX_SLOTS
+        if not 1 <= slots <= MAX_SLOTS:
+            raise ValueError("slots must be 1..%d (one stream per slot, four hardware queues)" % MAX_SLOTS)
         self.device = device
         self.slots = slots
         self.mem_budget = int(mem_budget)

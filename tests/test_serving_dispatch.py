@@ -112,3 +112,12 @@ def test_failed_host_work_is_reported(stand_in):
     assert not out[2].ok and "assembly failed" in out[2].error
     assert all(out[i].ok for i in (0, 1, 3))
     assert sorted(srv.starts) == [0, 1, 3]
+
+
+def test_slot_count_is_bounded():
+    """More slots than the process's hardware queues are refused before any
+    device context is made."""
+    with pytest.raises(ValueError):
+        serving.Server(slots=serving.MAX_SLOTS + 1)
+    with pytest.raises(ValueError):
+        serving.Server(slots=0)
