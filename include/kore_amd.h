@@ -210,6 +210,18 @@ int32_t kore_sys_prepare(const kore_layout* lay, int64_t sizmat,
                          const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
                          int32_t b_is_complex, kore_sys_plan** out);
 int32_t kore_sys_create_prepared(kore_ctx* ctx, kore_sys_plan* plan, kore_sys** out);
+/* kore_sys_prepare for A = A0 + e A1 on one pattern (a_indptr, a_indices):
+ * each value a0 + e a1 (the product, then the sum, each rounded) is formed as
+ * it is staged, so a caller holding A0 and A1 never builds A.  Kore's A is
+ * affine in the Ekman number on a fixed pattern (the viscous force Ek nabla^2 u,
+ * bin/operators.py:136; boundary rows and the Coriolis terms do not carry
+ * Ek), so a serving loop keeps A0, A1 per structure.  Replaces: the same
+ * set-up as kore_sys_prepare, for a request of a structure already seen. */
+int32_t kore_sys_prepare_affine(const kore_layout* lay, int64_t sizmat,
+                                const int32_t* a_indptr, const int32_t* a_indices,
+                                const double* a0_data, const double* a1_data, double e,
+                                const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
+                                int32_t b_is_complex, kore_sys_plan** out);
 void kore_sys_plan_destroy(kore_sys_plan* plan);
 /* New values of A (a_data: interleaved complex, one per entry of the CSR
  * given to kore_sys_create, in that order; same pattern), uploaded into the

@@ -104,6 +104,8 @@ SIGNATURES = [
     ("kore_sys_destroy", None, [_P]),
     ("kore_sys_prepare", _I32, [C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PI32, _PI32, _PD, _I32,
                                 C.POINTER(_P)]),
+    ("kore_sys_prepare_affine", _I32, [C.POINTER(Layout), _I64, _PI32, _PI32, _PD, _PD, _D, _PI32, _PI32, _PD,
+                                       _I32, C.POINTER(_P)]),
     ("kore_sys_create_prepared", _I32, [_P, _P, C.POINTER(_P)]),
     ("kore_sys_plan_destroy", None, [_P]),
     ("kore_sys_update_a", _I32, [_P, _PD]),
@@ -355,6 +357,28 @@ class SystemPlan:
         self.n = n
         self.nnz_a = nnz_a
         self.lay = lay
+
+    @classmethod
+    def affine(cls, lay: Layout, indptr, indices, a0, a1, e, B=None, shape=None):
+        """The plan of A = a0 + e a1 on the CSR pattern (indptr, indices),
+        without building A (kore_sys_prepare_affine)."""
+        n = int(shape[0]) if shape is not None else len(indptr) - 1
+        ip = np.ascontiguousarray(indptr, dtype=np.int32)
+        ix = np.ascontiguousarray(indices, dtype=np.int32)
+        v0 = np.ascontiguousarray(a0, dtype=np.complex128)
+        v1 = np.ascontiguousarray(a1, dtype=np.complex128)
+        if len(ip) != n + 1 or v0.shape != v1.shape or v0.size != ix.size:
+            raise ValueError("A0 and A1 must hold one value per entry of the pattern")
+        _, _, keep, args = _csr_args(ss.csr_matrix((n, n), dtype=np.complex128), B)
+        p = _P()
+        _check(lib().kore_sys_prepare_affine(C.byref(lay), n, ip.ctypes.data_as(_PI32), ix.ctypes.data_as(_PI32),
+                                             _dptr(v0), _dptr(v1), float(e), *args[3:], C.byref(p)))
+        self = cls.__new__(cls)
+        self._p = p
+        self.n = n
+        self.nnz_a = int(ix.size)
+        self.lay = lay
+        return self
 
     def close(self):
         if getattr(self, "_p", None):

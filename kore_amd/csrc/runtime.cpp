@@ -1964,9 +1964,12 @@ struct kore_sys_plan {
   int32_t *pix = nullptr, *pixb = nullptr;
 };
 
+// With a1_data, A's values are a_data + e a1_data (A(E) = A0 + E A1 on one
+// pattern), formed as each entry is staged.
 static int32_t sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr, const int32_t* a_indices,
                            const double* a_data, const int32_t* b_indptr, const int32_t* b_indices,
-                           const double* b_data, int32_t b_is_complex, std::unique_ptr<kore_sys_plan>& P) {
+                           const double* b_data, int32_t b_is_complex, std::unique_ptr<kore_sys_plan>& P,
+                           const double* a1_data = nullptr, double e = 0.0) {
   if (!lay || !a_indptr || !a_indices || !a_data) return fail(KORE_ERR_ARG, "NULL argument");
   P = std::make_unique<kore_sys_plan>();
   P->s = std::make_unique<kore_sys>();
@@ -2129,7 +2132,15 @@ static int32_t sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t
         while (q < row.size() && row[q].col == col) {
           const int64_t k = row[q].src;
           if (k >= 0) {
-            a += zd(a_data[2 * k], a_data[2 * k + 1]);
+            if (a1_data) {
+              // e a1, then + a0, each rounded (no fused multiply-add): the
+              // values numpy's a1 * e + a0 gives
+#pragma clang fp contract(off)
+              const double tr = e * a1_data[2 * k], ti = e * a1_data[2 * k + 1];
+              a += zd(a_data[2 * k] + tr, a_data[2 * k + 1] + ti);
+            } else {
+              a += zd(a_data[2 * k], a_data[2 * k + 1]);
+            }
           } else {
             const int64_t kb = -(k + 1);
             b += b_is_complex ? zd(b_data[2 * kb], b_data[2 * kb + 1]) : zd(b_data[kb], 0.0);
@@ -2411,6 +2422,20 @@ int32_t kore_sys_prepare(const kore_layout* lay, int64_t sizmat, const int32_t* 
   *out = nullptr;
   std::unique_ptr<kore_sys_plan> P;
   TRY(sys_prepare(lay, sizmat, a_indptr, a_indices, a_data, b_indptr, b_indices, b_data, b_is_complex, P));
+  *out = P.release();
+  return KORE_OK;
+}
+
+int32_t kore_sys_prepare_affine(const kore_layout* lay, int64_t sizmat, const int32_t* a_indptr,
+                                const int32_t* a_indices, const double* a0_data, const double* a1_data, double e,
+                                const int32_t* b_indptr, const int32_t* b_indices, const double* b_data,
+                                int32_t b_is_complex, kore_sys_plan** out) {
+  if (!out || !a1_data) return fail(KORE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (!std::isfinite(e)) return fail(KORE_ERR_ARG, "the coefficient of A1 is not finite");
+  std::unique_ptr<kore_sys_plan> P;
+  TRY(sys_prepare(lay, sizmat, a_indptr, a_indices, a0_data, b_indptr, b_indices, b_data, b_is_complex, P, a1_data,
+                  e));
   *out = P.release();
   return KORE_OK;
 }

@@ -182,6 +182,21 @@ def _affine_values(a0, a1, e):
     return out
 
 
+@dataclasses.dataclass
+class AffineA:
+    """A = a0 + e a1 on the CSR pattern (indptr, indices), not formed."""
+    indptr: np.ndarray
+    indices: np.ndarray
+    a0: np.ndarray
+    a1: np.ndarray
+    e: float
+    shape: tuple
+
+    def materialize(self):
+        return ss.csr_matrix((_affine_values(self.a0, self.a1, self.e), self.indices, self.indptr), shape=self.shape,
+                             copy=False)
+
+
 class OperatorCache:
     """Operators of repeated structures: in the hydrodynamic problem of the
     serving trace the Ekman number multiplies only the viscous terms (the
@@ -213,6 +228,12 @@ class OperatorCache:
 
     def assemble(self, req: Request):
         """(par, A, B) of the request, as assemble_request gives them."""
+        par, A, B = self.operators(req)
+        return par, (A.materialize() if isinstance(A, AffineA) else A), B
+
+    def operators(self, req: Request):
+        """(par, A, B) with A, for a structure whose A0 and A1 are known, as
+        an AffineA (A0, A1 and E on the pattern, A itself not formed)."""
         par = req.params()
         k = self.key(req)
         with self._lock:
@@ -222,11 +243,9 @@ class OperatorCache:
             first = self._first.get(k) if aff is None else None
         if aff is not None:
             indptr, indices, a0, a1, B, shape = aff
-            data = _affine_values(a0, a1, req.Ek)
-            A = ss.csr_matrix((data, indices, indptr), shape=shape, copy=False)
             with self._lock:
                 self.hits += 1
-            return par, A, B
+            return par, AffineA(indptr, indices, a0, a1, req.Ek, shape), B
         A, B = assembly.assemble(par, which="unscaled")
         A = ss.csr_matrix(A)
         B = ss.csr_matrix(B)
@@ -362,10 +381,18 @@ class Server:
         slot: its operators (from the cache when the structure repeats) and
         the host half of its system's set-up (kore_sys_prepare)."""
         t0 = time.perf_counter()
-        par, A0, B0 = (self.operator_cache.assemble(req) if self.operator_cache is not None
+        par, A0, B0 = (self.operator_cache.operators(req) if self.operator_cache is not None
                        else assemble_request(req))
         t1 = time.perf_counter()
-        plan = backend.SystemPlan(backend.layout_from_params(par), A0, B0) if self.prepare_systems else None
+        lay = backend.layout_from_params(par)
+        plan = None
+        if self.prepare_systems and isinstance(A0, AffineA):
+            # a cached structure: the plan forms A0 + E A1 as it stages the values
+            plan = backend.SystemPlan.affine(lay, A0.indptr, A0.indices, A0.a0, A0.a1, A0.e, B0, A0.shape)
+        elif self.prepare_systems:
+            plan = backend.SystemPlan(lay, A0, B0)
+        elif isinstance(A0, AffineA):
+            A0 = A0.materialize()
         return par, A0, B0, plan, (t1 - t0, time.perf_counter() - t1)
 
     def _execute(self, slot, req: Request, par, A0, B0, t0, plan=None) -> Result:

@@ -81,6 +81,34 @@ def test_failed_host_work_is_reported_and_the_rest_served():
     assert out[0].ok and out[2].ok, (out[0].error, out[2].error)
 
 
+def test_affine_plan_matches_formed_operator():
+    """A system staged from a cached structure's A0, A1 and E
+    (kore_sys_prepare_affine) is the system of the formed A = A0 + E A1: the
+    same FP64 and complex64 + GMRES solves, bit for bit."""
+    r = next(q for q in small_trace() if q.kind == "forced")
+    cache = serving.OperatorCache()
+    cache.operators(dataclasses.replace(r, Ek=r.Ek * 1.5))
+    cache.operators(dataclasses.replace(r, Ek=r.Ek * 0.7))
+    par, A, B = cache.operators(r)
+    assert isinstance(A, serving.AffineA)
+    print("case: affine plan vs formed operator, N", r.N, "m", r.m, flush=True)
+    lay = backend.layout_from_params(par)
+    ctx = backend.Context(0)
+    S1 = backend.System(ctx, lay, None, None,
+                        plan=backend.SystemPlan.affine(lay, A.indptr, A.indices, A.a0, A.a1, A.e, B, A.shape))
+    S2 = backend.System(ctx, lay, A.materialize(), B)
+    b = serving.rhs_for(r, S1.n)
+    for prec in (backend.PREC_FP64, backend.PREC_C64):
+        S1.factor(r.sigma, prec)
+        S2.factor(r.sigma, prec)
+        x1, _ = S1.solve_gmres(b, tol=1e-15, maxit=60)
+        x2, _ = S2.solve_gmres(b, tol=1e-15, maxit=60)
+        assert np.array_equal(x1, x2), prec
+    S1.close()
+    S2.close()
+    ctx.close()
+
+
 def test_device_residual_matches_host():
     """kore_residual + kore_sys_norm give the host formula's residuals (forced
     and eigen) for perturbed solutions (residuals well above rounding)."""

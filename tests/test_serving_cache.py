@@ -53,3 +53,19 @@ def test_planted_nonaffine_change_is_caught():
     from kore_amd import assembly
     A_other, _ = assembly.assemble(dataclasses.replace(par, ricb=0.4), which="unscaled")
     assert abs(A - ss.csr_matrix(A_other)).max() > 1e-6 * abs(A).max()
+
+
+def test_operators_of_a_cached_structure_are_left_affine():
+    """operators() gives a cached structure's A as A0, A1 and E (the server
+    stages A0 + E A1 without forming A); formed, it is exactly what
+    assemble() returns for the same request."""
+    cache = serving.OperatorCache()
+    cache.operators(_req(0, 3e-5))
+    cache.operators(_req(1, 7e-6))
+    _, A, B = cache.operators(_req(2, 1.3e-5))
+    assert isinstance(A, serving.AffineA) and A.e == 1.3e-5
+    F = A.materialize()
+    _, G, _ = cache.assemble(_req(3, 1.3e-5))
+    assert np.array_equal(F.indptr, G.indptr) and np.array_equal(F.indices, G.indices)
+    assert np.array_equal(F.data, G.data)
+    assert np.array_equal(F.data, A.a1 * A.e + A.a0)
