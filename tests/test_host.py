@@ -58,6 +58,20 @@ def test_factor_bytes_prediction():
     assert 0.45 < c64 / big < 0.55
 
 
+def test_affine_plan_argument_checks():
+    """kore_sys_prepare_affine refuses a non-finite coefficient and A0, A1
+    arrays that do not match the pattern, before any host or device work."""
+    import scipy.sparse as ss
+    par = Params(m=1, symm=1, Ek=1e-3, N=12, lmax=8)
+    A, B = assembly.assemble(par, which="unscaled")
+    A = ss.csr_matrix(A)
+    lay = backend.layout_from_params(par)
+    with pytest.raises(backend.KoreError, match="not finite"):
+        backend.SystemPlan.affine(lay, A.indptr, A.indices, A.data, A.data, float("nan"), B, A.shape)
+    with pytest.raises(ValueError):
+        backend.SystemPlan.affine(lay, A.indptr, A.indices, A.data, A.data[:-1], 1.0, B, A.shape)
+
+
 def test_host_thread_setting():
     """kore_set_host_threads: a positive count is used as given, 0 restores
     the default (the process's CPU share, at most 16, unless
