@@ -243,7 +243,7 @@ def test_solve_many_rhs(ctx, hydro_small):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("block", [1, 4])
+@pytest.mark.parametrize("block", [1, 2, 4])
 @pytest.mark.parametrize("driver", ["native", "python"])
 def test_eigs_hydro_small(ctx, hydro_small, precision, block, driver):
     from kore_amd import eigs
